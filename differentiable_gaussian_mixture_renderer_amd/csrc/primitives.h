@@ -1,0 +1,37 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Device-wide scan and stable LSD radix sort built from 64-lane wavefront
+// ballot / prefix primitives (replacing cub::DeviceScan::InclusiveSum and
+// cub::DeviceRadixSort::SortPairs at splat_forward.cu:127-137, 197-210).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace dgmr {
+
+// --- exclusive scan of uint32 ---------------------------------------------
+// scratch bytes needed to scan n items
+size_t scan_scratch_bytes(int64_t n);
+// out[i] = sum(in[0..i-1]); *total (device) = sum(in) when total != nullptr.
+// inclusive == true gives out[i] = sum(in[0..i]).  in may alias out.
+hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n, bool inclusive, void* scratch,
+    hipStream_t stream);
+
+// --- stable LSD radix sort of (uint32 key, uint32 value) pairs -----------
+// Sorts bits [begin_bit, end_bit) in 8-bit digits.  Pass p reads buffer
+// (p % 2 == 0 ? A : B) and writes the other one; the sorted result ends in A
+// when the number of passes is even, in B when it is odd.  n may be given on
+// the device (n_dev != nullptr, capacity n_max) so no host sync is needed.
+struct SortBuffers {
+    uint32_t* keys_a;
+    uint32_t* vals_a;
+    uint32_t* keys_b;
+    uint32_t* vals_b;
+};
+int radix_passes(int begin_bit, int end_bit);
+size_t radix_scratch_bytes(int64_t n_max);
+hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit,
+    void* scratch, hipStream_t stream);
+
+} // namespace dgmr

@@ -1,0 +1,262 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Scan and radix sort for gfx950 (wave64).  See primitives.h.
+#include "primitives.h"
+
+#include "device_math.h"
+
+namespace dgmr {
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;                       // per thread
+constexpr int kScanTile = kScanThreads * kScanItems; // 2048 items per block
+
+// inclusive scan of one value per thread across a 256-thread block
+__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s_waves, uint32_t* block_total)
+{
+    const unsigned lane = lane_id();
+    const unsigned wave = threadIdx.x / 64;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= unsigned(d))
+            v += o;
+    }
+    if (lane == 63)
+        s_waves[wave] = v;
+    __syncthreads();
+    uint32_t prefix = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        const uint32_t c = s_waves[w];
+        if (unsigned(w) < wave)
+            prefix += c;
+        total += c;
+    }
+    __syncthreads();
+    *block_total = total;
+    return v + prefix;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n, uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t s_waves[kScanThreads / 64];
+    const int64_t base = int64_t(blockIdx.x) * kScanTile;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const int64_t i = base + int64_t(j) * kScanThreads + threadIdx.x;
+        if (i < n)
+            acc += in[i];
+    }
+    uint32_t total;
+    block_inclusive_scan(acc, s_waves, &total);
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = total;
+}
+
+// scans one tile of 2048 items with a carried-in offset from `offsets` (or 0)
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile(const uint32_t* in, uint32_t* out, int64_t n, const uint32_t* __restrict__ offsets,
+    uint32_t* total_out, int inclusive)
+{
+    __shared__ uint32_t s_waves[kScanThreads / 64];
+    const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = (base + j < n) ? in[base + j] : 0u;
+        acc += v[j];
+    }
+    uint32_t total;
+    const uint32_t incl = block_inclusive_scan(acc, s_waves, &total);
+    uint32_t run = incl - acc + (offsets ? offsets[blockIdx.x] : 0u);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint32_t x = v[j];
+        if (base + j < n)
+            out[base + j] = inclusive ? run + x : run;
+        run += x;
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        *total_out = (offsets ? offsets[blockIdx.x] : 0u) + total;
+}
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+} // namespace
+
+size_t scan_scratch_bytes(int64_t n)
+{
+    size_t bytes = 0;
+    int64_t m = ceil_div(n, kScanTile);
+    while (m > 1) {
+        bytes += size_t(m) * sizeof(uint32_t) + 256;
+        m = ceil_div(m, kScanTile);
+    }
+    return bytes + 256;
+}
+
+hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n, bool inclusive, void* scratch, hipStream_t stream)
+{
+    if (n <= 0) {
+        if (total)
+            return hipMemsetAsync(total, 0, sizeof(uint32_t), stream);
+        return hipSuccess;
+    }
+    const int64_t blocks = ceil_div(n, kScanTile);
+    if (blocks == 1) {
+        k_scan_tile<<<1, kScanThreads, 0, stream>>>(in, out, n, nullptr, total, inclusive ? 1 : 0);
+        return hipGetLastError();
+    }
+    // reduce tiles, scan the tile sums recursively (exclusive), then rescan tiles with offsets
+    uint32_t* sums = static_cast<uint32_t*>(scratch);
+    void* rest = static_cast<char*>(scratch) + ((size_t(blocks) * sizeof(uint32_t) + 255) / 256) * 256;
+    k_scan_reduce<<<unsigned(blocks), kScanThreads, 0, stream>>>(in, n, sums);
+    hipError_t e = scan_u32(sums, sums, nullptr, blocks, false, rest, stream);
+    if (e != hipSuccess)
+        return e;
+    k_scan_tile<<<unsigned(blocks), kScanThreads, 0, stream>>>(in, out, n, sums, total, inclusive ? 1 : 0);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ radix sort
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;                           // per thread
+constexpr int kSortTile = kSortThreads * kSortItems;     // 4096 items per block
+constexpr int kRadix = 256;
+
+__device__ __forceinline__ int64_t sort_count(int64_t n_max, const uint32_t* n_dev)
+{
+    return n_dev ? int64_t(*n_dev) : n_max;
+}
+
+// per-block digit histogram, stored digit-major: hist[digit * n_blocks + block]
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n_max, const uint32_t* n_dev,
+    int shift, int n_blocks, uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t s_hist[4][kRadix];
+    const int64_t n = sort_count(n_max, n_dev);
+    const int64_t base = int64_t(blockIdx.x) * kSortTile;
+    for (int i = threadIdx.x; i < 4 * kRadix; i += kSortThreads)
+        (&s_hist[0][0])[i] = 0;
+    __syncthreads();
+    if (base < n) {
+        const unsigned wave = threadIdx.x / 64;
+#pragma unroll 4
+        for (int j = 0; j < kSortItems; ++j) {
+            const int64_t i = base + int64_t(j) * kSortThreads + threadIdx.x;
+            if (i < n)
+                atomicAdd(&s_hist[wave][(keys[i] >> shift) & 0xffu], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t c = s_hist[0][threadIdx.x] + s_hist[1][threadIdx.x] + s_hist[2][threadIdx.x] + s_hist[3][threadIdx.x];
+    hist[int64_t(threadIdx.x) * n_blocks + blockIdx.x] = c;
+}
+
+// stable scatter: chunks of 256 consecutive items are ranked with wave ballots
+// (lanes holding the same digit), waves are ordered through LDS, and chunks are
+// processed in order, so equal digits keep their input order.
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int n_blocks,
+    const uint32_t* __restrict__ offsets)
+{
+    __shared__ uint32_t s_base[kRadix];       // global offset of each digit for this block + running count
+    __shared__ uint32_t s_wave_cnt[4][kRadix];
+    __shared__ uint32_t s_wave_off[4][kRadix];
+    const int64_t n = sort_count(n_max, n_dev);
+    const int64_t base = int64_t(blockIdx.x) * kSortTile;
+    if (base >= n)
+        return;
+    s_base[threadIdx.x] = offsets[int64_t(threadIdx.x) * n_blocks + blockIdx.x];
+    for (int w = 0; w < 4; ++w)
+        s_wave_cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned lane = lane_id();
+    const unsigned wave = threadIdx.x / 64;
+    const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    for (int j = 0; j < kSortItems; ++j) {
+        const int64_t i = base + int64_t(j) * kSortThreads + threadIdx.x;
+        if (base + int64_t(j) * kSortThreads >= n)
+            break; // uniform across the block
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t val = valid ? vals_in[i] : 0u;
+        const uint32_t digit = (key >> shift) & 0xffu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (digit >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = __popcll(peers & lt_mask);
+        const bool leader = valid && (peers & lt_mask) == 0;
+        if (leader)
+            s_wave_cnt[wave][digit] = __popcll(peers);
+        __syncthreads();
+        {
+            uint32_t run = s_base[threadIdx.x];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const uint32_t c = s_wave_cnt[w][threadIdx.x];
+                s_wave_off[w][threadIdx.x] = run;
+                run += c;
+                s_wave_cnt[w][threadIdx.x] = 0;
+            }
+            s_base[threadIdx.x] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = s_wave_off[wave][digit] + rank;
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+    }
+}
+
+} // namespace
+
+int radix_passes(int begin_bit, int end_bit)
+{
+    return end_bit <= begin_bit ? 0 : (end_bit - begin_bit + 7) / 8;
+}
+
+size_t radix_scratch_bytes(int64_t n_max)
+{
+    const int64_t blocks = std::max<int64_t>(1, ceil_div(n_max, kSortTile));
+    const size_t hist = size_t(blocks) * kRadix * sizeof(uint32_t);
+    return ((hist + 255) / 256) * 256 + scan_scratch_bytes(blocks * kRadix);
+}
+
+hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit, void* scratch,
+    hipStream_t stream)
+{
+    const int passes = radix_passes(begin_bit, end_bit);
+    if (n_max <= 0 || passes == 0)
+        return hipSuccess;
+    const int blocks = int(std::max<int64_t>(1, ceil_div(n_max, kSortTile)));
+    uint32_t* hist = static_cast<uint32_t*>(scratch);
+    void* scan_scratch = static_cast<char*>(scratch) + ((size_t(blocks) * kRadix * sizeof(uint32_t) + 255) / 256) * 256;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = begin_bit + 8 * p;
+        const uint32_t* kin = (p % 2 == 0) ? buf.keys_a : buf.keys_b;
+        const uint32_t* vin = (p % 2 == 0) ? buf.vals_a : buf.vals_b;
+        uint32_t* kout = (p % 2 == 0) ? buf.keys_b : buf.keys_a;
+        uint32_t* vout = (p % 2 == 0) ? buf.vals_b : buf.vals_a;
+        k_radix_hist<<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, blocks, hist);
+        hipError_t e = scan_u32(hist, hist, nullptr, int64_t(blocks) * kRadix, false, scan_scratch, stream);
+        if (e != hipSuccess)
+            return e;
+        k_radix_scatter<<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, blocks, hist);
+        e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+} // namespace dgmr

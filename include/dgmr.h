@@ -1,0 +1,185 @@
+/*
+ * dgmr -- MI355X (gfx950) rendering hot path of the differentiable Gaussian
+ * mixture renderer: the tile splatter and the volumetric ray marcher, forward
+ * and backward, as a C ABI.
+ *
+ * Every pointer in a descriptor is a device pointer unless stated otherwise;
+ * camera matrices are host values.  No entry point allocates device memory:
+ * outputs, the per-frame cache and scratch come from the caller.  All work is
+ * enqueued on the given stream; entry points that must know the number of
+ * (tile, Gaussian) pairs read one 4-byte counter back from the device (the
+ * reference does the same with .item(), splat_forward.cu:141).
+ *
+ * Layouts follow the reference's torch tensors (python_bindings/vol_marcher.cu):
+ *   means [n,3], scales [n,3], rotations [n,4] (w,x,y,z), opacity [n],
+ *   sh [n,K,3] (K <= 16), matrices [4,4] where tensor row i is glm column i,
+ *   images planar [3,H,W].
+ */
+#ifndef DGMR_H
+#define DGMR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* dgmr_stream; /* hipStream_t; NULL = the null stream */
+
+enum dgmr_status {
+    DGMR_OK = 0,
+    DGMR_ERR_ARGUMENT = 1,   /* bad size, null pointer, unsupported option */
+    DGMR_ERR_CAPACITY = 2,   /* pair buffers too small; stats.n_pairs holds the size needed */
+    DGMR_ERR_WORKSPACE = 3,  /* workspace too small; query dgmr_*_workspace_size */
+    DGMR_ERR_DEVICE = 4      /* a HIP call failed; dgmr_last_error() has the text */
+};
+
+enum dgmr_dtype { DGMR_F32 = 0, DGMR_F64 = 1 };
+
+/* Gaussian parameters as the caller holds them (AoS, contiguous). */
+typedef struct dgmr_gaussians {
+    int64_t n;
+    int32_t dtype;            /* dgmr_dtype; the splatter accepts DGMR_F32 only */
+    const void* means;        /* [n,3] */
+    const void* scales;       /* [n,3] */
+    const void* rotations;    /* [n,4] w,x,y,z */
+    const void* opacity;      /* [n]   */
+    const void* sh;           /* [n,sh_coeffs,3] */
+    int32_t sh_coeffs;        /* K: coefficients stored per Gaussian, 1..16 */
+    int32_t sh_degree;        /* active degree 0..3 */
+    double scale_multiplier;  /* ForwardData::cov_scale_multiplier (1 in the reference binding) */
+} dgmr_gaussians;
+
+/* Pinhole camera, host values.  Replaces the .cpu() copies of
+ * python_bindings/vol_marcher.cu:16-40. */
+typedef struct dgmr_camera {
+    double view[16];          /* world_view_transform, tensor layout */
+    double proj[16];          /* full_proj_transform, tensor layout */
+    double position[3];
+    double background[3];
+    double tan_fovx, tan_fovy;
+    int32_t width, height;
+} dgmr_camera;
+
+/* Tile shape and LDS batching.  The reference fixes 16x16 tiles
+ * (constants.h:45-46) and 256-Gaussian batches (splat_forward.cu:268-270). */
+typedef struct dgmr_tiling {
+    int32_t tile_w, tile_h;   /* 8x8, 16x16 or 32x8 */
+    int32_t batch;            /* Gaussians staged in LDS per round: 128, 256 or 512 */
+} dgmr_tiling;
+
+typedef struct dgmr_stats {
+    int64_t n_pairs;          /* (tile, Gaussian) pairs R of this frame */
+    int64_t n_visible;        /* Gaussians with at least one tile */
+} dgmr_stats;
+
+/* ------------------------------------------------------------ splatter
+ * Per-Gaussian cache of the splatter forward (splat_forward.cu:45-64). */
+typedef struct dgmr_splat_geometry {
+    int32_t* rects;           /* [n,2] pixel extent (ceil(3 sigma)) */
+    uint32_t* tiles_touched;  /* [n]   0 = culled */
+    float* xy;                /* [n,2] screen centroid */
+    float* conic_opacity;     /* [n,4] inverse 2D covariance (xx,xy,yy) + screen opacity */
+    float* rgb;               /* [n,3] */
+    uint8_t* rgb_clamped;     /* [n,3] 1 where the SH colour was >= 0 */
+    float* depths;            /* [n]   */
+} dgmr_splat_geometry;
+
+/* Binning cache shared by both renderers (splat_forward.cu:120-240). */
+typedef struct dgmr_binning {
+    uint32_t* point_offsets;  /* [n] inclusive scan of tiles_touched in Gaussian order */
+    uint32_t* point_list;     /* [pair_capacity] Gaussian ids sorted by (tile, depth) */
+    uint32_t* ranges;         /* [tiles,2] [first,last) into point_list, 0 for empty tiles */
+    int64_t pair_capacity;
+} dgmr_binning;
+
+/* Bytes of scratch the splatter / marcher entry points need. */
+int dgmr_workspace_size(int64_t n, int32_t width, int32_t height, const dgmr_tiling* tiling, int64_t pair_capacity,
+    size_t* bytes);
+
+/* Splatter forward: replaces dgmr::splat_forward (splat_forward.cu:30) /
+ * sibr_interfaces::splat (sibr_interfaces.cpp:26-29).  Writes framebuffer
+ * [3,H,W] and final transmittance [H,W]; fills geometry and binning. */
+int dgmr_splat_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dgmr_tiling* tiling,
+    dgmr_splat_geometry* geom, dgmr_binning* bins, float* framebuffer, float* final_transmittance,
+    void* workspace, size_t workspace_bytes, dgmr_stats* stats, dgmr_stream stream);
+
+/* Splatter backward (no counterpart in the reference, which ships no splat
+ * backward; its adjoints are composed from grad/math.h and
+ * unittests/grad_inria_splatter_render.cu:67-156).  grad_* outputs are
+ * overwritten: means [n,3], scales [n,3], rotations [n,4], opacity [n],
+ * sh [n,K,3].  screen_grads is [n,9] scratch (xy 2, conic 3, opacity 1, rgb 3). */
+int dgmr_splat_backward(const dgmr_gaussians* g, const dgmr_camera* cam, const dgmr_tiling* tiling,
+    const dgmr_splat_geometry* geom, const dgmr_binning* bins, int64_t n_pairs, const float* framebuffer,
+    const float* final_transmittance, const float* grad_framebuffer, float* screen_grads,
+    float* grad_means, float* grad_scales, float* grad_rotations, float* grad_opacity, float* grad_sh,
+    dgmr_stream stream);
+
+/* ------------------------------------------------------------ marcher
+ * Per-Gaussian cache of the marcher forward (vol_marcher_forward.h:28-41);
+ * arrays of the scene dtype unless stated. */
+typedef struct dgmr_march_geometry {
+    int32_t* rects;           /* [n,2] */
+    void* rgb;                /* [n,3] */
+    uint8_t* rgb_clamped;     /* [n,3] */
+    void* depths;             /* [n] */
+    void* xy;                 /* [n,2] */
+    void* inverse_filtered_cov3d; /* [n,6] xx,xy,xz,yy,yz,zz */
+    void* filtered_masses;    /* [n] */
+    uint32_t* tiles_touched;  /* [n] */
+} dgmr_march_geometry;
+
+typedef struct dgmr_march_options {
+    int32_t max_large_steps;  /* 0 = unbounded (the reference); >0 caps large steps per ray */
+    int32_t n_frozen;         /* >0: fixed sample sections for gradient checks
+                                 (DGMR_TORCH_GRAD_CHECK_CONST_SAMPLES, vol_marcher.h:25) */
+    double frozen[8][3];      /* start, end, step of each frozen section */
+} dgmr_march_options;
+
+/* Marcher forward: replaces dgmr::vol_marcher::forward<T>
+ * (vol_marcher_forward.cu:42-43).  framebuffer [3,H,W], remaining
+ * transparency [H,W], both of the scene dtype. */
+int dgmr_march_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dgmr_tiling* tiling,
+    const dgmr_march_options* opt, dgmr_march_geometry* geom, dgmr_binning* bins, void* framebuffer,
+    void* remaining_transparency, void* workspace, size_t workspace_bytes, dgmr_stats* stats, dgmr_stream stream);
+
+/* Marcher backward: replaces dgmr::vol_marcher::backward<T>
+ * (vol_marcher_backward.cu:53-54).  render_grads is [n,13] scratch of the
+ * scene dtype (mass 1, rgb 3, means 3, inverse covariance 6).  grad_sh is
+ * [n,K,3]; SH coefficients beyond K read as 1 in this pass, as the reference
+ * binding pads them (python_bindings/vol_marcher.cu:177). */
+int dgmr_march_backward(const dgmr_gaussians* g, const dgmr_camera* cam, const dgmr_tiling* tiling,
+    const dgmr_march_options* opt, const dgmr_march_geometry* geom, const dgmr_binning* bins, int64_t n_pairs,
+    const void* framebuffer, const void* remaining_transparency, const void* grad_framebuffer, void* render_grads,
+    void* grad_means, void* grad_scales, void* grad_rotations, void* grad_opacity, void* grad_sh, dgmr_stream stream);
+
+/* ------------------------------------------------------------ batched views
+ * Many cameras of one scene in one persistent launch per stage (cameras of
+ * uneven resolution).  cams[i] renders into framebuffers[i] / transmittances[i]. */
+typedef struct dgmr_batch_view {
+    dgmr_camera cam;
+    float* framebuffer;           /* [3,H,W] */
+    float* final_transmittance;   /* [H,W]   */
+    const float* grad_framebuffer;/* [3,H,W] for the backward, else NULL */
+} dgmr_batch_view;
+
+int dgmr_batch_workspace_size(int64_t n, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,
+    int64_t pair_capacity, size_t* bytes);
+
+/* Splat forward (and, when every view has grad_framebuffer, backward) of all
+ * views.  Gradients of all views are summed into grad_* (overwritten). */
+int dgmr_splat_batch(const dgmr_gaussians* g, dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,
+    void* workspace, size_t workspace_bytes, int64_t pair_capacity,
+    float* grad_means, float* grad_scales, float* grad_rotations, float* grad_opacity, float* grad_sh,
+    dgmr_stats* stats, dgmr_stream stream);
+
+/* ------------------------------------------------------------ misc */
+const char* dgmr_last_error(void);
+const char* dgmr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGMR_H */
