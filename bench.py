@@ -1,0 +1,186 @@
+"""Benchmark: tile splat forward+backward at 1M Gaussians, 1920x1080.
+
+A step is one training iteration of the hot path on one frame: parameter
+activations (exp of log-scales, quaternion normalisation, sigmoid of opacity
+logits), splat forward, L1 loss against a seeded uniform-noise target, and the
+backward to every parameter (means, log-scales, quaternions, opacity logits,
+SH).  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (see README of the measurement contract in DESIGN.md).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fwd+bwd Mpixel/s @1M Gaussians 1080p; served p50/p99 latency vs plain path"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--tile", default="16x16")
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
+    ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame in the host sample (0: automatic)")
+    return ap.parse_args()
+
+
+def stage_times(lib):
+    from differentiable_gaussian_mixture_renderer_amd import _native as N
+    arr = (N.StageTime * 64)()
+    count = ctypes.c_int32(0)
+    lib.dgmr_profile_read(arr, 64, ctypes.byref(count))
+    return {arr[i].name.decode(): (arr[i].total_ms, int(arr[i].launches), arr[i].bytes) for i in range(count.value)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is described in DESIGN.md, not run)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from differentiable_gaussian_mixture_renderer_amd import _native as N
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    from differentiable_gaussian_mixture_renderer_amd.python_bindings import splatter
+
+    tile = tuple(int(v) for v in args.tile.split("x"))
+    W, H = args.width, args.height
+    scene, cam = scenes.config2(n=args.n, width=W, height=H)
+    dev = torch.device("cuda:0")
+    lib = N.load()
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    means = t(scene.means).requires_grad_()
+    log_scales = t(scene.log_scales).requires_grad_()
+    quats = t(scene.rotations).requires_grad_()
+    logits = t(scene.opacity_logits).requires_grad_()
+    sh = t(scene.sh).requires_grad_()
+    params = [means, log_scales, quats, logits, sh]
+    target = torch.tensor(scenes.Stream(2, 9).uniform(3 * H * W).reshape(3, H, W).astype(np.float32), device=dev)
+    # camera tensors stay on the host: no device->host copies inside the step
+    view = torch.tensor(cam.world_view_transform)
+    proj = torch.tensor(cam.full_proj_transform)
+    campos = torch.tensor(cam.camera_center)
+    bg = torch.zeros(3)
+
+    def render(sc, op, rot):
+        return splatter.Splatter.apply(sh, op, means, sc, rot, view, proj, campos, bg, W, H, cam.tan_fovx, cam.tan_fovy, 3)
+
+    def step():
+        img = render(torch.exp(log_scales), torch.sigmoid(logits), torch.nn.functional.normalize(quats, dim=-1))
+        loss = (img - target).abs().mean()
+        return torch.autograd.grad(loss, params)
+
+    if tile != (16, 16) or args.batch != 256:
+        splatter.DEFAULT_TILING = (tile, args.batch)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    mpx = W * H * args.steps / elapsed / 1e6
+
+    # per-stage durations from hipEvents on the library's stream, over extra steps
+    prof_steps = max(3, min(10, args.steps))
+    lib.dgmr_profile_enable(1)
+    for _ in range(prof_steps):
+        step()
+    torch.cuda.synchronize()
+    lib.dgmr_profile_enable(0)
+    stages = stage_times(lib)
+
+    # pair count of this frame (for the byte model)
+    with torch.no_grad():
+        cache = splatter.cpp_binding.forward(sh, torch.sigmoid(logits), means, torch.exp(log_scales),
+                                             torch.nn.functional.normalize(quats, dim=-1), view, proj, campos, bg, W, H, cam.tan_fovx,
+                                             cam.tan_fovy, 3)
+    R = int(cache[-1][0])
+    n_visible = int((cache[7] > 0).sum())
+    del cache
+
+    dominant = max(stages.items(), key=lambda kv: kv[1][0])
+    d_name, (d_ms, d_launches, d_bytes) = dominant
+    d_avg_ms = d_ms / max(d_launches, 1)
+    achieved = (d_bytes / max(d_launches, 1)) / (d_avg_ms * 1e-3) / 1e9
+    p_in = 44 + 12 * 16
+    frame_bytes = args.n * (3 * p_in + 160) + 124 * R + 44 * W * H + 8 * ((W + 15) // 16) * ((H + 15) // 16)
+    frame_hbm_frac = frame_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
+
+    traffic = None
+    tr_path = os.path.join(ROOT, "profiles", "traffic_splat_cfg2.json")
+    if os.path.exists(tr_path):
+        with open(tr_path) as f:
+            tr = json.load(f)
+        traffic = tr.get(d_name)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene, cam, args.cpu_rows)
+
+    line = {
+        "metric": METRIC,
+        "value": round(mpx, 3),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 scene and L1 target, BASELINE config 2 distributions)",
+        "config": {"workload": f"tile splat fwd+bwd, {args.n} Gaussians SH3, {W}x{H}, L1 loss", "tile": f"{tile[0]}x{tile[1]}",
+                   "lds_batch": args.batch, "pairs": R, "visible": n_visible},
+        "roofline": {"bound": "hbm", "kernel": d_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "avg_launch_ms": round(d_avg_ms, 4), "algorithmic_bytes_per_launch": d_bytes / max(d_launches, 1)},
+        "frame_hbm_fraction": round(frame_hbm_frac, 4),
+        "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(scene, cam, rows: int):
+    """The float restatement of the same algorithm (oracle/) timed on the host,
+    single thread, on a band of image rows of the same frame (all Gaussians are
+    preprocessed and binned; only the band's pixels are blended)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    H, W = cam.height, cam.width
+    rows = rows or 48
+    y0 = (H - rows) // 2
+    h = O.SplatOracle(O.scene_arrays(scene), cam, dbl=False)
+    grad = np.full((3, H, W), 1.0 / (3 * H * W), np.float32)
+    secs = h.time_fwd_bwd(np.zeros(3, np.float32), grad, crop=(0, y0, W, y0 + rows))
+    px = W * rows
+    return {"value": round(px / secs / 1e6, 4), "unit": "Mpixel/s", "cores": 1, "kind": "port",
+            "sample": f"float restatement, full preprocess+binning of {scene.n} Gaussians, blend fwd+bwd of a {W}x{rows} band "
+                      f"(rows {y0}-{y0 + rows}), {secs:.1f} s on the host, 1 thread"}
+
+
+if __name__ == "__main__":
+    main()

@@ -110,6 +110,10 @@ class BatchView(ctypes.Structure):
     ]
 
 
+class StageTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("total_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("bytes", ctypes.c_double)]
+
+
 _P = ctypes.POINTER
 # exported symbol -> (restype, argtypes); this list mirrors include/dgmr.h
 SIGNATURES = {
@@ -126,8 +130,13 @@ SIGNATURES = {
                                            c_void_p, c_void_p, c_void_p]),
     "dgmr_batch_workspace_size": (ctypes.c_int, [_P(Gaussians), _P(BatchView), ctypes.c_int32, _P(Tiling), ctypes.c_int64,
                                                  _P(ctypes.c_size_t)]),
-    "dgmr_splat_batch": (ctypes.c_int, [_P(Gaussians), _P(BatchView), ctypes.c_int32, _P(Tiling), c_void_p, ctypes.c_size_t,
-                                        ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, _P(Stats), c_void_p]),
+    "dgmr_splat_batch_forward": (ctypes.c_int, [_P(Gaussians), _P(BatchView), ctypes.c_int32, _P(Tiling), c_void_p, ctypes.c_size_t,
+                                                ctypes.c_int64, _P(Stats), c_void_p]),
+    "dgmr_splat_batch_backward": (ctypes.c_int, [_P(Gaussians), _P(BatchView), ctypes.c_int32, _P(Tiling), c_void_p, ctypes.c_size_t,
+                                                 ctypes.c_int64, ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                 c_void_p]),
+    "dgmr_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "dgmr_profile_read": (ctypes.c_int, [_P(StageTime), ctypes.c_int32, _P(ctypes.c_int32)]),
     "dgmr_last_error": (ctypes.c_char_p, []),
     "dgmr_version": (ctypes.c_char_p, []),
 }

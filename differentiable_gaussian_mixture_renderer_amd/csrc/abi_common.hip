@@ -4,6 +4,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "host_common.h"
 #include "primitives.h"
@@ -102,6 +103,56 @@ DevCamera<S> make_dev_camera(const dgmr_camera& c, const dgmr_tiling& t)
 }
 
 template DevCamera<float> make_dev_camera<float>(const dgmr_camera&, const dgmr_tiling&);
+
+// ---------------------------------------------------------------- stage timing
+namespace {
+struct Record {
+    const char* name;
+    hipEvent_t start, stop;
+    double bytes;
+};
+struct Profiler {
+    bool on = false;
+    std::vector<Record> records;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get()
+    {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess)
+            return nullptr;
+        return e;
+    }
+};
+std::mutex g_prof_mutex;
+Profiler g_prof;
+} // namespace
+
+StageScope::StageScope(const char* name, hipStream_t s, double bytes) : stream(s)
+{
+    std::lock_guard<std::mutex> lk(g_prof_mutex);
+    if (!g_prof.on)
+        return;
+    Record r { name, g_prof.get(), g_prof.get(), bytes };
+    if (!r.start || !r.stop)
+        return;
+    (void)hipEventRecord(r.start, s);
+    g_prof.records.push_back(r);
+    slot = int(g_prof.records.size()) - 1;
+}
+
+StageScope::~StageScope()
+{
+    if (slot < 0)
+        return;
+    std::lock_guard<std::mutex> lk(g_prof_mutex);
+    if (slot < int(g_prof.records.size()))
+        (void)hipEventRecord(g_prof.records[slot].stop, stream);
+}
 template DevCamera<double> make_dev_camera<double>(const dgmr_camera&, const dgmr_tiling&);
 
 } // namespace dgmr
@@ -111,5 +162,40 @@ extern "C" {
 const char* dgmr_last_error(void) { return dgmr::g_last_error.c_str(); }
 
 const char* dgmr_version(void) { return "dgmr-mi355x 0.1 (gfx950)"; }
+
+int dgmr_profile_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(dgmr::g_prof_mutex);
+    dgmr::g_prof.on = on != 0;
+    return DGMR_OK;
+}
+
+int dgmr_profile_read(dgmr_stage_time* out, int32_t max_stages, int32_t* n_stages)
+{
+    std::lock_guard<std::mutex> lk(dgmr::g_prof_mutex);
+    std::vector<dgmr_stage_time> acc;
+    for (auto& r : dgmr::g_prof.records) {
+        float ms = 0.f;
+        if (hipEventSynchronize(r.stop) != hipSuccess || hipEventElapsedTime(&ms, r.start, r.stop) != hipSuccess)
+            ms = 0.f;
+        size_t k = 0;
+        while (k < acc.size() && std::strcmp(acc[k].name, r.name) != 0)
+            ++k;
+        if (k == acc.size())
+            acc.push_back(dgmr_stage_time { r.name, 0.0, 0, 0.0 });
+        acc[k].total_ms += ms;
+        acc[k].launches += 1;
+        acc[k].bytes += r.bytes;
+        dgmr::g_prof.pool.push_back(r.start);
+        dgmr::g_prof.pool.push_back(r.stop);
+    }
+    dgmr::g_prof.records.clear();
+    const int32_t count = int32_t(acc.size());
+    if (n_stages)
+        *n_stages = count;
+    for (int32_t i = 0; i < count && i < max_stages && out; ++i)
+        out[i] = acc[i];
+    return DGMR_OK;
+}
 
 } // extern "C"

@@ -1,0 +1,332 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Batched multi-view splatting: many cameras of one scene with uneven
+// resolutions in one preprocess launch, one binning pass and one persistent
+// blend launch (forward and backward).
+//
+// Layout: per-view geometry is stored at slot v * n + i, so Gaussian ids in
+// the pair list index every array directly; view v's tiles occupy a contiguous
+// range of the global tile index space.  The blend runs as a persistent grid
+// whose workgroups take the next tile of any view from a device counter, so
+// small and large views share the chip without per-view launches or syncs.
+#include <algorithm>
+#include <vector>
+
+#include "host_common.h"
+#include "primitives.h"
+#include "splat_device.h"
+
+namespace dgmr {
+namespace {
+
+struct ViewBlend {
+    int width, height, grid_x, tile_base;
+    float bg[3];
+    float* framebuffer;
+    float* final_T;
+    const float* grad_fb;
+};
+
+__global__ __launch_bounds__(256) void k_batch_preprocess(int64_t n, SplatParams p, const DevCamera<float>* __restrict__ cams, SplatOut out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (i < n)
+        preprocess_gaussian(i, int64_t(v) * n + i, p, cams[v], out);
+}
+
+__device__ __forceinline__ int find_view(const ViewBlend* views, int n_views, int tile)
+{
+    int lo = 0, hi = n_views - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (views[mid].tile_base <= tile)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Persistent blend over the global tile queue.  The queue counter starts at 0;
+// every workgroup exits once the counter has passed the last tile.
+template <int TW, int TH, int B, bool BACKWARD>
+__global__ __launch_bounds__(TW* TH) void k_batch_blend(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
+    const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, const ViewBlend* __restrict__ views,
+    int n_views, int total_tiles, uint32_t* __restrict__ queue, float* __restrict__ screen_grads)
+{
+    __shared__ BlendSmem<B> sm;
+    __shared__ int s_tile;
+    while (true) {
+        if (threadIdx.x == 0)
+            s_tile = int(atomicAdd(queue, 1u));
+        __syncthreads();
+        const int tile = s_tile;
+        __syncthreads();
+        if (tile >= total_tiles)
+            break;
+        const ViewBlend& vb = views[find_view(views, n_views, tile)];
+        const int local = tile - vb.tile_base;
+        const int px0 = (local % vb.grid_x) * TW, py0 = (local / vb.grid_x) * TH;
+        if (BACKWARD)
+            blend_tile_backward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
+                vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T, vb.grad_fb, screen_grads);
+        else
+            blend_tile_forward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
+                vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T);
+        __syncthreads();
+    }
+}
+
+// thread per Gaussian, summing its adjoint over the views that see it
+__global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, int n_views, SplatParams p, const DevCamera<float>* __restrict__ cams,
+    const uint32_t* __restrict__ tiles, const float4* __restrict__ conic_opacity, const uint8_t* __restrict__ clamped,
+    const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations,
+    float* __restrict__ g_opacity, float* __restrict__ g_sh)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float g_mu[3] = { 0.f, 0.f, 0.f }, g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f }, g_w = 0.f;
+    float g_shv[48];
+#pragma unroll
+    for (int k = 0; k < 48; ++k)
+        g_shv[k] = 0.f;
+    for (int v = 0; v < n_views; ++v) {
+        const int64_t o = int64_t(v) * n + i;
+        if (tiles[o] != 0)
+            preprocess_backward_gaussian(i, p, cams[v], conic_opacity[o], clamped + 3 * o, screen_grads + o * 9, g_mu, g_s, g_q, &g_w, g_shv);
+    }
+    for (int a = 0; a < 3; ++a) {
+        g_means[3 * i + a] = g_mu[a];
+        g_scales[3 * i + a] = g_s[a];
+    }
+    for (int a = 0; a < 4; ++a)
+        g_rotations[4 * i + a] = g_q[a];
+    g_opacity[i] = g_w;
+    float* gsh = g_sh + i * p.sh_coeffs * 3;
+#pragma unroll
+    for (int k = 0; k < 48; ++k)
+        if (k < p.sh_coeffs * 3)
+            gsh[k] = g_shv[k];
+}
+
+// Workspace layout shared by forward and backward (the forward's geometry and
+// binning must survive until the backward).
+struct BatchLayout {
+    DevCamera<float>* cams;
+    ViewBlend* views;
+    int4* grids;
+    uint32_t* queue;
+    int32_t* rects;
+    uint32_t* tiles;
+    float* xy;
+    float4* conic_opacity;
+    float* rgb;
+    uint8_t* clamped;
+    float* depths;
+    uint32_t* point_list;
+    uint32_t* ranges;
+    float* screen_grads;
+    void* binning;
+    size_t binning_bytes;
+    int64_t total_tiles;
+};
+
+BatchLayout carve_batch(void* ws, int64_t n, const dgmr_batch_view* views, int n_views, const dgmr_tiling& t, int64_t cap, size_t* used)
+{
+    Carver cv(ws, ~size_t(0));
+    BatchLayout L;
+    int64_t total_tiles = 0;
+    for (int v = 0; v < n_views; ++v)
+        total_tiles += int64_t((views[v].cam.width + t.tile_w - 1) / t.tile_w) * ((views[v].cam.height + t.tile_h - 1) / t.tile_h);
+    const int64_t m = std::max<int64_t>(1, n * n_views);
+    L.total_tiles = total_tiles;
+    L.cams = cv.take<DevCamera<float>>(n_views);
+    L.views = cv.take<ViewBlend>(n_views);
+    L.grids = cv.take<int4>(n_views);
+    L.queue = cv.take<uint32_t>(64);
+    L.rects = cv.take<int32_t>(2 * m);
+    L.tiles = cv.take<uint32_t>(m);
+    L.xy = cv.take<float>(2 * m);
+    L.conic_opacity = cv.take<float4>(m);
+    L.rgb = cv.take<float>(3 * m);
+    L.clamped = cv.take<uint8_t>(3 * m);
+    L.depths = cv.take<float>(m);
+    L.point_list = cv.take<uint32_t>(std::max<int64_t>(cap, 1));
+    L.ranges = cv.take<uint32_t>(2 * std::max<int64_t>(total_tiles, 1));
+    L.screen_grads = cv.take<float>(9 * m);
+    L.binning_bytes = splat_workspace_bytes(m, cap);
+    L.binning = cv.take<char>(L.binning_bytes);
+    *used = cv.used;
+    return L;
+}
+
+template <int TW, int TH>
+void launch_batch_blend(bool backward, int batch, int grid, hipStream_t st, const BatchLayout& L, int n_views)
+{
+    const auto* xy2 = reinterpret_cast<const float2*>(L.xy);
+    const int tt = int(L.total_tiles);
+#define DGMR_BATCH_BLEND(B)                                                                                                            \
+    if (backward)                                                                                                                      \
+        k_batch_blend<TW, TH, B, true><<<grid, TW * TH, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt,  \
+            L.queue, L.screen_grads);                                                                                                  \
+    else                                                                                                                               \
+        k_batch_blend<TW, TH, B, false><<<grid, TW * TH, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt, \
+            L.queue, L.screen_grads);
+    if (batch == 128) {
+        DGMR_BATCH_BLEND(128)
+    } else if (batch == 256) {
+        DGMR_BATCH_BLEND(256)
+    } else {
+        DGMR_BATCH_BLEND(512)
+    }
+#undef DGMR_BATCH_BLEND
+}
+
+void batch_blend(bool backward, const dgmr_tiling& t, hipStream_t st, const BatchLayout& L, int n_views)
+{
+    // persistent grid: a few workgroups per CU, never more than there are tiles
+    const int per_cu = (t.tile_w * t.tile_h == 64) ? 16 : 8;
+    const int grid = int(std::min<int64_t>(L.total_tiles, 256 * per_cu));
+    if (t.tile_w == 16)
+        launch_batch_blend<16, 16>(backward, t.batch, grid, st, L, n_views);
+    else if (t.tile_w == 8)
+        launch_batch_blend<8, 8>(backward, t.batch, grid, st, L, n_views);
+    else
+        launch_batch_blend<32, 8>(backward, t.batch, grid, st, L, n_views);
+}
+
+int check_batch_args(const dgmr_gaussians* g, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling)
+{
+    DGMR_ARG_CHECK(g && views && n_views > 0 && n_views <= 4096, "dgmr batch: bad views");
+    DGMR_ARG_CHECK(g->dtype == DGMR_F32, "dgmr batch: the splatter computes in float32");
+    DGMR_ARG_CHECK(valid_tiling(tiling), "dgmr batch: bad tiling");
+    DGMR_ARG_CHECK(g->n >= 0 && g->n * int64_t(n_views) < (int64_t(1) << 32), "dgmr batch: n * views must stay below 2^32");
+    DGMR_ARG_CHECK(g->sh_coeffs >= 1 && g->sh_coeffs <= 16 && g->sh_degree >= 0 && g->sh_degree <= 3, "dgmr batch: bad SH layout");
+    for (int v = 0; v < n_views; ++v)
+        DGMR_ARG_CHECK(views[v].cam.width > 0 && views[v].cam.height > 0 && views[v].framebuffer && views[v].final_transmittance,
+            "dgmr batch: bad view");
+    return DGMR_OK;
+}
+
+} // namespace
+} // namespace dgmr
+
+using namespace dgmr;
+
+extern "C" {
+
+int dgmr_batch_workspace_size(int64_t n, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling, int64_t pair_capacity,
+    size_t* bytes)
+{
+    DGMR_ARG_CHECK(views && n_views > 0 && valid_tiling(tiling) && bytes && pair_capacity >= 0 && n >= 0, "dgmr_batch_workspace_size: bad argument");
+    carve_batch(nullptr, n, views, n_views, *tiling, pair_capacity, bytes);
+    return DGMR_OK;
+}
+
+int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling, void* workspace,
+    size_t workspace_bytes, int64_t pair_capacity, dgmr_stats* stats, dgmr_stream stream)
+{
+    const int rc0 = check_batch_args(g, views, n_views, tiling);
+    if (rc0 != DGMR_OK)
+        return rc0;
+    DGMR_ARG_CHECK(workspace, "dgmr_splat_batch_forward: null workspace");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int64_t n = g->n;
+    size_t need = 0;
+    BatchLayout L = carve_batch(workspace, n, views, n_views, *tiling, pair_capacity, &need);
+    DGMR_ARG_CHECK(workspace_bytes >= need, "dgmr_splat_batch_forward: workspace too small");
+    std::vector<DevCamera<float>> cams(n_views);
+    std::vector<ViewBlend> vb(n_views);
+    std::vector<int4> grids(n_views);
+    int64_t base = 0;
+    for (int v = 0; v < n_views; ++v) {
+        cams[v] = make_dev_camera<float>(views[v].cam, *tiling);
+        vb[v] = ViewBlend { views[v].cam.width, views[v].cam.height, cams[v].grid_x, int(base),
+            { float(views[v].cam.background[0]), float(views[v].cam.background[1]), float(views[v].cam.background[2]) }, views[v].framebuffer,
+            views[v].final_transmittance, views[v].grad_framebuffer };
+        grids[v] = make_int4(cams[v].grid_x, cams[v].grid_y, int(base), 0);
+        base += int64_t(cams[v].grid_x) * cams[v].grid_y;
+    }
+    DGMR_HIP_CHECK(hipMemcpyAsync(L.cams, cams.data(), sizeof(DevCamera<float>) * n_views, hipMemcpyHostToDevice, st));
+    DGMR_HIP_CHECK(hipMemcpyAsync(L.views, vb.data(), sizeof(ViewBlend) * n_views, hipMemcpyHostToDevice, st));
+    DGMR_HIP_CHECK(hipMemcpyAsync(L.grids, grids.data(), sizeof(int4) * n_views, hipMemcpyHostToDevice, st));
+    if (stats)
+        stats->n_pairs = 0;
+    const int64_t m = n * n_views;
+    uint32_t *dkeys, *dids;
+    binning_key_buffers(L.binning, m, &dkeys, &dids);
+    if (n > 0) {
+        StageScope sc("batch.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 67.0);
+        const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
+            static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier) };
+        k_batch_preprocess<<<dim3(unsigned(ceil_div64(n, 256)), unsigned(n_views)), 256, 0, st>>>(n, p, L.cams,
+            SplatOut { L.rects, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, L.depths, dkeys, dids });
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
+    dgmr_binning bins { nullptr, L.point_list, L.ranges, pair_capacity };
+    Carver cv(L.binning, L.binning_bytes);
+    const BatchBinning bb { L.grids, std::max<int64_t>(n, 1), n_views, L.total_tiles };
+    int64_t R = 0;
+    if (m > 0) {
+        const int rc = bin_gaussians(m, L.rects, L.tiles, L.xy, false, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &R, st, &bb);
+        if (stats)
+            stats->n_pairs = R;
+        if (rc != DGMR_OK)
+            return rc;
+    } else {
+        DGMR_HIP_CHECK(hipMemsetAsync(L.ranges, 0, size_t(L.total_tiles) * 2 * sizeof(uint32_t), st));
+    }
+    StageScope sc("batch.blend_forward", st, 40.0 * double(R));
+    DGMR_HIP_CHECK(hipMemsetAsync(L.queue, 0, sizeof(uint32_t), st));
+    batch_blend(false, *tiling, st, L, n_views);
+    DGMR_HIP_CHECK(hipGetLastError());
+    return DGMR_OK;
+}
+
+int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling, void* workspace,
+    size_t workspace_bytes, int64_t pair_capacity, int64_t n_pairs, float* grad_means, float* grad_scales, float* grad_rotations,
+    float* grad_opacity, float* grad_sh, dgmr_stream stream)
+{
+    const int rc0 = check_batch_args(g, views, n_views, tiling);
+    if (rc0 != DGMR_OK)
+        return rc0;
+    DGMR_ARG_CHECK(workspace && grad_means && grad_scales && grad_rotations && grad_opacity && grad_sh, "dgmr_splat_batch_backward: null argument");
+    for (int v = 0; v < n_views; ++v)
+        DGMR_ARG_CHECK(views[v].grad_framebuffer, "dgmr_splat_batch_backward: every view needs grad_framebuffer");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int64_t n = g->n;
+    size_t need = 0;
+    BatchLayout L = carve_batch(workspace, n, views, n_views, *tiling, pair_capacity, &need);
+    DGMR_ARG_CHECK(workspace_bytes >= need, "dgmr_splat_batch_backward: workspace too small");
+    if (n == 0)
+        return DGMR_OK;
+    // the grad pointers may differ from the forward call: refresh the view table
+    std::vector<ViewBlend> vb(n_views);
+    int64_t base = 0;
+    for (int v = 0; v < n_views; ++v) {
+        const DevCamera<float> c = make_dev_camera<float>(views[v].cam, *tiling);
+        vb[v] = ViewBlend { views[v].cam.width, views[v].cam.height, c.grid_x, int(base),
+            { float(views[v].cam.background[0]), float(views[v].cam.background[1]), float(views[v].cam.background[2]) }, views[v].framebuffer,
+            views[v].final_transmittance, views[v].grad_framebuffer };
+        base += int64_t(c.grid_x) * c.grid_y;
+    }
+    DGMR_HIP_CHECK(hipMemcpyAsync(L.views, vb.data(), sizeof(ViewBlend) * n_views, hipMemcpyHostToDevice, st));
+    const int64_t m = n * n_views;
+    DGMR_HIP_CHECK(hipMemsetAsync(L.screen_grads, 0, size_t(m) * 9 * sizeof(float), st));
+    if (n_pairs > 0) {
+        StageScope sc("batch.blend_backward", st, 40.0 * double(n_pairs));
+        DGMR_HIP_CHECK(hipMemsetAsync(L.queue, 0, sizeof(uint32_t), st));
+        batch_blend(true, *tiling, st, L, n_views);
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
+    StageScope sc("batch.preprocess_backward", st, double(n) * 2.0 * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 40.0);
+    const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
+        static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier) };
+    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams, L.tiles, L.conic_opacity, L.clamped,
+        L.screen_grads, grad_means, grad_scales, grad_rotations, grad_opacity, grad_sh);
+    DGMR_HIP_CHECK(hipGetLastError());
+    return DGMR_OK;
+}
+
+} // extern "C"

@@ -58,9 +58,9 @@ __device__ __forceinline__ S dot3(const S* a, const S* b) { return (a[0] * b[0] 
 template <class S>
 __device__ __forceinline__ S sqrt_rn(S v);
 template <>
-__device__ __forceinline__ float sqrt_rn<float>(float v) { return __fsqrt_rn(v); }
+__device__ __forceinline__ float sqrt_rn<float>(float v) { return __builtin_sqrtf(v); } // correctly rounded (__fsqrt_rn is the native approximation here)
 template <>
-__device__ __forceinline__ double sqrt_rn<double>(double v) { return __dsqrt_rn(v); }
+__device__ __forceinline__ double sqrt_rn<double>(double v) { return __builtin_sqrt(v); }
 
 template <class S>
 __device__ __forceinline__ S div_rn(S a, S b);

@@ -60,4 +60,31 @@ DevCamera<S> make_dev_camera(const dgmr_camera& c, const dgmr_tiling& t);
 
 uint32_t higher_msb(uint32_t n);
 
+// stage timing (dgmr_profile_enable): a scope records a start event on
+// construction and a stop event on destruction when profiling is on
+struct StageScope {
+    StageScope(const char* name, hipStream_t stream, double bytes = 0);
+    ~StageScope();
+    int slot = -1;
+    hipStream_t stream;
+};
+
+// multi-view binning: Gaussian ids span n_views blocks of n_per_view; view v's
+// tiles occupy [views[v].z, views[v].z + views[v].x * views[v].y) of the
+// global tile range (views: device array of (grid_x, grid_y, first tile, -))
+struct BatchBinning {
+    const int4* views;
+    int64_t n_per_view;
+    int n_views;
+    int64_t total_tiles;
+};
+
+// binning shared by both renderers (implemented with the splatter): depth sort,
+// scan, emission, pair sort, tile ranges.  The preprocess must have filled the
+// depth-key and id buffers returned by binning_key_buffers.
+int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, int grid_x, int grid_y, int tile_w,
+    int tile_h, dgmr_binning* bins, Carver& cv, int64_t* n_pairs, hipStream_t st, const BatchBinning* batch);
+void binning_key_buffers(void* workspace, int64_t n, uint32_t** keys, uint32_t** ids);
+size_t splat_workspace_bytes(int64_t n, int64_t cap);
+
 } // namespace dgmr

@@ -23,6 +23,7 @@
 #include "device_math.h"
 #include "host_common.h"
 #include "primitives.h"
+#include "splat_device.h"
 
 namespace dgmr {
 
@@ -30,82 +31,11 @@ namespace {
 
 constexpr int kPreThreads = 256;
 
-__global__ __launch_bounds__(kPreThreads) void k_splat_preprocess(int64_t n, const float* __restrict__ means, const float* __restrict__ scales,
-    const float* __restrict__ rotations, const float* __restrict__ opacity, const float* __restrict__ sh, int sh_coeffs, int sh_degree,
-    float scale_multiplier, DevCamera<float> cam, int32_t* __restrict__ rects, uint32_t* __restrict__ tiles, float* __restrict__ xy_out,
-    float4* __restrict__ conic_opacity, float* __restrict__ rgb_out, uint8_t* __restrict__ clamped_out, float* __restrict__ depths,
-    uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ ids)
+__global__ __launch_bounds__(kPreThreads) void k_splat_preprocess(int64_t n, SplatParams p, DevCamera<float> cam, SplatOut out)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    tiles[i] = 0;
-    rects[2 * i] = 0;
-    rects[2 * i + 1] = 0;
-    xy_out[2 * i] = 0.f;
-    xy_out[2 * i + 1] = 0.f;
-    depth_keys[i] = 0xffffffffu;
-    ids[i] = uint32_t(i);
-    const float mu[3] = { means[3 * i], means[3 * i + 1], means[3 * i + 2] };
-    float t[3];
-    transform_point(cam.view, mu, t, 3);
-    if (t[2] < 0.2f)
-        return;
-    const float s[3] = { scales[3 * i] * scale_multiplier, scales[3 * i + 1] * scale_multiplier, scales[3 * i + 2] * scale_multiplier };
-    const float q[4] = { rotations[4 * i], rotations[4 * i + 1], rotations[4 * i + 2], rotations[4 * i + 3] };
-    const float w = opacity[i];
-    SplatIn<float> in;
-    splat_internals(mu, s, q, cam, &in);
-    const float cov_f[3] = { in.cov2[0] + 0.3f, in.cov2[1], in.cov2[2] + 0.3f };
-    float xy[2];
-    project_to_screen(mu, cam, xy);
-    const int ext_x = guarded_int(ceilf(3.f * __fsqrt_rn(cov_f[0])));
-    const int ext_y = guarded_int(ceilf(3.f * __fsqrt_rn(cov_f[2])));
-    rects[2 * i] = ext_x;
-    rects[2 * i + 1] = ext_y;
-    int x0, y0, x1, y1;
-    get_rect(xy[0], xy[1], ext_x, ext_y, cam.grid_x, cam.grid_y, cam.tile_w, cam.tile_h, &x0, &y0, &x1, &y1);
-    const uint32_t touched = uint32_t(x1 - x0) * uint32_t(y1 - y0);
-    if (touched == 0)
-        return;
-    tiles[i] = touched;
-    xy_out[2 * i] = xy[0];
-    xy_out[2 * i + 1] = xy[1];
-    const float dv[3] = { cam.pos[0] - mu[0], cam.pos[1] - mu[1], cam.pos[2] - mu[2] };
-    const float depth = length3(dv);
-    depths[i] = depth;
-    depth_keys[i] = __float_as_uint(depth);
-    // colour from SH along normalize(mu - cam) (math.h:233-271)
-    const float v[3] = { -dv[0], -dv[1], -dv[2] };
-    const float inv_len = __fdiv_rn(1.f, __fsqrt_rn(dot3(v, v)));
-    const float dir[3] = { v[0] * inv_len, v[1] * inv_len, v[2] * inv_len };
-    float Y[16];
-    sh_basis(sh_degree, dir, Y);
-    const int kmax = min(sh_coeffs, (sh_degree + 1) * (sh_degree + 1));
-    const float* c = sh + int64_t(i) * sh_coeffs * 3;
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-    for (int k = 0; k < kmax; ++k) {
-        r0 = fmaf(Y[k], c[3 * k + 0], r0);
-        r1 = fmaf(Y[k], c[3 * k + 1], r1);
-        r2 = fmaf(Y[k], c[3 * k + 2], r2);
-    }
-    r0 += 0.5f;
-    r1 += 0.5f;
-    r2 += 0.5f;
-    rgb_out[3 * i + 0] = fmaxf(r0, 0.f);
-    rgb_out[3 * i + 1] = fmaxf(r1, 0.f);
-    rgb_out[3 * i + 2] = fmaxf(r2, 0.f);
-    clamped_out[3 * i + 0] = r0 >= 0.f;
-    clamped_out[3 * i + 1] = r1 >= 0.f;
-    clamped_out[3 * i + 2] = r2 >= 0.f;
-    // screen-space weight, Ots formulation (math.h:389-397)
-    int i0, i1;
-    larger2_idx(s, &i0, &i1);
-    const float i2prime = float(2 * kPi) * s[i0] * s[i1];
-    const float det2 = cov_f[0] * cov_f[2] - cov_f[1] * cov_f[1];
-    const float i2 = float(2 * kPi) * __fsqrt_rn(det2);
-    const float weight = w * det3(in.J) * i2prime / i2;
-    conic_opacity[i] = make_float4(cov_f[2] / det2, -cov_f[1] / det2, cov_f[0] / det2, weight);
+    if (i < n)
+        preprocess_gaussian(i, i, p, cam, out);
 }
 
 __global__ void k_gather_counts(int64_t n, const uint32_t* __restrict__ ids, const uint32_t* __restrict__ tiles, uint32_t* __restrict__ counts)
@@ -121,25 +51,38 @@ __global__ void k_gather_counts(int64_t n, const uint32_t* __restrict__ ids, con
 // owns 64 consecutive depth-sorted Gaussians and writes their pairs 64 at a
 // time, each lane locating its Gaussian by binary search over the wave's
 // offsets in LDS, so one large Gaussian no longer serialises a thread.
+// views: per-view (grid_x, grid_y, first global tile, -) when ids span several
+// views of n_per_view Gaussians each; null for a single view.
+template <class S>
 __global__ __launch_bounds__(256) void k_emit_pairs(int64_t n, const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ offsets,
-    const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const float* __restrict__ xy, int grid_x, int grid_y, int tile_w,
-    int tile_h, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
+    const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const S* __restrict__ xy, int grid_x, int grid_y, int tile_w,
+    int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
 {
-    __shared__ uint32_t s_off[4][64], s_x0[4][64], s_y0[4][64], s_w[4][64], s_id[4][64];
+    __shared__ uint32_t s_off[4][64], s_x0[4][64], s_y0[4][64], s_w[4][64], s_id[4][64], s_gx[4][64], s_tb[4][64];
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / 64;
     const int64_t j = (int64_t(blockIdx.x) * 4 + wave) * 64 + lane;
-    uint32_t off = 0xffffffffu, cnt = 0, x0 = 0, y0 = 0, w = 1, id = 0;
+    uint32_t off = 0xffffffffu, cnt = 0, x0 = 0, y0 = 0, w = 1, id = 0, gxv = uint32_t(grid_x), tb = 0;
     if (j < n) {
         id = sorted_ids[j];
         off = offsets[j];
         cnt = tiles[id];
         if (cnt) {
+            int gx = grid_x, gy = grid_y, tbase = 0;
+            if (views) {
+                const int4 v = views[id / n_per_view];
+                gx = v.x;
+                gy = v.y;
+                tbase = v.z;
+            }
             int a, b, c, d;
-            get_rect(xy[2 * id], xy[2 * id + 1], rects[2 * id], rects[2 * id + 1], grid_x, grid_y, tile_w, tile_h, &a, &b, &c, &d);
+            get_rect(xy[2 * id], xy[2 * id + 1], rects[2 * id], rects[2 * id + 1], gx, gy, tile_w, tile_h, &a, &b, &c, &d);
+            // a view's tiles occupy [tbase, tbase + gx * gy) of the global tile range
             x0 = uint32_t(a);
             y0 = uint32_t(b);
             w = uint32_t(c - a);
+            gxv = uint32_t(gx);
+            tb = uint32_t(tbase);
         }
     }
     s_off[wave][lane] = off;
@@ -147,6 +90,8 @@ __global__ __launch_bounds__(256) void k_emit_pairs(int64_t n, const uint32_t* _
     s_y0[wave][lane] = y0;
     s_w[wave][lane] = w;
     s_id[wave][lane] = id;
+    s_gx[wave][lane] = gxv;
+    s_tb[wave][lane] = tb;
     uint32_t end = (j < n) ? off + cnt : 0u;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1)
@@ -178,7 +123,7 @@ __global__ __launch_bounds__(256) void k_emit_pairs(int64_t n, const uint32_t* _
             ++ty;
             tx -= int32_t(ww);
         }
-        keys_out[target] = (s_y0[wave][lo] + ty) * uint32_t(grid_x) + s_x0[wave][lo] + uint32_t(tx);
+        keys_out[target] = s_tb[wave][lo] + (s_y0[wave][lo] + ty) * s_gx[wave][lo] + s_x0[wave][lo] + uint32_t(tx);
         vals_out[target] = s_id[wave][lo];
     }
 }
@@ -196,281 +141,75 @@ __global__ void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t*
         ranges[2 * tile + 1] = k + 1;
 }
 
-// logical tile of a block: workgroups are dealt round-robin over the 8 XCDs,
-// so give each XCD one contiguous band of tiles (shared Gaussians stay in
-// that XCD's L2)
-__device__ __forceinline__ int xcd_tile(int b, int n_tiles)
-{
-    const int q = n_tiles / 8, r = n_tiles % 8;
-    const int x = b % 8, slot = b / 8;
-    return x * q + min(x, r) + slot;
-}
-
 template <int TW, int TH, int B>
 __global__ __launch_bounds__(TW* TH) void k_blend_forward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, int width, int height, int grid_x,
     int n_tiles, float bg0, float bg1, float bg2, float* __restrict__ framebuffer, float* __restrict__ final_T)
 {
-    constexpr int NT = TW * TH;
-    __shared__ float4 s_geo[B];  // x, y, -a/2, -b
-    __shared__ float4 s_col[B];  // -c/2, opacity, r, g
-    __shared__ float s_blue[B];
+    __shared__ BlendSmem<B> sm;
     const int tile = xcd_tile(blockIdx.x, n_tiles);
-    const int tx = tile % grid_x, ty = tile / grid_x;
-    const int px = tx * TW + int(threadIdx.x) % TW;
-    const int py = ty * TH + int(threadIdx.x) / TW;
-    const bool inside = px < width && py < height;
-    const float pxf = float(px), pyf = float(py);
-    const uint32_t first = ranges[2 * tile], last = ranges[2 * tile + 1];
-    float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
-    bool done = !inside;
-    for (uint32_t base = first; base < last; base += B) {
-        if (__syncthreads_count(!done) == 0)
-            break;
-        for (int i = threadIdx.x; i < B; i += NT) {
-            const uint32_t k = base + uint32_t(i);
-            if (k < last) {
-                const uint32_t id = point_list[k];
-                const float2 p = xy[id];
-                const float4 co = conic_opacity[id];
-                s_geo[i] = make_float4(p.x, p.y, -0.5f * co.x, -co.y);
-                s_col[i] = make_float4(-0.5f * co.z, co.w, rgb[3 * id], rgb[3 * id + 1]);
-                s_blue[i] = rgb[3 * id + 2];
-            }
-        }
-        __syncthreads();
-        if (!done) {
-            const int cnt = int(min<uint32_t>(B, last - base));
-            for (int j = 0; j < cnt; ++j) {
-                const float4 g = s_geo[j];
-                const float4 c = s_col[j];
-                const float dx = pxf - g.x, dy = pyf - g.y;
-                const float power = fmaf(g.z * dx, dx, fmaf(c.x * dy, dy, g.w * dx * dy));
-                const float alpha = fminf(0.99f, c.y * __expf(power));
-                if (alpha < 1.0f / 255.0f)
-                    continue;
-                const float test_T = T * (1.f - alpha);
-                if (test_T < 0.0001f) {
-                    done = true;
-                    break;
-                }
-                const float wgt = alpha * T;
-                C0 = fmaf(c.z, wgt, C0);
-                C1 = fmaf(c.w, wgt, C1);
-                C2 = fmaf(s_blue[j], wgt, C2);
-                T = test_T;
-            }
-        }
-    }
-    if (!inside)
-        return;
-    const int64_t p = int64_t(py) * width + px;
-    const int64_t plane = int64_t(width) * height;
-    framebuffer[p] = fmaf(T, bg0, C0);
-    framebuffer[plane + p] = fmaf(T, bg1, C1);
-    framebuffer[2 * plane + p] = fmaf(T, bg2, C2);
-    final_T[p] = T;
+    blend_tile_forward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], (tile % grid_x) * TW, (tile / grid_x) * TH, width, height, bg0, bg1,
+        bg2, point_list, xy, conic_opacity, rgb, framebuffer, final_T);
 }
 
-// Blend adjoint, front to back.  The remaining colour is carried unnormalised
-// (C_final minus what has been composited so far), which is the reference
-// prototype's remaining colour (grad_inria_splatter_render.cu:100-156) times
-// the current transmittance, without the repeated division by (1 - alpha).
-// Per-Gaussian gradients are summed over the 64 pixels of a wave with DPP
-// before one lane issues the global atomics.
 template <int TW, int TH, int B>
 __global__ __launch_bounds__(TW* TH) void k_blend_backward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, int width, int height, int grid_x,
     int n_tiles, float bg0, float bg1, float bg2, const float* __restrict__ framebuffer, const float* __restrict__ final_T,
     const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
 {
-    constexpr int NT = TW * TH;
-    __shared__ float4 s_geo[B];  // x, y, a, b
-    __shared__ float4 s_col[B];  // c, opacity, r, g
-    __shared__ float s_blue[B];
-    __shared__ uint32_t s_id[B];
+    __shared__ BlendSmem<B> sm;
     const int tile = xcd_tile(blockIdx.x, n_tiles);
-    const int tx = tile % grid_x, ty = tile / grid_x;
-    const int px = tx * TW + int(threadIdx.x) % TW;
-    const int py = ty * TH + int(threadIdx.x) / TW;
-    const bool inside = px < width && py < height;
-    const float pxf = float(px), pyf = float(py);
-    const uint32_t first = ranges[2 * tile], last = ranges[2 * tile + 1];
-    float gC0 = 0.f, gC1 = 0.f, gC2 = 0.f, Tf = 1.f, S0 = 0.f, S1 = 0.f, S2 = 0.f;
-    if (inside) {
-        const int64_t p = int64_t(py) * width + px;
-        const int64_t plane = int64_t(width) * height;
-        gC0 = grad_fb[p];
-        gC1 = grad_fb[plane + p];
-        gC2 = grad_fb[2 * plane + p];
-        Tf = final_T[p];
-        S0 = framebuffer[p] - Tf * bg0;
-        S1 = framebuffer[plane + p] - Tf * bg1;
-        S2 = framebuffer[2 * plane + p] - Tf * bg2;
-    }
-    const float gT_Tf = (gC0 * bg0 + gC1 * bg1 + gC2 * bg2) * Tf;
-    float T = 1.f;
-    bool done = !inside || (gC0 == 0.f && gC1 == 0.f && gC2 == 0.f);
-    for (uint32_t base = first; base < last; base += B) {
-        if (__syncthreads_count(!done) == 0)
-            break;
-        for (int i = threadIdx.x; i < B; i += NT) {
-            const uint32_t k = base + uint32_t(i);
-            if (k < last) {
-                const uint32_t id = point_list[k];
-                const float2 p = xy[id];
-                const float4 co = conic_opacity[id];
-                s_geo[i] = make_float4(p.x, p.y, co.x, co.y);
-                s_col[i] = make_float4(co.z, co.w, rgb[3 * id], rgb[3 * id + 1]);
-                s_blue[i] = rgb[3 * id + 2];
-                s_id[i] = id;
-            }
-        }
-        __syncthreads();
-        const int cnt = int(min<uint32_t>(B, last - base));
-        for (int j = 0; j < cnt; ++j) {
-            if (__ballot(!done) == 0)
-                break; // the whole wave has finished
-            const float4 g = s_geo[j];
-            const float4 c = s_col[j];
-            const float blue = s_blue[j];
-            float gx = 0.f, gy = 0.f, ga = 0.f, gb = 0.f, gc = 0.f, go = 0.f, gr = 0.f, gg = 0.f, gbl = 0.f;
-            bool contrib = false;
-            if (!done) {
-                const float dx = pxf - g.x, dy = pyf - g.y;
-                const float power = fmaf(-0.5f * g.z * dx, dx, fmaf(-0.5f * c.x * dy, dy, -g.w * dx * dy));
-                const float G = __expf(power);
-                const float raw = c.y * G;
-                const float alpha = fminf(0.99f, raw);
-                if (alpha >= 1.0f / 255.0f) {
-                    const float tk = 1.f - alpha;
-                    const float test_T = T * tk;
-                    if (test_T < 0.0001f) {
-                        done = true;
-                    } else {
-                        contrib = true;
-                        const float wgt = alpha * T;
-                        S0 = fmaf(-c.z, wgt, S0);
-                        S1 = fmaf(-c.w, wgt, S1);
-                        S2 = fmaf(-blue, wgt, S2);
-                        gr = gC0 * wgt;
-                        gg = gC1 * wgt;
-                        gbl = gC2 * wgt;
-                        const float g_alpha = (gC0 * c.z + gC1 * c.w + gC2 * blue) * T - (gC0 * S0 + gC1 * S1 + gC2 * S2 + gT_Tf) / tk;
-                        const float g_raw = raw >= 0.99f ? 0.f : g_alpha;
-                        go = G * g_raw;
-                        const float gp = c.y * g_raw * G;
-                        ga = gp * (-0.5f * dx * dx);
-                        gb = gp * (-dx * dy);
-                        gc = gp * (-0.5f * dy * dy);
-                        gx = gp * (g.z * dx + g.w * dy);
-                        gy = gp * (c.x * dy + g.w * dx);
-                        T = test_T;
-                    }
-                }
-            }
-            if (__ballot(contrib) != 0) {
-                gx = wave_sum_to_last(gx);
-                gy = wave_sum_to_last(gy);
-                ga = wave_sum_to_last(ga);
-                gb = wave_sum_to_last(gb);
-                gc = wave_sum_to_last(gc);
-                go = wave_sum_to_last(go);
-                gr = wave_sum_to_last(gr);
-                gg = wave_sum_to_last(gg);
-                gbl = wave_sum_to_last(gbl);
-                if (lane_id() == 63) {
-                    float* dst = screen_grads + int64_t(s_id[j]) * 9;
-                    unsafeAtomicAdd(dst + 0, gx);
-                    unsafeAtomicAdd(dst + 1, gy);
-                    unsafeAtomicAdd(dst + 2, ga);
-                    unsafeAtomicAdd(dst + 3, gb);
-                    unsafeAtomicAdd(dst + 4, gc);
-                    unsafeAtomicAdd(dst + 5, go);
-                    unsafeAtomicAdd(dst + 6, gr);
-                    unsafeAtomicAdd(dst + 7, gg);
-                    unsafeAtomicAdd(dst + 8, gbl);
-                }
-            }
-        }
-    }
+    blend_tile_backward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], (tile % grid_x) * TW, (tile / grid_x) * TH, width, height, bg0, bg1,
+        bg2, point_list, xy, conic_opacity, rgb, framebuffer, final_T, grad_fb, screen_grads);
 }
 
 namespace {
 
-// Per-Gaussian adjoint of the preprocess: conic -> covariance (2x2 inverse
-// adjoint), grad::splat<Ots>, grad::sh_to_colour, normalize.
-__global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64_t n, const float* __restrict__ means,
-    const float* __restrict__ scales, const float* __restrict__ rotations, const float* __restrict__ opacity, const float* __restrict__ sh,
-    int sh_coeffs, int sh_degree, float scale_multiplier, DevCamera<float> cam, const uint32_t* __restrict__ tiles,
-    const float4* __restrict__ conic_opacity, const uint8_t* __restrict__ clamped, const float* __restrict__ screen_grads,
-    float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations, float* __restrict__ g_opacity,
-    float* __restrict__ g_sh)
+__global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64_t n, SplatParams p, DevCamera<float> cam,
+    const uint32_t* __restrict__ tiles, const float4* __restrict__ conic_opacity, const uint8_t* __restrict__ clamped,
+    const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations,
+    float* __restrict__ g_opacity, float* __restrict__ g_sh)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    float* gsh = g_sh + i * sh_coeffs * 3;
-    if (tiles[i] == 0) {
-        for (int a = 0; a < 3; ++a)
-            g_means[3 * i + a] = g_scales[3 * i + a] = 0.f;
-        for (int a = 0; a < 4; ++a)
-            g_rotations[4 * i + a] = 0.f;
-        g_opacity[i] = 0.f;
-        for (int k = 0; k < sh_coeffs * 3; ++k)
-            gsh[k] = 0.f;
-        return;
-    }
-    const float* sg = screen_grads + i * 9;
-    const float mu[3] = { means[3 * i], means[3 * i + 1], means[3 * i + 2] };
-    const float m = scale_multiplier;
-    const float s[3] = { scales[3 * i] * m, scales[3 * i + 1] * m, scales[3 * i + 2] * m };
-    const float q[4] = { rotations[4 * i], rotations[4 * i + 1], rotations[4 * i + 2], rotations[4 * i + 3] };
-    // conic = cov^-1: dL/dcov = -A G A
-    const float4 A = conic_opacity[i];
-    const float G00 = sg[2], G01 = 0.5f * sg[3], G11 = sg[4];
-    const float t00 = A.x * G00 + A.y * G01, t01 = A.x * G01 + A.y * G11;
-    const float t10 = A.y * G00 + A.z * G01, t11 = A.y * G01 + A.z * G11;
-    const float r00 = t00 * A.x + t01 * A.y, r01 = t00 * A.y + t01 * A.z;
-    const float r10 = t10 * A.x + t11 * A.y, r11 = t10 * A.y + t11 * A.z;
-    const float g_cov2[3] = { -r00, -(r01 + r10), -r11 };
-    float g_w, g_mu[3], g_s[3], g_q[4];
-    splat_ots_grad(opacity[i], mu, s, q, cam, 0.3f, sg[5], sg, g_cov2, &g_w, g_mu, g_s, g_q);
-    // colour
-    const float v[3] = { mu[0] - cam.pos[0], mu[1] - cam.pos[1], mu[2] - cam.pos[2] };
-    const float len = sqrtf(dot3(v, v));
-    const float dir[3] = { v[0] / len, v[1] / len, v[2] / len };
-    const float gr[3] = { clamped[3 * i] ? sg[6] : 0.f, clamped[3 * i + 1] ? sg[7] : 0.f, clamped[3 * i + 2] ? sg[8] : 0.f };
-    float Y[16], dY[16][3];
-    sh_basis(sh_degree, dir, Y);
-    sh_basis_grad(sh_degree, dir, dY);
-    const float* c = sh + i * sh_coeffs * 3;
-    const int kmax = min(sh_coeffs, (sh_degree + 1) * (sh_degree + 1));
-    float g_dir[3] = { 0.f, 0.f, 0.f };
-    for (int k = 0; k < sh_coeffs; ++k) {
-        if (k < kmax) {
-            const float c0 = c[3 * k], c1 = c[3 * k + 1], c2 = c[3 * k + 2];
-            gsh[3 * k + 0] = Y[k] * gr[0];
-            gsh[3 * k + 1] = Y[k] * gr[1];
-            gsh[3 * k + 2] = Y[k] * gr[2];
-            const float dot_c = c0 * gr[0] + c1 * gr[1] + c2 * gr[2];
-            g_dir[0] = fmaf(dY[k][0], dot_c, g_dir[0]);
-            g_dir[1] = fmaf(dY[k][1], dot_c, g_dir[1]);
-            g_dir[2] = fmaf(dY[k][2], dot_c, g_dir[2]);
-        } else {
-            gsh[3 * k + 0] = gsh[3 * k + 1] = gsh[3 * k + 2] = 0.f;
-        }
-    }
-    const float dd = dot3(g_dir, dir);
+    float g_mu[3] = { 0.f, 0.f, 0.f }, g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f }, g_w = 0.f;
+    float g_shv[48];
+#pragma unroll
+    for (int k = 0; k < 48; ++k)
+        g_shv[k] = 0.f;
+    if (tiles[i] != 0)
+        preprocess_backward_gaussian(i, p, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9, g_mu, g_s, g_q, &g_w, g_shv);
     for (int a = 0; a < 3; ++a) {
-        g_means[3 * i + a] = g_mu[a] + (g_dir[a] - dd * dir[a]) / len;
-        g_scales[3 * i + a] = g_s[a] * m;
+        g_means[3 * i + a] = g_mu[a];
+        g_scales[3 * i + a] = g_s[a];
     }
     for (int a = 0; a < 4; ++a)
         g_rotations[4 * i + a] = g_q[a];
     g_opacity[i] = g_w;
+    float* gsh = g_sh + i * p.sh_coeffs * 3;
+#pragma unroll
+    for (int k = 0; k < 48; ++k)
+        if (k < p.sh_coeffs * 3)
+            gsh[k] = g_shv[k];
 }
 
+SplatParams make_params(const dgmr_gaussians* g)
+{
+    return SplatParams { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
+        static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier) };
+}
+
+} // namespace
+
+namespace {
+__global__ void k_view_keys(int64_t n, const uint32_t* __restrict__ ids, int64_t n_per_view, uint32_t* __restrict__ keys)
+{
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < n)
+        keys[j] = uint32_t(ids[j] / n_per_view);
+}
 } // namespace
 
 // ------------------------------------------------------------------ host side
@@ -572,26 +311,44 @@ void splat_blend_backward(const dgmr_tiling& t, int n_tiles, hipStream_t st, con
 // binning shared by the splatter and the marcher: depth sort, scan, emission,
 // pair sort, ranges.  depth_keys/ids are the workspace's A buffers, filled by
 // the preprocess.  Returns the pair count through *n_pairs (host).
-int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const float* xy_f32, int grid_x, int grid_y, int tile_w,
-    int tile_h, dgmr_binning* bins, Carver& cv, int64_t* n_pairs, hipStream_t st)
+int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, int grid_x, int grid_y, int tile_w,
+    int tile_h, dgmr_binning* bins, Carver& cv, int64_t* n_pairs, hipStream_t st, const BatchBinning* batch)
 {
+    const int4* views = batch ? batch->views : nullptr;
+    const int64_t n_per_view = batch ? batch->n_per_view : 0;
     SplatWorkspace w = carve_splat(cv, n, bins->pair_capacity);
     // stable sort of Gaussian ids by depth bits
     SortBuffers db { w.dkeys_a, w.dvals_a, w.dkeys_b, w.dvals_b };
-    DGMR_HIP_CHECK(radix_sort_pairs(db, n, nullptr, 0, 32, w.sort_scratch, st));
+    {
+        StageScope sc("bin.depth_sort", st, 16.0 * double(n) * radix_passes(0, 32));
+        DGMR_HIP_CHECK(radix_sort_pairs(db, n, nullptr, 0, 32, w.sort_scratch, st));
+    }
     const uint32_t* sorted_ids = (radix_passes(0, 32) % 2 == 0) ? w.dvals_a : w.dvals_b;
+    if (batch && batch->n_views > 1) {
+        // stable pass on the view index: order becomes (view, depth bits, id)
+        StageScope sc("bin.view_sort", st, 16.0 * double(n));
+        k_view_keys<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, sorted_ids, n_per_view, w.dkeys_a);
+        DGMR_HIP_CHECK(hipGetLastError());
+        SortBuffers vb { w.dkeys_a, w.dvals_a, w.dkeys_b, w.dvals_b };
+        const int vbits = int(higher_msb(uint32_t(batch->n_views - 1)));
+        DGMR_HIP_CHECK(radix_sort_pairs(vb, n, nullptr, 0, vbits, w.sort_scratch, st));
+        sorted_ids = (radix_passes(0, vbits) % 2 == 0) ? w.dvals_a : w.dvals_b;
+    }
     const unsigned blocks_n = unsigned(ceil_div64(n, 256));
-    k_gather_counts<<<blocks_n, 256, 0, st>>>(n, sorted_ids, tiles, w.counts);
-    DGMR_HIP_CHECK(hipGetLastError());
-    DGMR_HIP_CHECK(scan_u32(w.counts, w.pair_offsets, w.counter, n, false, w.scan_scratch, st));
-    // the reference's cache keeps the inclusive scan in Gaussian order
-    if (bins->point_offsets)
-        DGMR_HIP_CHECK(scan_u32(tiles, bins->point_offsets, nullptr, n, true, w.scan_scratch, st));
+    {
+        StageScope sc("bin.scan", st, 24.0 * double(n));
+        k_gather_counts<<<blocks_n, 256, 0, st>>>(n, sorted_ids, tiles, w.counts);
+        DGMR_HIP_CHECK(hipGetLastError());
+        DGMR_HIP_CHECK(scan_u32(w.counts, w.pair_offsets, w.counter, n, false, w.scan_scratch, st));
+        // the reference's cache keeps the inclusive scan in Gaussian order
+        if (bins->point_offsets)
+            DGMR_HIP_CHECK(scan_u32(tiles, bins->point_offsets, nullptr, n, true, w.scan_scratch, st));
+    }
     uint32_t R = 0;
     DGMR_HIP_CHECK(hipMemcpyAsync(&R, w.counter, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     DGMR_HIP_CHECK(hipStreamSynchronize(st));
     *n_pairs = int64_t(R);
-    const int64_t n_tiles = int64_t(grid_x) * grid_y;
+    const int64_t n_tiles = batch ? batch->total_tiles : int64_t(grid_x) * grid_y;
     DGMR_HIP_CHECK(hipMemsetAsync(bins->ranges, 0, size_t(n_tiles) * 2 * sizeof(uint32_t), st));
     if (R == 0)
         return DGMR_OK;
@@ -608,13 +365,26 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     else
         pb = SortBuffers { w.pkeys_a, w.pvals_a, w.pkeys_b, bins->point_list };
     const unsigned emit_blocks = unsigned(ceil_div64(n, 256));
-    k_emit_pairs<<<emit_blocks, 256, 0, st>>>(n, sorted_ids, w.pair_offsets, rects, tiles, xy_f32, grid_x, grid_y, tile_w, tile_h, pb.keys_a,
-        pb.vals_a);
-    DGMR_HIP_CHECK(hipGetLastError());
-    DGMR_HIP_CHECK(radix_sort_pairs(pb, int64_t(R), nullptr, 0, tile_bits, w.sort_scratch, st));
+    {
+        StageScope sc("bin.emit", st, 8.0 * double(R) + 28.0 * double(n));
+        if (xy_double)
+            k_emit_pairs<double><<<emit_blocks, 256, 0, st>>>(n, sorted_ids, w.pair_offsets, rects, tiles, static_cast<const double*>(xy), grid_x,
+                grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
+        else
+            k_emit_pairs<float><<<emit_blocks, 256, 0, st>>>(n, sorted_ids, w.pair_offsets, rects, tiles, static_cast<const float*>(xy), grid_x,
+                grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
+    {
+        StageScope sc("bin.pair_sort", st, 20.0 * double(R) * passes);
+        DGMR_HIP_CHECK(radix_sort_pairs(pb, int64_t(R), nullptr, 0, tile_bits, w.sort_scratch, st));
+    }
     const uint32_t* sorted_keys = (passes % 2 == 0) ? pb.keys_a : pb.keys_b;
-    k_tile_ranges<<<unsigned(ceil_div64(R, 256)), 256, 0, st>>>(sorted_keys, w.counter, bins->ranges);
-    DGMR_HIP_CHECK(hipGetLastError());
+    {
+        StageScope sc("bin.ranges", st, 4.0 * double(R) + 8.0 * double(n_tiles));
+        k_tile_ranges<<<unsigned(ceil_div64(R, 256)), 256, 0, st>>>(sorted_keys, w.counter, bins->ranges);
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
     return DGMR_OK;
 }
 
@@ -672,22 +442,27 @@ int dgmr_splat_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dg
     }
     uint32_t *dkeys, *dids;
     binning_key_buffers(workspace, n, &dkeys, &dids);
-    k_splat_preprocess<<<unsigned(ceil_div64(n, kPreThreads)), kPreThreads, 0, st>>>(n, static_cast<const float*>(g->means),
-        static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations), static_cast<const float*>(g->opacity),
-        static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), dc, geom->rects, geom->tiles_touched,
-        geom->xy, reinterpret_cast<float4*>(geom->conic_opacity), geom->rgb, geom->rgb_clamped, geom->depths, dkeys, dids);
+    {
+    StageScope sc("splat.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs + 67.0));
+    k_splat_preprocess<<<unsigned(ceil_div64(n, kPreThreads)), kPreThreads, 0, st>>>(n, make_params(g), dc,
+        SplatOut { geom->rects, geom->tiles_touched, geom->xy, reinterpret_cast<float4*>(geom->conic_opacity), geom->rgb, geom->rgb_clamped,
+            geom->depths, dkeys, dids });
     DGMR_HIP_CHECK(hipGetLastError());
+    }
     Carver cv(workspace, workspace_bytes);
     int64_t R = 0;
-    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv, &R, st);
+    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, false, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv, &R, st,
+        nullptr);
     if (stats)
         stats->n_pairs = R;
     if (rc != DGMR_OK)
         return rc;
-    (void)plane;
-    splat_blend_forward(*tiling, int(n_tiles), st, bins->ranges, bins->point_list, geom->xy, geom->conic_opacity, geom->rgb, cam->width,
-        cam->height, dc.grid_x, cam->background, framebuffer, final_transmittance);
-    DGMR_HIP_CHECK(hipGetLastError());
+    {
+        StageScope sc("splat.blend_forward", st, 40.0 * double(R) + 16.0 * double(plane) + 8.0 * double(n_tiles));
+        splat_blend_forward(*tiling, int(n_tiles), st, bins->ranges, bins->point_list, geom->xy, geom->conic_opacity, geom->rgb, cam->width,
+            cam->height, dc.grid_x, cam->background, framebuffer, final_transmittance);
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
     return DGMR_OK;
 }
 
@@ -708,13 +483,13 @@ int dgmr_splat_backward(const dgmr_gaussians* g, const dgmr_camera* cam, const d
     const int64_t n_tiles = int64_t(dc.grid_x) * dc.grid_y;
     DGMR_HIP_CHECK(hipMemsetAsync(screen_grads, 0, size_t(n) * 9 * sizeof(float), st));
     if (n_pairs > 0) {
+        StageScope sc("splat.blend_backward", st, 40.0 * double(n_pairs) + 28.0 * double(int64_t(cam->width) * cam->height) + 8.0 * double(n_tiles));
         splat_blend_backward(*tiling, int(n_tiles), st, bins->ranges, bins->point_list, geom->xy, geom->conic_opacity, geom->rgb, cam->width,
             cam->height, dc.grid_x, cam->background, framebuffer, final_transmittance, grad_framebuffer, screen_grads);
         DGMR_HIP_CHECK(hipGetLastError());
     }
-    k_splat_preprocess_backward<<<unsigned(ceil_div64(n, kPreThreads)), kPreThreads, 0, st>>>(n, static_cast<const float*>(g->means),
-        static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations), static_cast<const float*>(g->opacity),
-        static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), dc, geom->tiles_touched,
+    StageScope sc("splat.preprocess_backward", st, double(n) * (2.0 * (44.0 + 12.0 * g->sh_coeffs) + 59.0));
+    k_splat_preprocess_backward<<<unsigned(ceil_div64(n, kPreThreads)), kPreThreads, 0, st>>>(n, make_params(g), dc, geom->tiles_touched,
         reinterpret_cast<const float4*>(geom->conic_opacity), geom->rgb_clamped, screen_grads, grad_means, grad_scales, grad_rotations,
         grad_opacity, grad_sh);
     DGMR_HIP_CHECK(hipGetLastError());

@@ -24,6 +24,9 @@ import torch
 from .. import _common as C
 from .. import _native as N
 
+# tile shape and LDS batch used when the caller passes none
+DEFAULT_TILING = ((16, 16), 256)
+
 # cache layout returned by forward (index: content)
 CACHE_FIELDS = ("framebuffer", "rects", "rgb", "rgb_sh_clamped", "depths", "points_xy_image", "conic_opacity",
                 "tiles_touched", "point_offsets", "i_ranges", "b_point_list", "final_transparency", "n_pairs")
@@ -31,7 +34,9 @@ CACHE_FIELDS = ("framebuffer", "rects", "rgb", "rgb_sh_clamped", "depths", "poin
 
 def _forward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, proj_matrix, cam_position, background,
              render_width: int, render_height: int, tan_fovx: float, tan_fovy: float, sh_degree: int,
-             tile=(16, 16), batch: int = 256, scale_multiplier: float = 1.0):
+             tile=None, batch: int = 0, scale_multiplier: float = 1.0):
+    tile = tile or DEFAULT_TILING[0]
+    batch = batch or DEFAULT_TILING[1]
     shs, weights, centroids, cov_scales, cov_rotations = C.prepare_inputs(shs, weights, centroids, cov_scales, cov_rotations, True)
     dev = centroids.device
     n = centroids.shape[0]
@@ -82,7 +87,9 @@ def _forward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, pr
 
 def _backward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, proj_matrix, cam_position, background,
               render_width: int, render_height: int, tan_fovx: float, tan_fovy: float, sh_degree: int, grad_framebuffer, cache,
-              tile=(16, 16), batch: int = 256, scale_multiplier: float = 1.0):
+              tile=None, batch: int = 0, scale_multiplier: float = 1.0):
+    tile = tile or DEFAULT_TILING[0]
+    batch = batch or DEFAULT_TILING[1]
     shs, weights, centroids, cov_scales, cov_rotations = C.prepare_inputs(shs, weights, centroids, cov_scales, cov_rotations, True)
     fb, rects, rgb, clamped, depths, xy, conic, tiles, offsets, ranges, point_list, T, n_pairs = cache
     dev = centroids.device

@@ -155,24 +155,46 @@ int dgmr_march_backward(const dgmr_gaussians* g, const dgmr_camera* cam, const d
     void* grad_means, void* grad_scales, void* grad_rotations, void* grad_opacity, void* grad_sh, dgmr_stream stream);
 
 /* ------------------------------------------------------------ batched views
- * Many cameras of one scene in one persistent launch per stage (cameras of
- * uneven resolution).  cams[i] renders into framebuffers[i] / transmittances[i]. */
+ * Many cameras of one scene (uneven resolutions) rendered together: one
+ * preprocess launch over all (view, Gaussian) pairs, one binning pass over the
+ * global tile range of all views, and one persistent blend launch whose
+ * workgroups pull tiles of every view from a device queue.  No counterpart in
+ * the reference (one camera per call there, vol_marcher.py:19-65).
+ * The workspace carries the per-view geometry and binning from the forward to
+ * the backward: keep it untouched in between. */
 typedef struct dgmr_batch_view {
     dgmr_camera cam;
-    float* framebuffer;           /* [3,H,W] */
-    float* final_transmittance;   /* [H,W]   */
-    const float* grad_framebuffer;/* [3,H,W] for the backward, else NULL */
+    float* framebuffer;           /* [3,H,W] output of the forward */
+    float* final_transmittance;   /* [H,W]   output of the forward */
+    const float* grad_framebuffer;/* [3,H,W] input of the backward */
 } dgmr_batch_view;
 
 int dgmr_batch_workspace_size(int64_t n, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,
     int64_t pair_capacity, size_t* bytes);
 
-/* Splat forward (and, when every view has grad_framebuffer, backward) of all
- * views.  Gradients of all views are summed into grad_* (overwritten). */
-int dgmr_splat_batch(const dgmr_gaussians* g, dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,
-    void* workspace, size_t workspace_bytes, int64_t pair_capacity,
-    float* grad_means, float* grad_scales, float* grad_rotations, float* grad_opacity, float* grad_sh,
-    dgmr_stats* stats, dgmr_stream stream);
+int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,
+    void* workspace, size_t workspace_bytes, int64_t pair_capacity, dgmr_stats* stats, dgmr_stream stream);
+
+/* Gradients of all views are summed into grad_* (overwritten). */
+int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,
+    void* workspace, size_t workspace_bytes, int64_t pair_capacity, int64_t n_pairs,
+    float* grad_means, float* grad_scales, float* grad_rotations, float* grad_opacity, float* grad_sh, dgmr_stream stream);
+
+/* ------------------------------------------------------------ stage timing
+ * When enabled, every entry point brackets each kernel stage with hipEvents on
+ * its stream (no extra synchronisation).  dgmr_profile_read waits for the
+ * recorded events and returns per-stage totals; names are static strings.
+ * No counterpart in the reference (its timers are commented out,
+ * sibr_interfaces.cpp:33-43). */
+typedef struct dgmr_stage_time {
+    const char* name;
+    double total_ms;
+    int64_t launches;
+    double bytes;             /* algorithmic bytes summed over launches (0 if not modelled) */
+} dgmr_stage_time;
+
+int dgmr_profile_enable(int on);
+int dgmr_profile_read(dgmr_stage_time* out, int32_t max_stages, int32_t* n_stages);
 
 /* ------------------------------------------------------------ misc */
 const char* dgmr_last_error(void);

@@ -1,0 +1,211 @@
+"""GPU parity of the volumetric ray marcher against the host oracle.
+
+Bars:
+  * binning (rects from the iso-line of 1/255, tiles, point offsets, sorted
+    point list, tile ranges): bit-exact against the restatement of the same
+    precision;
+  * double GPU vs double oracle: image and gradients within 1e-9 (relative to
+    the largest magnitude), i.e. the same algorithm up to libm rounding;
+  * float GPU vs double oracle (float binning): image within 1e-3 except on a
+    bounded fraction of pixels where float-vs-double sample placement differs;
+  * double gradients against central finite differences of the GPU forward on
+    fixed sample sections (the reference's gradient-check mode), 32 random
+    parameters, relative error <= 1e-4;
+  * float gradients: L2-relative <= 2e-3 against the float restatement and
+    <= 2e-2 against the double oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not gpu_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def small_scene(n=3000, extent=1.5, w=64, h=48, seed=3):
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    scene = scenes.make_scene(n, seed=seed, extent=extent, sh_degree=3)
+    scene.scales = (scene.scales * 2.0).astype(np.float32)
+    t = 0.5
+    cam = scenes.look_at_camera([0.2, -0.1, -6.0], [0.0, 0.0, 0.0], w, h, t * w / h, t)
+    return scene, cam
+
+
+def run_gpu(scene, cam, bg, dtype, grad_fb=None, max_large_steps=0, frozen=None, tile=(16, 16)):
+    torch = _torch()
+    from differentiable_gaussian_mixture_renderer_amd.python_bindings import vol_marcher as vm
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev, dtype=dtype)
+    args = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations)]
+    cam_args = [t(cam.world_view_transform), t(cam.full_proj_transform), t(cam.camera_center), t(bg), cam.width, cam.height,
+                cam.tan_fovx, cam.tan_fovy, scene.sh_degree]
+    print(f"[gpu march] n={scene.n} {cam.width}x{cam.height} {dtype} frozen={frozen is not None}", flush=True)
+    cache = vm.cpp_binding.forward(*args, *cam_args, tile=tile, max_large_steps=max_large_steps, frozen=frozen)
+    names = ("framebuffer", "rects", "rgb", "clamped", "depths", "xy", "inv_cov", "masses", "tiles", "offsets", "ranges",
+             "point_list", "remaining")
+    out = {k: v.cpu().numpy() for k, v in zip(names, cache)}
+    if grad_fb is not None:
+        grads = vm.cpp_binding.backward(*args, *cam_args, t(grad_fb), cache, tile=tile, max_large_steps=max_large_steps, frozen=frozen)
+        out["grads"] = dict(zip(("sh", "opacity", "means", "scales", "rotations"), [g.cpu().numpy() for g in grads]))
+    torch.cuda.synchronize()
+    return out
+
+
+def check_binning(out, ref):
+    g = ref.geometry()
+    b = ref.binning()
+    vis = g["tiles"] > 0
+    np.testing.assert_array_equal(out["tiles"].view(np.uint32), g["tiles"])
+    np.testing.assert_array_equal(out["rects"][vis], g["rects"][vis])
+    np.testing.assert_array_equal(out["offsets"].view(np.uint32), b["point_offsets"])
+    np.testing.assert_array_equal(out["point_list"].view(np.uint32), b["point_list"])
+    np.testing.assert_array_equal(out["ranges"].view(np.uint32), b["ranges"])
+    return vis
+
+
+def test_forward_double_matches_double_oracle():
+    torch = _torch()
+    scene, cam = small_scene()
+    bg = np.array([0.2, 0.3, 0.4])
+    out = run_gpu(scene, cam, bg, torch.float64)
+    ref = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=False)
+    vis = check_binning(out, ref)
+    assert vis.sum() > 100
+    g = ref.geometry()
+    np.testing.assert_allclose(out["inv_cov"][vis], g["inv_cov"][vis], rtol=1e-12)
+    np.testing.assert_allclose(out["masses"][vis], g["mass"][vis], rtol=1e-12)
+    fb, T, nl = ref.render(bg)
+    err = np.abs(out["framebuffer"] - fb).max()
+    print(f"double: max image err {err:.3g}, large steps max {nl.max()}")
+    assert err <= 1e-9
+    assert np.abs(out["remaining"] - T).max() <= 1e-9
+
+
+def test_forward_float_binning_and_image():
+    torch = _torch()
+    scene, cam = small_scene()
+    bg = np.array([0.2, 0.3, 0.4], np.float32)
+    out = run_gpu(scene, cam, bg, torch.float32)
+    ref32 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=False)
+    check_binning(out, ref32)
+    ref64 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=True)
+    fb, T, _ = ref64.render(bg)
+    err = np.abs(out["framebuffer"] - fb).max(axis=0)
+    frac = float((err > 1e-3).mean())
+    print(f"float vs double: median {np.median(err):.3g}, p99 {np.quantile(err, 0.99):.3g}, >1e-3: {frac:.4f}")
+    assert np.median(err) <= 1e-5
+    assert frac <= 0.02
+
+
+def test_backward_double_matches_double_oracle():
+    torch = _torch()
+    scene, cam = small_scene(n=1500, w=48, h=32)
+    bg = np.array([0.2, 0.3, 0.4])
+    ref = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=False)
+    fb, T, _ = ref.render(bg)
+    rng = np.random.default_rng(3)
+    grad = rng.normal(size=fb.shape) / fb.size
+    out = run_gpu(scene, cam, bg, torch.float64, grad_fb=grad)
+    r = ref.backward(bg, fb, T, grad)
+    n, k = scene.n, scene.sh.shape[1]
+    for name, got, want in [("means", out["grads"]["means"], r["means"]), ("scales", out["grads"]["scales"], r["scales"]),
+                            ("rotations", out["grads"]["rotations"], r["rotations"]),
+                            ("opacity", out["grads"]["opacity"].reshape(n), r["opacity"]), ("sh", out["grads"]["sh"], r["sh"][:, :k])]:
+        scale = np.abs(want).max()
+        assert scale > 0, name
+        rel = np.abs(got - want).max() / scale
+        print(f"{name}: max rel {rel:.3g}")
+        assert rel <= 1e-9, name
+
+
+def test_backward_against_finite_differences():
+    """Frozen sample sections make the forward smooth (vol_marcher.h:25); 32
+    random parameters are perturbed (in float64) by central differences of the
+    GPU forward and compared with the GPU backward."""
+    torch = _torch()
+    scene, cam = small_scene(n=400, extent=1.0, w=32, h=32, seed=5)
+    for name in ("means", "scales", "rotations", "opacity", "sh"):
+        setattr(scene, name, getattr(scene, name).astype(np.float64))
+    frozen = [(3.5, 6.0, 2.5 / 31), (6.0, 8.5, 2.5 / 31)]
+    bg = np.array([0.2, 0.3, 0.4])
+    rng = np.random.default_rng(11)
+    out0 = run_gpu(scene, cam, bg, torch.float64, frozen=frozen)
+    grad = rng.normal(size=out0["framebuffer"].shape)
+    out = run_gpu(scene, cam, bg, torch.float64, grad_fb=grad, frozen=frozen)
+    names = ["means", "scales", "rotations", "opacity", "sh"]
+    vis = np.nonzero(out0["tiles"] > 0)[0]
+    errs = []
+    for _ in range(32):
+        name = names[rng.integers(len(names))]
+        i = int(vis[rng.integers(len(vis))])
+        base = getattr(scene, name)
+        j = int(rng.integers(3 if name == "sh" else base[i].size))
+        v = float(base[i].reshape(-1)[j])
+        h = 1e-5 * max(1.0, abs(v))
+        vals = []
+        for sgn in (1, -1):
+            pert = base.copy()
+            pert[i].reshape(-1)[j] = v + sgn * h
+            setattr(scene, name, pert)
+            vals.append(float((run_gpu(scene, cam, bg, torch.float64, frozen=frozen)["framebuffer"] * grad).sum()))
+        setattr(scene, name, base)
+        fd = (vals[0] - vals[1]) / (2 * h)
+        errs.append((name, fd, float(out["grads"][name][i].reshape(-1)[j])))
+    scale = max(abs(fd) for _, fd, _ in errs)
+    worst = max(abs(fd - an) for _, fd, an in errs) / scale
+    print("worst relative FD error", worst)
+    assert worst <= 1e-4, errs
+
+
+def test_backward_float_matches_oracle():
+    """float GPU gradients against the float restatement (same decisions) and
+    the double oracle (float binning): relative L2 error per parameter tensor."""
+    torch = _torch()
+    scene, cam = small_scene(n=1500, w=48, h=32)
+    bg = np.array([0.2, 0.3, 0.4], np.float32)
+    rng = np.random.default_rng(5)
+    ref32 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=False)
+    fb32, T32, _ = ref32.render(bg)
+    grad = (rng.normal(size=fb32.shape) / fb32.size).astype(np.float32)
+    out = run_gpu(scene, cam, bg, torch.float32, grad_fb=grad)
+    r32 = ref32.backward(bg, fb32, T32, grad)
+    ref64 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=True)
+    fb64, T64, _ = ref64.render(bg)
+    r64 = ref64.backward(bg, fb64, T64, grad.astype(np.float64))
+    n, k = scene.n, scene.sh.shape[1]
+    for name in ("means", "scales", "rotations", "opacity", "sh"):
+        got = out["grads"][name].reshape(n, -1)
+        for ref, tol, label in ((r32, 2e-3, "f32 oracle"), (r64, 2e-2, "f64 oracle")):
+            want = ref[name].reshape(n, -1)[:, :got.shape[1]]
+            rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+            print(f"{name} vs {label}: l2 rel {rel:.3g}")
+            assert rel <= tol, (name, label)
+
+
+def test_volmarcher_autograd_and_large_step_cap():
+    torch = _torch()
+    from differentiable_gaussian_mixture_renderer_amd.python_bindings import vol_marcher as vm
+    scene, cam = small_scene(n=800, w=32, h=32)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    params = [t(scene.sh).requires_grad_(), t(scene.opacity).requires_grad_(), t(scene.means).requires_grad_(),
+              t(scene.scales).requires_grad_(), t(scene.rotations).requires_grad_()]
+    img = vm.apply(*params, t(cam.world_view_transform), t(cam.full_proj_transform), t(cam.camera_center),
+                   t(np.zeros(3, np.float32)), cam.width, cam.height, cam.tan_fovx, cam.tan_fovy, 3)
+    img.sum().backward()
+    assert all(p.grad is not None and torch.isfinite(p.grad).all() for p in params)
+    # a cap of 2 large steps (<= 254 bins per ray) matches the oracle under the same cap
+    bg = np.zeros(3, np.float32)
+    out = run_gpu(scene, cam, bg, torch.float64, max_large_steps=2)
+    ref = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=False)
+    fb, _, nl = ref.render(bg, max_large_steps=2)
+    assert nl.max() <= 2
+    assert np.abs(out["framebuffer"] - fb).max() <= 1e-9
