@@ -128,7 +128,7 @@ SIGNATURES = {
     "dgmr_march_backward": (ctypes.c_int, [_P(Gaussians), _P(Camera), _P(Tiling), _P(MarchOptions), _P(MarchGeometry), _P(Binning),
                                            ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p]),
-    "dgmr_batch_workspace_size": (ctypes.c_int, [_P(Gaussians), _P(BatchView), ctypes.c_int32, _P(Tiling), ctypes.c_int64,
+    "dgmr_batch_workspace_size": (ctypes.c_int, [ctypes.c_int64, _P(BatchView), ctypes.c_int32, _P(Tiling), ctypes.c_int64,
                                                  _P(ctypes.c_size_t)]),
     "dgmr_splat_batch_forward": (ctypes.c_int, [_P(Gaussians), _P(BatchView), ctypes.c_int32, _P(Tiling), c_void_p, ctypes.c_size_t,
                                                 ctypes.c_int64, _P(Stats), c_void_p]),

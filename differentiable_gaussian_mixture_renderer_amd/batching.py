@@ -1,0 +1,129 @@
+"""Batched multi-view splatting through one persistent launch per stage.
+
+`BatchSplatter` renders one scene from many cameras of uneven resolution with
+the C ABI's dgmr_splat_batch_forward / dgmr_splat_batch_backward: one
+preprocess launch over all (view, Gaussian) pairs, one binning pass over the
+global tile range, one persistent blend launch pulling tiles of every view from
+a device queue.  The workspace carries geometry and binning from the forward to
+the backward.
+
+`render_plain` is the one-camera-per-launch path (a Splatter call per view)
+that the batched path is measured against.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Sequence
+
+import torch
+
+from . import _common as C
+from . import _native as N
+
+
+def _cam_struct(cam, bg):
+    return N.camera_struct(cam.world_view_transform, cam.full_proj_transform, cam.camera_center, bg, cam.tan_fovx, cam.tan_fovy,
+                           cam.width, cam.height)
+
+
+class BatchSplatter:
+    """Forward and backward of one scene over a list of cameras (scenes.Camera)."""
+
+    def __init__(self, tile=(16, 16), batch: int = 256, device="cuda:0"):
+        self.tiling = C.tiling_struct(tile, batch)
+        self.device = torch.device(device)
+        self.capacity = 0
+        self._ws = None
+        self._views = None
+        self._n_pairs = 0
+        self._keep = []
+
+    def _view_array(self, cams, bg, fbs, Ts, grads=None):
+        arr = (N.BatchView * len(cams))()
+        for v, cam in enumerate(cams):
+            arr[v].cam = _cam_struct(cam, bg)
+            arr[v].framebuffer = fbs[v].data_ptr()
+            arr[v].final_transmittance = Ts[v].data_ptr()
+            arr[v].grad_framebuffer = grads[v].data_ptr() if grads is not None else None
+        return arr
+
+    def forward(self, shs, opacity, means, scales, rotations, cameras: Sequence, bg=(0.0, 0.0, 0.0), sh_degree: int = 3):
+        shs, opacity, means, scales, rotations = C.prepare_inputs(shs, opacity, means, scales, rotations, True)
+        dev = means.device
+        n = means.shape[0]
+        lib = N.load()
+        self._g = C.gaussians_struct(shs, opacity, means, scales, rotations, sh_degree)
+        self._keep = [shs, opacity, means, scales, rotations]
+        self.cameras = list(cameras)
+        self.bg = tuple(float(b) for b in bg)
+        fbs = [torch.empty((3, c.height, c.width), dtype=torch.float32, device=dev) for c in self.cameras]
+        Ts = [torch.empty((c.height, c.width), dtype=torch.float32, device=dev) for c in self.cameras]
+        views = self._view_array(self.cameras, self.bg, fbs, Ts)
+        cap = self.capacity or C.initial_capacity(dev, "batch", 4 * n * len(self.cameras))
+        stats = N.Stats()
+        stream = C.stream_handle(dev)
+        with torch.cuda.device(dev):
+            for _attempt in range(3):
+                size = ctypes.c_size_t(0)
+                N.check(lib.dgmr_batch_workspace_size(n, views, len(self.cameras), ctypes.byref(self.tiling), cap, ctypes.byref(size)),
+                        "dgmr_batch_workspace_size")
+                if self._ws is None or self._ws.numel() < size.value:
+                    self._ws = None
+                    self._ws = torch.empty(int(size.value * 1.05) + 256, dtype=torch.uint8, device=dev)
+                rc = lib.dgmr_splat_batch_forward(ctypes.byref(self._g), views, len(self.cameras), ctypes.byref(self.tiling),
+                                                  self._ws.data_ptr(), self._ws.numel(), cap, ctypes.byref(stats), stream)
+                if rc == N.DGMR_ERR_CAPACITY:
+                    cap = int(stats.n_pairs * 1.25) + 4096
+                    continue
+                N.check(rc, "dgmr_splat_batch_forward")
+                break
+            else:
+                raise N.DgmrError(N.DGMR_ERR_CAPACITY, "dgmr_splat_batch_forward", "pair capacity kept growing")
+        self.capacity = cap
+        C.remember_capacity(dev, "batch", cap)
+        self._n_pairs = int(stats.n_pairs)
+        self._fbs, self._Ts = fbs, Ts
+        return fbs, Ts
+
+    @property
+    def n_pairs(self) -> int:
+        return self._n_pairs
+
+    def backward(self, grad_images: List[torch.Tensor]):
+        shs, opacity, means, scales, rotations = self._keep
+        dev = means.device
+        n = means.shape[0]
+        grads = [g.to(torch.float32).contiguous() for g in grad_images]
+        views = self._view_array(self.cameras, self.bg, self._fbs, self._Ts, grads)
+        f32 = dict(dtype=torch.float32, device=dev)
+        g_means = torch.empty((n, 3), **f32)
+        g_scales = torch.empty((n, 3), **f32)
+        g_rot = torch.empty((n, 4), **f32)
+        g_op = torch.empty(opacity.shape, **f32)
+        g_sh = torch.empty(shs.shape, **f32)
+        with torch.cuda.device(dev):
+            rc = N.load().dgmr_splat_batch_backward(ctypes.byref(self._g), views, len(self.cameras), ctypes.byref(self.tiling),
+                                                    self._ws.data_ptr(), self._ws.numel(), self.capacity, self._n_pairs,
+                                                    g_means.data_ptr(), g_scales.data_ptr(), g_rot.data_ptr(), g_op.data_ptr(),
+                                                    g_sh.data_ptr(), C.stream_handle(dev))
+            N.check(rc, "dgmr_splat_batch_backward")
+        return g_sh, g_op, g_means, g_scales, g_rot
+
+
+def render_plain(shs, opacity, means, scales, rotations, cameras: Sequence, bg=(0.0, 0.0, 0.0), sh_degree: int = 3,
+                 grad_fn=None, tile=(16, 16), batch: int = 256):
+    """One Splatter forward (and backward when grad_fn maps image -> dL/dimage)
+    per camera; gradients summed over views."""
+    from .python_bindings import splatter
+    bg_t = torch.tensor(bg, dtype=torch.float32)
+    images = []
+    acc = None
+    for cam in cameras:
+        args = (shs, opacity, means, scales, rotations, torch.tensor(cam.world_view_transform), torch.tensor(cam.full_proj_transform),
+                torch.tensor(cam.camera_center), bg_t, cam.width, cam.height, cam.tan_fovx, cam.tan_fovy, sh_degree)
+        cache = splatter.cpp_binding.forward(*args, tile=tile, batch=batch)
+        images.append(cache[0])
+        if grad_fn is not None:
+            g = splatter.cpp_binding.backward(*args, grad_fn(cache[0]), cache, tile=tile, batch=batch)
+            acc = g if acc is None else [a + b for a, b in zip(acc, g)]
+    return images, acc

@@ -9,6 +9,7 @@
 // whose workgroups take the next tile of any view from a device counter, so
 // small and large views share the chip without per-view launches or syncs.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "host_common.h"
@@ -26,12 +27,19 @@ struct ViewBlend {
     const float* grad_fb;
 };
 
-__global__ __launch_bounds__(256) void k_batch_preprocess(int64_t n, SplatParams p, const DevCamera<float>* __restrict__ cams, SplatOut out)
+// thread per Gaussian: its parameters are read once and projected into every view
+__global__ __launch_bounds__(256) void k_batch_preprocess(int64_t n, int n_views, SplatParams p, const DevCamera<float>* __restrict__ cams,
+    SplatOut out)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int v = blockIdx.y;
-    if (i < n)
-        preprocess_gaussian(i, int64_t(v) * n + i, p, cams[v], out);
+    __shared__ float s_sh[256 * kShStride];
+    const int64_t first = int64_t(blockIdx.x) * blockDim.x;
+    stage_sh(s_sh, p.sh, first, int(min<int64_t>(blockDim.x, n - first)), p.sh_coeffs);
+    __syncthreads();
+    const int64_t i = first + threadIdx.x;
+    if (i >= n)
+        return;
+    for (int v = 0; v < n_views; ++v)
+        preprocess_gaussian(i, int64_t(v) * n + i, p, s_sh + threadIdx.x * kShStride, cams[v], out);
 }
 
 __device__ __forceinline__ int find_view(const ViewBlend* views, int n_views, int tile)
@@ -54,7 +62,7 @@ __global__ __launch_bounds__(TW* TH) void k_batch_blend(const uint32_t* __restri
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, const ViewBlend* __restrict__ views,
     int n_views, int total_tiles, uint32_t* __restrict__ queue, float* __restrict__ screen_grads)
 {
-    __shared__ BlendSmem<B> sm;
+    __shared__ typename std::conditional<BACKWARD, BlendSmemBwd<B>, BlendSmem<B>>::type sm;
     __shared__ int s_tile;
     while (true) {
         if (threadIdx.x == 0)
@@ -67,7 +75,7 @@ __global__ __launch_bounds__(TW* TH) void k_batch_blend(const uint32_t* __restri
         const ViewBlend& vb = views[find_view(views, n_views, tile)];
         const int local = tile - vb.tile_base;
         const int px0 = (local % vb.grid_x) * TW, py0 = (local / vb.grid_x) * TH;
-        if (BACKWARD)
+        if constexpr (BACKWARD)
             blend_tile_backward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
                 vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T, vb.grad_fb, screen_grads);
         else
@@ -83,31 +91,40 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
     const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations,
     float* __restrict__ g_opacity, float* __restrict__ g_sh)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
+    __shared__ float s_sh[256 * kShStride];
+    const int64_t first = int64_t(blockIdx.x) * blockDim.x;
+    const int count = int(min<int64_t>(blockDim.x, n - first));
+    stage_sh(s_sh, p.sh, first, count, p.sh_coeffs);
+    __syncthreads();
+    const int64_t i = first + threadIdx.x;
     float g_mu[3] = { 0.f, 0.f, 0.f }, g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f }, g_w = 0.f;
     float g_shv[48];
 #pragma unroll
     for (int k = 0; k < 48; ++k)
         g_shv[k] = 0.f;
-    for (int v = 0; v < n_views; ++v) {
-        const int64_t o = int64_t(v) * n + i;
-        if (tiles[o] != 0)
-            preprocess_backward_gaussian(i, p, cams[v], conic_opacity[o], clamped + 3 * o, screen_grads + o * 9, g_mu, g_s, g_q, &g_w, g_shv);
+    float* row = s_sh + threadIdx.x * kShStride;
+    if (i < n) {
+        for (int v = 0; v < n_views; ++v) {
+            const int64_t o = int64_t(v) * n + i;
+            if (tiles[o] != 0)
+                preprocess_backward_gaussian(i, p, row, cams[v], conic_opacity[o], clamped + 3 * o, screen_grads + o * 9, g_mu, g_s, g_q, &g_w,
+                    g_shv);
+        }
+        for (int a = 0; a < 3; ++a) {
+            g_means[3 * i + a] = g_mu[a];
+            g_scales[3 * i + a] = g_s[a];
+        }
+        for (int a = 0; a < 4; ++a)
+            g_rotations[4 * i + a] = g_q[a];
+        g_opacity[i] = g_w;
     }
-    for (int a = 0; a < 3; ++a) {
-        g_means[3 * i + a] = g_mu[a];
-        g_scales[3 * i + a] = g_s[a];
-    }
-    for (int a = 0; a < 4; ++a)
-        g_rotations[4 * i + a] = g_q[a];
-    g_opacity[i] = g_w;
-    float* gsh = g_sh + i * p.sh_coeffs * 3;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < 48; ++k)
         if (k < p.sh_coeffs * 3)
-            gsh[k] = g_shv[k];
+            row[k] = g_shv[k];
+    __syncthreads();
+    unstage_sh(g_sh, s_sh, first, count, p.sh_coeffs);
 }
 
 // Workspace layout shared by forward and backward (the forward's geometry and
@@ -260,7 +277,7 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
         StageScope sc("batch.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 67.0);
         const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
             static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier) };
-        k_batch_preprocess<<<dim3(unsigned(ceil_div64(n, 256)), unsigned(n_views)), 256, 0, st>>>(n, p, L.cams,
+        k_batch_preprocess<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams,
             SplatOut { L.rects, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, L.depths, dkeys, dids });
         DGMR_HIP_CHECK(hipGetLastError());
     }

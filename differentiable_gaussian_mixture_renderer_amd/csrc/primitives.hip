@@ -157,34 +157,75 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     hist[int64_t(threadIdx.x) * n_blocks + blockIdx.x] = c;
 }
 
-// stable scatter: chunks of 256 consecutive items are ranked with wave ballots
-// (lanes holding the same digit), waves are ordered through LDS, and chunks are
-// processed in order, so equal digits keep their input order.
+// Stable scatter.  The block's 4096 items are first sorted by digit in LDS:
+// chunks of 256 consecutive items are ranked with wave ballots (lanes holding
+// the same digit), waves are ordered through LDS, chunks are processed in
+// order, so equal digits keep their input order.  The LDS copy is then written
+// out in digit order, so each digit's items leave the block as one contiguous
+// run instead of 4-byte scattered stores.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int n_blocks,
     const uint32_t* __restrict__ offsets)
 {
-    __shared__ uint32_t s_base[kRadix];       // global offset of each digit for this block + running count
+    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_local[kRadix];    // block-local start of each digit
+    __shared__ uint32_t s_global[kRadix];   // global start of each digit for this block
+    __shared__ uint32_t s_run[kRadix];      // running local position per digit
     __shared__ uint32_t s_wave_cnt[4][kRadix];
     __shared__ uint32_t s_wave_off[4][kRadix];
+    __shared__ uint32_t s_scan[4];
     const int64_t n = sort_count(n_max, n_dev);
     const int64_t base = int64_t(blockIdx.x) * kSortTile;
     if (base >= n)
         return;
-    s_base[threadIdx.x] = offsets[int64_t(threadIdx.x) * n_blocks + blockIdx.x];
+    const int count = int(min<int64_t>(kSortTile, n - base));
+    const unsigned lane = lane_id();
+    const unsigned wave = threadIdx.x / 64;
+    // block histogram
+    s_run[threadIdx.x] = 0;
     for (int w = 0; w < 4; ++w)
         s_wave_cnt[w][threadIdx.x] = 0;
     __syncthreads();
-    const unsigned lane = lane_id();
-    const unsigned wave = threadIdx.x / 64;
+    uint32_t keys[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const int i = j * kSortThreads + threadIdx.x;
+        keys[j] = i < count ? keys_in[base + i] : 0u;
+        if (i < count)
+            atomicAdd(&s_run[(keys[j] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the 256 digit counts
+    {
+        const uint32_t c = s_run[threadIdx.x];
+        uint32_t v = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(v, d, 64);
+            if (lane >= unsigned(d))
+                v += o;
+        }
+        if (lane == 63)
+            s_scan[wave] = v;
+        __syncthreads();
+        uint32_t prefix = 0;
+        for (unsigned w = 0; w < wave; ++w)
+            prefix += s_scan[w];
+        const uint32_t excl = prefix + v - c;
+        s_local[threadIdx.x] = excl;
+        s_run[threadIdx.x] = excl;
+        s_global[threadIdx.x] = offsets[int64_t(threadIdx.x) * n_blocks + blockIdx.x];
+    }
+    __syncthreads();
     const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
     for (int j = 0; j < kSortItems; ++j) {
-        const int64_t i = base + int64_t(j) * kSortThreads + threadIdx.x;
-        if (base + int64_t(j) * kSortThreads >= n)
+        if (j * kSortThreads >= count)
             break; // uniform across the block
-        const bool valid = i < n;
-        const uint32_t key = valid ? keys_in[i] : 0u;
-        const uint32_t val = valid ? vals_in[i] : 0u;
+        const int i = j * kSortThreads + threadIdx.x;
+        const bool valid = i < count;
+        const uint32_t key = keys[j];
+        const uint32_t val = valid ? vals_in[base + i] : 0u;
         const uint32_t digit = (key >> shift) & 0xffu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -194,12 +235,11 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
             peers &= bit ? m : ~m;
         }
         const uint32_t rank = __popcll(peers & lt_mask);
-        const bool leader = valid && (peers & lt_mask) == 0;
-        if (leader)
+        if (valid && (peers & lt_mask) == 0)
             s_wave_cnt[wave][digit] = __popcll(peers);
         __syncthreads();
         {
-            uint32_t run = s_base[threadIdx.x];
+            uint32_t run = s_run[threadIdx.x];
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
                 const uint32_t c = s_wave_cnt[w][threadIdx.x];
@@ -207,14 +247,22 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
                 run += c;
                 s_wave_cnt[w][threadIdx.x] = 0;
             }
-            s_base[threadIdx.x] = run;
+            s_run[threadIdx.x] = run;
         }
         __syncthreads();
         if (valid) {
             const uint32_t pos = s_wave_off[wave][digit] + rank;
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            s_keys[pos] = key;
+            s_vals[pos] = val;
         }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < count; q += kSortThreads) {
+        const uint32_t key = s_keys[q];
+        const uint32_t d = (key >> shift) & 0xffu;
+        const uint32_t dst = s_global[d] + (uint32_t(q) - s_local[d]);
+        keys_out[dst] = key;
+        vals_out[dst] = s_vals[q];
     }
 }
 

@@ -33,9 +33,13 @@ constexpr int kPreThreads = 256;
 
 __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess(int64_t n, SplatParams p, DevCamera<float> cam, SplatOut out)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    __shared__ float s_sh[kPreThreads * kShStride];
+    const int64_t first = int64_t(blockIdx.x) * blockDim.x;
+    stage_sh(s_sh, p.sh, first, int(min<int64_t>(blockDim.x, n - first)), p.sh_coeffs);
+    __syncthreads();
+    const int64_t i = first + threadIdx.x;
     if (i < n)
-        preprocess_gaussian(i, i, p, cam, out);
+        preprocess_gaussian(i, i, p, s_sh + threadIdx.x * kShStride, cam, out);
 }
 
 __global__ void k_gather_counts(int64_t n, const uint32_t* __restrict__ ids, const uint32_t* __restrict__ tiles, uint32_t* __restrict__ counts)
@@ -158,7 +162,7 @@ __global__ __launch_bounds__(TW* TH) void k_blend_backward(const uint32_t* __res
     int n_tiles, float bg0, float bg1, float bg2, const float* __restrict__ framebuffer, const float* __restrict__ final_T,
     const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
 {
-    __shared__ BlendSmem<B> sm;
+    __shared__ BlendSmemBwd<B> sm;
     const int tile = xcd_tile(blockIdx.x, n_tiles);
     blend_tile_backward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], (tile % grid_x) * TW, (tile / grid_x) * TH, width, height, bg0, bg1,
         bg2, point_list, xy, conic_opacity, rgb, framebuffer, final_T, grad_fb, screen_grads);
@@ -171,28 +175,36 @@ __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64
     const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations,
     float* __restrict__ g_opacity, float* __restrict__ g_sh)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
+    __shared__ float s_sh[kPreThreads * kShStride];
+    const int64_t first = int64_t(blockIdx.x) * blockDim.x;
+    const int count = int(min<int64_t>(blockDim.x, n - first));
+    stage_sh(s_sh, p.sh, first, count, p.sh_coeffs);
+    __syncthreads();
+    const int64_t i = first + threadIdx.x;
     float g_mu[3] = { 0.f, 0.f, 0.f }, g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f }, g_w = 0.f;
     float g_shv[48];
 #pragma unroll
     for (int k = 0; k < 48; ++k)
         g_shv[k] = 0.f;
-    if (tiles[i] != 0)
-        preprocess_backward_gaussian(i, p, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9, g_mu, g_s, g_q, &g_w, g_shv);
-    for (int a = 0; a < 3; ++a) {
-        g_means[3 * i + a] = g_mu[a];
-        g_scales[3 * i + a] = g_s[a];
+    float* row = s_sh + threadIdx.x * kShStride;
+    if (i < n && tiles[i] != 0)
+        preprocess_backward_gaussian(i, p, row, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9, g_mu, g_s, g_q, &g_w, g_shv);
+    if (i < n) {
+        for (int a = 0; a < 3; ++a) {
+            g_means[3 * i + a] = g_mu[a];
+            g_scales[3 * i + a] = g_s[a];
+        }
+        for (int a = 0; a < 4; ++a)
+            g_rotations[4 * i + a] = g_q[a];
+        g_opacity[i] = g_w;
     }
-    for (int a = 0; a < 4; ++a)
-        g_rotations[4 * i + a] = g_q[a];
-    g_opacity[i] = g_w;
-    float* gsh = g_sh + i * p.sh_coeffs * 3;
+    __syncthreads(); // every row has been read; reuse the rows for the SH gradient
 #pragma unroll
     for (int k = 0; k < 48; ++k)
         if (k < p.sh_coeffs * 3)
-            gsh[k] = g_shv[k];
+            row[k] = g_shv[k];
+    __syncthreads();
+    unstage_sh(g_sh, s_sh, first, count, p.sh_coeffs);
 }
 
 SplatParams make_params(const dgmr_gaussians* g)

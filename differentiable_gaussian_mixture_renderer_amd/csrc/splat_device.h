@@ -31,10 +31,35 @@ struct SplatOut {
     uint32_t* ids;
 };
 
+// SH coefficients of a block of Gaussians staged in LDS: the block's
+// contiguous [n_block, K, 3] slice is read with coalesced loads and each
+// Gaussian's row sits at a stride of 3K + 1 floats (odd, so the 64 lanes of a
+// wave reading their own rows hit distinct LDS banks).
+constexpr int kShStride = 49;
+
+__device__ __forceinline__ void stage_sh(float* s_sh, const float* sh, int64_t first, int count, int sh_coeffs)
+{
+    const int row = sh_coeffs * 3;
+    const int total = count * row;
+    const float* src = sh + first * row;
+    for (int e = threadIdx.x; e < total; e += blockDim.x)
+        s_sh[(e / row) * kShStride + (e % row)] = src[e];
+}
+
+__device__ __forceinline__ void unstage_sh(float* dst, const float* s_sh, int64_t first, int count, int sh_coeffs)
+{
+    const int row = sh_coeffs * 3;
+    const int total = count * row;
+    float* out = dst + first * row;
+    for (int e = threadIdx.x; e < total; e += blockDim.x)
+        out[e] = s_sh[(e / row) * kShStride + (e % row)];
+}
+
 // preprocess of Gaussian i for one camera (splat_forward.cu:71-118): cull
 // view-z < 0.2, EWA splat (Ots, 0.3 px^2 filter), rect ceil(3 sigma), tiles
-// touched, depth, SH colour, conic.  Writes slot o.
-__device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const SplatParams& p, const DevCamera<float>& cam, const SplatOut& out)
+// touched, depth, SH colour, conic.  Writes slot o.  c: this Gaussian's SH row.
+__device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const SplatParams& p, const float* c, const DevCamera<float>& cam,
+    const SplatOut& out)
 {
     out.tiles[o] = 0;
     out.rects[2 * o] = 0;
@@ -80,7 +105,6 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     float Y[16];
     sh_basis(p.sh_degree, dir, Y);
     const int kmax = min(p.sh_coeffs, (p.sh_degree + 1) * (p.sh_degree + 1));
-    const float* c = p.sh + i * p.sh_coeffs * 3;
     float r0 = 0.f, r1 = 0.f, r2 = 0.f;
     for (int k = 0; k < kmax; ++k) {
         r0 = fmaf(Y[k], c[3 * k + 0], r0);
@@ -177,15 +201,25 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
     final_T[p] = T;
 }
 
+template <int B>
+struct BlendSmemBwd {
+    float4 geo[B];  // x, y, conic a, conic b
+    float4 col[B];  // conic c, opacity, r, g
+    float blue[B];
+    uint32_t id[B];
+    float acc[9][B]; // per-batch gradient sums: xy 2, conic 3, opacity 1, rgb 3
+};
+
 // Blend adjoint of one tile, front to back.  The remaining colour is carried
 // unnormalised (C_final minus what has been composited so far), which is the
 // reference prototype's remaining colour (grad_inria_splatter_render.cu:100-156)
 // times the current transmittance, without its repeated division by (1 - alpha).
 // Per-Gaussian gradients (xy 2, conic 3, opacity 1, rgb 3) are summed over the
-// 64 pixels of a wave with DPP before one lane issues the global atomics.
+// 64 pixels of a wave with DPP, over the waves of the tile with LDS atomics,
+// and leave the tile as one global atomic per value and Gaussian.
 template <int TW, int TH, int B>
-__device__ __forceinline__ void blend_tile_backward(BlendSmem<B>& sm, uint32_t first, uint32_t last, int px0, int py0, int width, int height,
-    float bg0, float bg1, float bg2, const uint32_t* __restrict__ point_list, const float2* __restrict__ xy,
+__device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_t first, uint32_t last, int px0, int py0, int width,
+    int height, float bg0, float bg1, float bg2, const uint32_t* __restrict__ point_list, const float2* __restrict__ xy,
     const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, const float* __restrict__ framebuffer,
     const float* __restrict__ final_T, const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
 {
@@ -212,6 +246,7 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmem<B>& sm, uint32_t f
     for (uint32_t base = first; base < last; base += B) {
         if (__syncthreads_count(!done) == 0)
             break;
+        const int cnt = int(min<uint32_t>(B, last - base));
         for (int i = threadIdx.x; i < B; i += NT) {
             const uint32_t k = base + uint32_t(i);
             if (k < last) {
@@ -223,9 +258,11 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmem<B>& sm, uint32_t f
                 sm.blue[i] = rgb[3 * id + 2];
                 sm.id[i] = id;
             }
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                sm.acc[q][i] = 0.f;
         }
         __syncthreads();
-        const int cnt = int(min<uint32_t>(B, last - base));
         for (int j = 0; j < cnt; ++j) {
             if (__ballot(!done) == 0)
                 break; // the whole wave has finished
@@ -254,7 +291,9 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmem<B>& sm, uint32_t f
                         gr = gC0 * wgt;
                         gg = gC1 * wgt;
                         gbl = gC2 * wgt;
-                        const float g_alpha = (gC0 * c.z + gC1 * c.w + gC2 * blue) * T - (gC0 * S0 + gC1 * S1 + gC2 * S2 + gT_Tf) / tk;
+                        // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
+                        const float inv_tk = __builtin_amdgcn_rcpf(tk);
+                        const float g_alpha = (gC0 * c.z + gC1 * c.w + gC2 * blue) * T - (gC0 * S0 + gC1 * S1 + gC2 * S2 + gT_Tf) * inv_tk;
                         const float g_raw = raw >= 0.99f ? 0.f : g_alpha;
                         go = G * g_raw;
                         const float gp = c.y * g_raw * G;
@@ -278,17 +317,26 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmem<B>& sm, uint32_t f
                 gg = wave_sum_to_last(gg);
                 gbl = wave_sum_to_last(gbl);
                 if (lane_id() == 63) {
-                    float* dst = screen_grads + int64_t(sm.id[j]) * 9;
-                    unsafeAtomicAdd(dst + 0, gx);
-                    unsafeAtomicAdd(dst + 1, gy);
-                    unsafeAtomicAdd(dst + 2, ga);
-                    unsafeAtomicAdd(dst + 3, gb);
-                    unsafeAtomicAdd(dst + 4, gc);
-                    unsafeAtomicAdd(dst + 5, go);
-                    unsafeAtomicAdd(dst + 6, gr);
-                    unsafeAtomicAdd(dst + 7, gg);
-                    unsafeAtomicAdd(dst + 8, gbl);
+                    atomicAdd(&sm.acc[0][j], gx);
+                    atomicAdd(&sm.acc[1][j], gy);
+                    atomicAdd(&sm.acc[2][j], ga);
+                    atomicAdd(&sm.acc[3][j], gb);
+                    atomicAdd(&sm.acc[4][j], gc);
+                    atomicAdd(&sm.acc[5][j], go);
+                    atomicAdd(&sm.acc[6][j], gr);
+                    atomicAdd(&sm.acc[7][j], gg);
+                    atomicAdd(&sm.acc[8][j], gbl);
                 }
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < cnt; i += NT) {
+            float* dst = screen_grads + int64_t(sm.id[i]) * 9;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const float v = sm.acc[q][i];
+                if (v != 0.f)
+                    unsafeAtomicAdd(dst + q, v);
             }
         }
     }
@@ -298,8 +346,9 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmem<B>& sm, uint32_t f
 // (2x2 inverse adjoint), grad::splat<Ots> (grad/math.h:333-527),
 // grad::sh_to_colour (grad/math.h:203-282) and normalize.  Adds into the
 // accumulators (g_sh has 48 slots; only k < sh_coeffs are meaningful).
-__device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const SplatParams& p, const DevCamera<float>& cam, const float4 A,
-    const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc, float* g_q_acc, float* g_w_acc, float* g_sh_acc)
+// c: this Gaussian's SH row.
+__device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const SplatParams& p, const float* c, const DevCamera<float>& cam,
+    const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc, float* g_q_acc, float* g_w_acc, float* g_sh_acc)
 {
     const float mu[3] = { p.means[3 * i], p.means[3 * i + 1], p.means[3 * i + 2] };
     const float m = p.scale_multiplier;
@@ -321,7 +370,6 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
     float Y[16], dY[16][3];
     sh_basis(p.sh_degree, dir, Y);
     sh_basis_grad(p.sh_degree, dir, dY);
-    const float* c = p.sh + i * p.sh_coeffs * 3;
     const int kmax = min(p.sh_coeffs, (p.sh_degree + 1) * (p.sh_degree + 1));
     float g_dir[3] = { 0.f, 0.f, 0.f };
 #pragma unroll

@@ -1,0 +1,268 @@
+"""Continuous-batching render server over scenes resident in HBM.
+
+Requests (scene id, camera, forward-only or forward+backward) arrive on an
+open-loop trace.  A scheduler thread admits waiting requests while the
+estimated HBM footprint of everything in flight stays under a budget, groups
+the admitted requests of one scene into a batch, and hands the batch to the
+next free worker.  Each worker owns a HIP stream and runs the batch through the
+multi-view path (BatchSplatter: one preprocess, one binning, one persistent
+blend launch); workers run concurrently, so batches of different scenes, and
+new arrivals admitted while others are on the GPU, overlap on the device.
+Latency is completion time minus arrival time.
+
+`FifoServer` is the plain baseline: one request at a time through the
+one-camera-per-launch Splatter path, in arrival order.
+"""
+from __future__ import annotations
+
+import math
+import threading
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import scenes
+from .batching import BatchSplatter
+
+# bytes of device memory per (view, Gaussian) slot of the batched path and per
+# (tile, Gaussian) pair, used for admission (geometry + binning + screen grads)
+BYTES_PER_VIEW_GAUSSIAN = 112
+BYTES_PER_PAIR = 24
+PAIRS_PER_PIXEL = 8
+
+
+@dataclass
+class Request:
+    rid: int
+    scene: int
+    camera: scenes.Camera
+    backward: bool
+    arrival: float            # seconds on the server clock
+    target_seed: int = 0
+    done_at: float = -1.0
+    image: Optional[torch.Tensor] = None
+    grads: Optional[list] = None
+
+    @property
+    def pixels(self) -> int:
+        return self.camera.width * self.camera.height
+
+
+@dataclass
+class ResidentScene:
+    params: list               # sh, opacity, means, scales, rotations on the device
+    sh_degree: int
+
+    @property
+    def n(self) -> int:
+        return self.params[2].shape[0]
+
+
+def make_resident_scenes(count: int, seed: int, n_min: int = 50_000, n_max: int = 3_000_000, device="cuda:0") -> List[ResidentScene]:
+    """Scenes with Gaussian counts log-uniform in [n_min, n_max] (config 2 distributions)."""
+    st = scenes.Stream(seed, 11)
+    u = st.uniform(count)
+    out = []
+    for k in range(count):
+        n = int(round(math.exp(math.log(n_min) + u[k] * (math.log(n_max) - math.log(n_min)))))
+        sc = scenes.make_scene(n, seed=1000 * seed + k, extent=5.0, sh_degree=3)
+        t = lambda a: torch.tensor(np.ascontiguousarray(a), device=device)
+        out.append(ResidentScene([t(sc.sh), t(sc.opacity), t(sc.means), t(sc.scales), t(sc.rotations)], sc.sh_degree))
+    return out
+
+
+def make_trace(n_requests: int, rate_hz: float, n_scenes: int, seed: int, backward_fraction: float = 0.3,
+               min_res=(256, 256), max_res=(1920, 1080)) -> List[Request]:
+    """Poisson arrivals; random camera on the radius-6 sphere; width and height
+    uniform between min_res and max_res; forward+backward with the given
+    probability."""
+    st = scenes.Stream(seed, 12)
+    gaps = -np.log1p(-st.uniform(n_requests)) / rate_hz
+    arrivals = np.cumsum(gaps) - gaps[0]
+    pick = (st.uniform(n_requests) * n_scenes).astype(np.int64)
+    bwd = st.uniform(n_requests) < backward_fraction
+    ws = (min_res[0] + st.uniform(n_requests) * (max_res[0] - min_res[0])).astype(np.int64)
+    hs = (min_res[1] + st.uniform(n_requests) * (max_res[1] - min_res[1])).astype(np.int64)
+    z = st.uniform(n_requests) * 2 - 1
+    phi = st.uniform(n_requests) * 2 * math.pi
+    ty = math.tan(math.radians(30.0))
+    reqs = []
+    for i in range(n_requests):
+        r = math.sqrt(max(0.0, 1 - z[i] * z[i]))
+        pos = 6.0 * np.array([r * math.cos(phi[i]), r * math.sin(phi[i]), z[i]])
+        w, h = int(ws[i]), int(hs[i])
+        cam = scenes.look_at_camera(pos, [0, 0, 0], w, h, ty * w / h, ty)
+        reqs.append(Request(i, int(pick[i]), cam, bool(bwd[i]), float(arrivals[i]), target_seed=i))
+    return reqs
+
+
+def l1_grad(img: torch.Tensor, seed: int) -> torch.Tensor:
+    """dL/dimg of L = mean |img - target| for a seeded uniform-noise target."""
+    g = torch.Generator(device=img.device)
+    g.manual_seed(seed)
+    target = torch.rand(img.shape, generator=g, device=img.device, dtype=img.dtype)
+    return torch.sign(img - target) / img.numel()
+
+
+def estimate_bytes(scene: ResidentScene, reqs: List[Request]) -> int:
+    px = sum(r.pixels for r in reqs)
+    return len(reqs) * scene.n * BYTES_PER_VIEW_GAUSSIAN + px * (PAIRS_PER_PIXEL * BYTES_PER_PAIR + 32)
+
+
+@dataclass
+class ServerStats:
+    latencies: List[float] = field(default_factory=list)
+    pixels: int = 0
+    wall: float = 0.0
+    batches: int = 0
+    max_in_flight_bytes: int = 0
+
+    def summary(self) -> Dict[str, float]:
+        lat = np.array(self.latencies) * 1e3
+        return {"p50_ms": float(np.percentile(lat, 50)) if lat.size else float("nan"),
+                "p99_ms": float(np.percentile(lat, 99)) if lat.size else float("nan"),
+                "mpixel_per_s": self.pixels / self.wall / 1e6 if self.wall > 0 else 0.0,
+                "requests": len(self.latencies), "batches": self.batches, "wall_s": self.wall}
+
+
+class ContinuousBatchingServer:
+    def __init__(self, resident: List[ResidentScene], hbm_budget_bytes: int, workers: int = 3, max_batch_views: int = 32,
+                 tile=(16, 16), lds_batch: int = 256, device="cuda:0"):
+        self.resident = resident
+        self.budget = int(hbm_budget_bytes)
+        self.max_views = max_batch_views
+        self.device = torch.device(device)
+        self.tile, self.lds_batch = tile, lds_batch
+        self.n_workers = workers
+        self.stats = ServerStats()
+        self._lock = threading.Condition()
+        self._waiting: deque = deque()
+        self._in_flight = 0
+        self._work: deque = deque()
+        self._stop = False
+        self._t0 = 0.0
+
+    # ------------------------------------------------------------------ clock
+    def now(self) -> float:
+        return time.perf_counter() - self._t0
+
+    # ---------------------------------------------------------------- workers
+    def _worker(self):
+        stream = torch.cuda.Stream(device=self.device)
+        splatter = BatchSplatter(tile=self.tile, batch=self.lds_batch, device=self.device)
+        while True:
+            with self._lock:
+                while not self._work and not self._stop:
+                    self._lock.wait()
+                if not self._work and self._stop:
+                    return
+                scene_id, batch, cost = self._work.popleft()
+            with torch.cuda.stream(stream):
+                sc = self.resident[scene_id]
+                imgs, _ = splatter.forward(*sc.params, [r.camera for r in batch], sh_degree=sc.sh_degree)
+                grads = None
+                if any(r.backward for r in batch):
+                    gimg = [l1_grad(img, r.target_seed) if r.backward else torch.zeros_like(img) for img, r in zip(imgs, batch)]
+                    grads = splatter.backward(gimg)
+                stream.synchronize()
+            t = self.now()
+            with self._lock:
+                for r, img in zip(batch, imgs):
+                    r.done_at = t
+                    r.image = img
+                    r.grads = grads if r.backward else None
+                    self.stats.latencies.append(t - r.arrival)
+                    self.stats.pixels += r.pixels
+                self.stats.batches += 1
+                self._in_flight -= cost
+                self._lock.notify_all()
+
+    # -------------------------------------------------------------- scheduler
+    def _schedule(self):
+        """Admits the waiting requests of the scene with the oldest waiting
+        request, as many as fit the budget and the batch limit."""
+        if not self._waiting:
+            return False
+        scene_id = self._waiting[0].scene
+        picked, rest = [], deque()
+        while self._waiting:
+            r = self._waiting.popleft()
+            if r.scene == scene_id and len(picked) < self.max_views:
+                cost = estimate_bytes(self.resident[scene_id], picked + [r])
+                if self._in_flight + cost <= self.budget or (not picked and self._in_flight == 0):
+                    picked.append(r)
+                    continue
+            rest.append(r)
+        self._waiting = rest
+        if not picked:
+            return False
+        cost = estimate_bytes(self.resident[scene_id], picked)
+        self._in_flight += cost
+        self.stats.max_in_flight_bytes = max(self.stats.max_in_flight_bytes, self._in_flight)
+        self._work.append((scene_id, picked, cost))
+        self._lock.notify_all()
+        return True
+
+    def run(self, trace: List[Request]) -> ServerStats:
+        self._t0 = time.perf_counter()
+        workers = [threading.Thread(target=self._worker, daemon=True) for _ in range(self.n_workers)]
+        for w in workers:
+            w.start()
+        pending = deque(sorted(trace, key=lambda r: r.arrival))
+        total = len(pending)
+        while True:
+            with self._lock:
+                t = self.now()
+                while pending and pending[0].arrival <= t:
+                    self._waiting.append(pending.popleft())
+                # keep at most one queued batch per idle worker: the rest keep waiting and join later batches
+                while len(self._work) < self.n_workers and self._schedule():
+                    pass
+                if not pending and not self._waiting and not self._work and len(self.stats.latencies) == total:
+                    break
+                timeout = (pending[0].arrival - t) if pending else 0.005
+                self._lock.wait(timeout=max(0.0005, min(timeout, 0.005)))
+        with self._lock:
+            self._stop = True
+            self._lock.notify_all()
+        for w in workers:
+            w.join()
+        self.stats.wall = self.now()
+        return self.stats
+
+
+class FifoServer:
+    """One request at a time, in arrival order, one camera per launch."""
+
+    def __init__(self, resident: List[ResidentScene], tile=(16, 16), lds_batch: int = 256):
+        self.resident = resident
+        self.tile, self.lds_batch = tile, lds_batch
+        self.stats = ServerStats()
+
+    def run(self, trace: List[Request]) -> ServerStats:
+        from .python_bindings import splatter
+        t0 = time.perf_counter()
+        bg = torch.zeros(3)
+        for r in sorted(trace, key=lambda q: q.arrival):
+            while time.perf_counter() - t0 < r.arrival:
+                time.sleep(0.0002)
+            sc = self.resident[r.scene]
+            cam = r.camera
+            args = (*sc.params[:5], torch.tensor(cam.world_view_transform), torch.tensor(cam.full_proj_transform),
+                    torch.tensor(cam.camera_center), bg, cam.width, cam.height, cam.tan_fovx, cam.tan_fovy, sc.sh_degree)
+            cache = splatter.cpp_binding.forward(*args, tile=self.tile, batch=self.lds_batch)
+            r.image = cache[0]
+            if r.backward:
+                r.grads = splatter.cpp_binding.backward(*args, l1_grad(cache[0], r.target_seed), cache, tile=self.tile,
+                                                        batch=self.lds_batch)
+            torch.cuda.synchronize()
+            r.done_at = time.perf_counter() - t0
+            self.stats.latencies.append(r.done_at - r.arrival)
+            self.stats.pixels += r.pixels
+            self.stats.batches += 1
+        self.stats.wall = time.perf_counter() - t0
+        return self.stats
