@@ -6,7 +6,8 @@ logits), splat forward, L1 loss against a seeded uniform-noise target, and the
 backward to every parameter (means, log-scales, quaternions, opacity logits,
 SH).  Inputs are resident in HBM before the timed region.
 
-Prints one JSON line (see README of the measurement contract in DESIGN.md).
+Prints one JSON line: throughput, per-stage durations from HIP events, the
+roofline of the dominant kernel and a host baseline of the same algorithm.
 """
 from __future__ import annotations
 
@@ -36,8 +37,21 @@ def parse():
     ap.add_argument("--tile", default="16x16")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
-    ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame in the host sample (0: automatic)")
+    ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame in the host sample (0: the whole frame)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (calibrates HBM byte counters) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib: int):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}))
 
 
 def stage_times(lib):
@@ -51,8 +65,11 @@ def stage_times(lib):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is described in DESIGN.md, not run)"}))
+        print(json.dumps({"error": "one GPU only: --gpus must be 1 (multi-GPU sharding is a design note, not a run)"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import numpy as np
     import torch
 
@@ -62,7 +79,7 @@ def main():
 
     tile = tuple(int(v) for v in args.tile.split("x"))
     W, H = args.width, args.height
-    scene, cam = scenes.config2(n=args.n, width=W, height=H)
+    scene, cam = scenes.training_frame(n=args.n, width=W, height=H)
     dev = torch.device("cuda:0")
     lib = N.load()
     t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
@@ -128,7 +145,7 @@ def main():
     frame_hbm_frac = frame_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     traffic = None
-    tr_path = os.path.join(ROOT, "profiles", "traffic_splat_cfg2.json")
+    tr_path = os.path.join(ROOT, "profiles", "traffic_splat_1M_1080p.json")
     if os.path.exists(tr_path):
         with open(tr_path) as f:
             tr = json.load(f)
@@ -150,7 +167,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded SplitMix64 scene and L1 target, BASELINE config 2 distributions)",
+        "data": "synthetic (seeded SplitMix64 scene: means U[-5,5]^3, log-scales N(-3,0.5), SH degree 3; uniform L1 target)",
         "config": {"workload": f"tile splat fwd+bwd, {args.n} Gaussians SH3, {W}x{H}, L1 loss", "tile": f"{tile[0]}x{tile[1]}",
                    "lds_batch": args.batch, "pairs": R, "visible": n_visible},
         "roofline": {"bound": "hbm", "kernel": d_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -171,7 +188,7 @@ def cpu_baseline(scene, cam, rows: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     H, W = cam.height, cam.width
-    rows = rows or 48
+    rows = min(rows or H, H)
     y0 = (H - rows) // 2
     h = O.SplatOracle(O.scene_arrays(scene), cam, dbl=False)
     grad = np.full((3, H, W), 1.0 / (3 * H * W), np.float32)

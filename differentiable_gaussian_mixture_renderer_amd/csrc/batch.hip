@@ -221,7 +221,7 @@ int check_batch_args(const dgmr_gaussians* g, const dgmr_batch_view* views, int3
     DGMR_ARG_CHECK(g->n >= 0 && g->n * int64_t(n_views) < (int64_t(1) << 32), "dgmr batch: n * views must stay below 2^32");
     DGMR_ARG_CHECK(g->sh_coeffs >= 1 && g->sh_coeffs <= 16 && g->sh_degree >= 0 && g->sh_degree <= 3, "dgmr batch: bad SH layout");
     for (int v = 0; v < n_views; ++v)
-        DGMR_ARG_CHECK(views[v].cam.width > 0 && views[v].cam.height > 0 && views[v].framebuffer && views[v].final_transmittance,
+        DGMR_ARG_CHECK(valid_image(views[v].cam.width, views[v].cam.height) && views[v].framebuffer && views[v].final_transmittance,
             "dgmr batch: bad view");
     return DGMR_OK;
 }

@@ -15,6 +15,12 @@ namespace dgmr {
 
 void set_error(const std::string& msg);
 
+// largest image side accepted: tile grids stay below 2^16 tiles per axis (the
+// pair emission packs a grid width and a rectangle width into 16 bits each)
+constexpr int32_t kMaxImageSide = 1 << 15;
+
+inline bool valid_image(int32_t w, int32_t h) { return w > 0 && h > 0 && w <= kMaxImageSide && h <= kMaxImageSide; }
+
 #define DGMR_HIP_CHECK(expr)                                                                                   \
     do {                                                                                                       \
         hipError_t e_ = (expr);                                                                                \

@@ -33,13 +33,12 @@ constexpr int kPreThreads = 256;
 
 __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess(int64_t n, SplatParams p, DevCamera<float> cam, SplatOut out)
 {
-    __shared__ float s_sh[kPreThreads * kShStride];
-    const int64_t first = int64_t(blockIdx.x) * blockDim.x;
-    stage_sh(s_sh, p.sh, first, int(min<int64_t>(blockDim.x, n - first)), p.sh_coeffs);
-    __syncthreads();
-    const int64_t i = first + threadIdx.x;
+    // SH rows are read straight from global memory: the 12 row loads of a wave
+    // cover one contiguous span that stays in cache, and keeping LDS free
+    // keeps the occupancy that hides the gather latency
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n)
-        preprocess_gaussian(i, i, p, s_sh + threadIdx.x * kShStride, cam, out);
+        preprocess_gaussian(i, i, p, p.sh + i * p.sh_coeffs * 3, cam, out);
 }
 
 __global__ void k_gather_counts(int64_t n, const uint32_t* __restrict__ ids, const uint32_t* __restrict__ tiles, uint32_t* __restrict__ counts)
@@ -51,84 +50,165 @@ __global__ void k_gather_counts(int64_t n, const uint32_t* __restrict__ ids, con
 
 } // namespace
 
-// Emission of (tile, Gaussian) pairs with per-wave load balancing: a wave
-// owns 64 consecutive depth-sorted Gaussians and writes their pairs 64 at a
-// time, each lane locating its Gaussian by binary search over the wave's
-// offsets in LDS, so one large Gaussian no longer serialises a thread.
+// Emission of (tile, Gaussian) pairs balanced over pairs: block b writes pairs
+// [b * kEmitPairs, (b + 1) * kEmitPairs) of the depth-ordered pair list.  Its
+// first and last owners come from a 64-ary search of the exclusive offsets;
+// the owners in between (culled Gaussians carry the largest depth key, so the
+// owners of a block's pairs are contiguous and each owns at least one pair)
+// are staged in LDS with their tile rectangle, and every thread writes 8
+// consecutive pairs.  A window wider than the LDS stage (only possible with
+// zero-pair Gaussians among visible ones) takes a per-pair search instead.
 // views: per-view (grid_x, grid_y, first global tile, -) when ids span several
 // views of n_per_view Gaussians each; null for a single view.
-template <class S>
-__global__ __launch_bounds__(256) void k_emit_pairs(int64_t n, const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ offsets,
-    const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const S* __restrict__ xy, int grid_x, int grid_y, int tile_w,
-    int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
+constexpr int kEmitThreads = 256;
+constexpr int kEmitPer = 8;
+constexpr int kEmitPairs = kEmitThreads * kEmitPer;
+
+// last j in [lo, hi] with off[j] <= t, given off[lo] <= t (one wave, 64-ary)
+__device__ uint32_t wave_owner_search(const uint32_t* __restrict__ off, uint32_t lo, uint32_t hi, uint32_t t)
 {
-    __shared__ uint32_t s_off[4][64], s_x0[4][64], s_y0[4][64], s_w[4][64], s_id[4][64], s_gx[4][64], s_tb[4][64];
     const unsigned lane = lane_id();
-    const unsigned wave = threadIdx.x / 64;
-    const int64_t j = (int64_t(blockIdx.x) * 4 + wave) * 64 + lane;
-    uint32_t off = 0xffffffffu, cnt = 0, x0 = 0, y0 = 0, w = 1, id = 0, gxv = uint32_t(grid_x), tb = 0;
-    if (j < n) {
-        id = sorted_ids[j];
-        off = offsets[j];
-        cnt = tiles[id];
-        if (cnt) {
-            int gx = grid_x, gy = grid_y, tbase = 0;
-            if (views) {
-                const int4 v = views[id / n_per_view];
-                gx = v.x;
-                gy = v.y;
-                tbase = v.z;
-            }
-            int a, b, c, d;
-            get_rect(xy[2 * id], xy[2 * id + 1], rects[2 * id], rects[2 * id + 1], gx, gy, tile_w, tile_h, &a, &b, &c, &d);
-            // a view's tiles occupy [tbase, tbase + gx * gy) of the global tile range
-            x0 = uint32_t(a);
-            y0 = uint32_t(b);
-            w = uint32_t(c - a);
-            gxv = uint32_t(gx);
-            tb = uint32_t(tbase);
-        }
+    while (hi > lo) {
+        const uint32_t len = hi - lo + 1;
+        const uint32_t stride = (len + 63) / 64;
+        const uint32_t idx = lo + lane * stride;
+        const bool pred = idx <= hi && off[idx] <= t;
+        const uint64_t m = __ballot(pred);
+        const unsigned last = 63 - __clzll(m); // lane 0 always holds
+        const uint32_t nlo = lo + last * stride;
+        hi = min(hi, nlo + stride - 1);
+        lo = nlo;
+        if (stride == 1)
+            break;
     }
-    s_off[wave][lane] = off;
-    s_x0[wave][lane] = x0;
-    s_y0[wave][lane] = y0;
-    s_w[wave][lane] = w;
-    s_id[wave][lane] = id;
-    s_gx[wave][lane] = gxv;
-    s_tb[wave][lane] = tb;
-    uint32_t end = (j < n) ? off + cnt : 0u;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1)
-        end = max(end, __shfl_xor(end, d, 64));
-    const uint32_t base = __shfl(off, 0, 64);
-    __syncthreads();
-    if (base == 0xffffffffu)
+    return lo;
+}
+
+// tile rectangle of Gaussian id: first tile and (grid_x << 16 | width); 0 shape when it has no tiles
+template <class S>
+__device__ __forceinline__ void owner_rect(uint32_t id, const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles,
+    const S* __restrict__ xy, int grid_x, int grid_y, int tile_w, int tile_h, const int4* __restrict__ views, int64_t n_per_view,
+    uint32_t* start, uint32_t* shape)
+{
+    *start = 0;
+    *shape = 0;
+    if (tiles[id] == 0)
         return;
-    const uint32_t total = end - base;
-    for (uint32_t p = lane; p < total; p += 64) {
-        const uint32_t target = base + p;
-        // last lane whose offset is <= target
-        unsigned lo = 0, hi = 63;
+    int gx = grid_x, gy = grid_y, tbase = 0;
+    if (views) {
+        const int4 v = views[id / n_per_view];
+        gx = v.x;
+        gy = v.y;
+        tbase = v.z;
+    }
+    int a, b, c, d;
+    get_rect(xy[2 * id], xy[2 * id + 1], rects[2 * id], rects[2 * id + 1], gx, gy, tile_w, tile_h, &a, &b, &c, &d);
+    // a view's tiles occupy [tbase, tbase + gx * gy) of the global tile range
+    *start = uint32_t(tbase) + uint32_t(b) * uint32_t(gx) + uint32_t(a);
+    *shape = (uint32_t(gx) << 16) | uint32_t(c - a);
+}
+
+template <class S>
+__global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, uint32_t R, const uint32_t* __restrict__ sorted_ids,
+    const uint32_t* __restrict__ offsets, const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const S* __restrict__ xy,
+    int grid_x, int grid_y, int tile_w, int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out)
+{
+    __shared__ uint32_t s_off[kEmitPairs], s_id[kEmitPairs], s_start[kEmitPairs], s_shape[kEmitPairs];
+    __shared__ uint32_t s_bounds[2];
+    const uint32_t first = blockIdx.x * uint32_t(kEmitPairs);
+    const uint32_t end = min(R, first + uint32_t(kEmitPairs));
+    const unsigned wave = threadIdx.x / 64;
+    if (wave < 2) {
+        const uint32_t g = wave_owner_search(offsets, 0, uint32_t(n - 1), wave == 0 ? first : end - 1);
+        if (lane_id() == 0)
+            s_bounds[wave] = g;
+    }
+    __syncthreads();
+    const uint32_t g_lo = s_bounds[0], g_hi = s_bounds[1];
+    const uint32_t window = g_hi - g_lo + 1;
+    if (window <= uint32_t(kEmitPairs)) {
+        for (uint32_t k = threadIdx.x; k < window; k += kEmitThreads) {
+            const uint32_t id = sorted_ids[g_lo + k];
+            s_off[k] = offsets[g_lo + k];
+            s_id[k] = id;
+            owner_rect(id, rects, tiles, xy, grid_x, grid_y, tile_w, tile_h, views, n_per_view, &s_start[k], &s_shape[k]);
+        }
+        __syncthreads();
+        const uint32_t p0 = first + threadIdx.x * kEmitPer;
+        if (p0 >= end)
+            return;
+        // owner of p0: last k with s_off[k] <= p0
+        uint32_t lo = 0, hi = window - 1;
         while (lo < hi) {
-            const unsigned mid = (lo + hi + 1) >> 1;
-            if (s_off[wave][mid] <= target)
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= p0)
                 lo = mid;
             else
                 hi = mid - 1;
         }
-        const uint32_t local = target - s_off[wave][lo];
-        const uint32_t ww = s_w[wave][lo];
-        uint32_t ty = uint32_t(__fdiv_rn(float(local), float(ww)));
-        int32_t tx = int32_t(local) - int32_t(ty * ww);
-        if (tx < 0) {
-            --ty;
-            tx += int32_t(ww);
-        } else if (tx >= int32_t(ww)) {
-            ++ty;
-            tx -= int32_t(ww);
+        uint32_t k = lo;
+        uint32_t next = k + 1 < window ? s_off[k + 1] : 0xffffffffu;
+        uint32_t shape = s_shape[k], local = p0 - s_off[k];
+        uint32_t w = shape & 0xffffu;
+        uint32_t ty = local / w, tx = local - ty * w;
+        uint32_t kv[kEmitPer], vv[kEmitPer];
+#pragma unroll
+        for (int q = 0; q < kEmitPer; ++q) {
+            const uint32_t p = p0 + q;
+            if (p >= next) {
+                do {
+                    ++k;
+                    next = k + 1 < window ? s_off[k + 1] : 0xffffffffu;
+                } while (p >= next);
+                shape = s_shape[k];
+                w = shape & 0xffffu;
+                tx = 0;
+                ty = 0;
+            }
+            kv[q] = s_start[k] + ty * (shape >> 16) + tx;
+            vv[q] = s_id[k];
+            if (++tx == w) {
+                tx = 0;
+                ++ty;
+            }
         }
-        keys_out[target] = s_tb[wave][lo] + (s_y0[wave][lo] + ty) * s_gx[wave][lo] + s_x0[wave][lo] + uint32_t(tx);
-        vals_out[target] = s_id[wave][lo];
+        const bool aligned = ((reinterpret_cast<uintptr_t>(keys_out) | reinterpret_cast<uintptr_t>(vals_out)) & 15u) == 0;
+        if (aligned && p0 + kEmitPer <= end) {
+            // 16-byte aligned: p0 is a multiple of 8
+            uint4* kd = reinterpret_cast<uint4*>(keys_out + p0);
+            uint4* vd = reinterpret_cast<uint4*>(vals_out + p0);
+            kd[0] = make_uint4(kv[0], kv[1], kv[2], kv[3]);
+            kd[1] = make_uint4(kv[4], kv[5], kv[6], kv[7]);
+            vd[0] = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+            vd[1] = make_uint4(vv[4], vv[5], vv[6], vv[7]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < kEmitPer; ++q)
+                if (p0 + q < end) {
+                    keys_out[p0 + q] = kv[q];
+                    vals_out[p0 + q] = vv[q];
+                }
+        }
+    } else {
+        for (uint32_t p = first + threadIdx.x; p < end; p += kEmitThreads) {
+            uint32_t lo = g_lo, hi = g_hi;
+            while (lo < hi) {
+                const uint32_t mid = lo + ((hi - lo + 1) >> 1);
+                if (offsets[mid] <= p)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const uint32_t id = sorted_ids[lo];
+            uint32_t start, shape;
+            owner_rect(id, rects, tiles, xy, grid_x, grid_y, tile_w, tile_h, views, n_per_view, &start, &shape);
+            const uint32_t local = p - offsets[lo];
+            const uint32_t w = shape & 0xffffu;
+            const uint32_t ty = local / w, tx = local - ty * w;
+            keys_out[p] = start + ty * (shape >> 16) + tx;
+            vals_out[p] = id;
+        }
     }
 }
 
@@ -376,15 +456,15 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
         pb = SortBuffers { w.pkeys_a, bins->point_list, w.pkeys_b, w.pvals_a };
     else
         pb = SortBuffers { w.pkeys_a, w.pvals_a, w.pkeys_b, bins->point_list };
-    const unsigned emit_blocks = unsigned(ceil_div64(n, 256));
+    const unsigned emit_blocks = unsigned(ceil_div64(R, kEmitPairs));
     {
         StageScope sc("bin.emit", st, 8.0 * double(R) + 28.0 * double(n));
         if (xy_double)
-            k_emit_pairs<double><<<emit_blocks, 256, 0, st>>>(n, sorted_ids, w.pair_offsets, rects, tiles, static_cast<const double*>(xy), grid_x,
-                grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
+            k_emit_pairs<double><<<emit_blocks, kEmitThreads, 0, st>>>(n, R, sorted_ids, w.pair_offsets, rects, tiles,
+                static_cast<const double*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
         else
-            k_emit_pairs<float><<<emit_blocks, 256, 0, st>>>(n, sorted_ids, w.pair_offsets, rects, tiles, static_cast<const float*>(xy), grid_x,
-                grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
+            k_emit_pairs<float><<<emit_blocks, kEmitThreads, 0, st>>>(n, R, sorted_ids, w.pair_offsets, rects, tiles,
+                static_cast<const float*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
         DGMR_HIP_CHECK(hipGetLastError());
     }
     {
@@ -419,7 +499,7 @@ extern "C" {
 
 int dgmr_workspace_size(int64_t n, int32_t width, int32_t height, const dgmr_tiling* tiling, int64_t pair_capacity, size_t* bytes)
 {
-    DGMR_ARG_CHECK(bytes && n >= 0 && width > 0 && height > 0 && valid_tiling(tiling) && pair_capacity >= 0, "dgmr_workspace_size: bad argument");
+    DGMR_ARG_CHECK(bytes && n >= 0 && valid_image(width, height) && valid_tiling(tiling) && pair_capacity >= 0, "dgmr_workspace_size: bad argument");
     *bytes = splat_workspace_bytes(n, pair_capacity);
     return DGMR_OK;
 }
@@ -433,7 +513,7 @@ int dgmr_splat_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dg
     DGMR_ARG_CHECK(valid_tiling(tiling), "dgmr_splat_forward: tiling must be 8x8, 16x16 or 32x8 with batch 128/256/512");
     DGMR_ARG_CHECK(g->n >= 0 && g->n < (int64_t(1) << 31), "dgmr_splat_forward: bad Gaussian count");
     DGMR_ARG_CHECK(g->sh_coeffs >= 1 && g->sh_coeffs <= 16 && g->sh_degree >= 0 && g->sh_degree <= 3, "dgmr_splat_forward: bad SH layout");
-    DGMR_ARG_CHECK(cam->width > 0 && cam->height > 0, "dgmr_splat_forward: bad image size");
+    DGMR_ARG_CHECK(valid_image(cam->width, cam->height), "dgmr_splat_forward: bad image size");
     DGMR_ARG_CHECK(bins->point_list && bins->ranges, "dgmr_splat_forward: null binning buffers");
     const int64_t n = g->n;
     DGMR_ARG_CHECK(workspace_bytes >= splat_workspace_bytes(n, bins->pair_capacity), "dgmr_splat_forward: workspace too small");

@@ -158,7 +158,7 @@ def look_at_camera(position, target, width: int, height: int, tan_fovx: float, t
 
 
 # ----------------------------------------------------------------- workloads
-def config1():
+def small_rgb_frame():
     """Plain forward splat: 10k Gaussians, U[-1,1]^3, degree-0 colour, camera at
     distance 4, fov 60 degrees both axes, 256x256."""
     scene = make_scene(10_000, seed=1, extent=1.0, sh_mode="rgb")
@@ -167,7 +167,7 @@ def config1():
     return scene, cam
 
 
-def config2(n: int = 1_000_000, width: int = 1920, height: int = 1080):
+def training_frame(n: int = 1_000_000, width: int = 1920, height: int = 1080):
     """Tile splat forward+backward: 1M Gaussians in U[-5,5]^3, SH degree 3,
     camera at (0,0,-6) looking at the origin, fov_y 60 degrees, 1920x1080."""
     scene = make_scene(n, seed=2, extent=5.0, sh_degree=3)
@@ -176,8 +176,8 @@ def config2(n: int = 1_000_000, width: int = 1920, height: int = 1080):
     return scene, cam
 
 
-def config3(n: int = 200_000, width: int = 1024, height: int = 1024):
-    """Volumetric marcher: 200k Gaussians as in config 2, 1024x1024 rays."""
+def marcher_frame(n: int = 200_000, width: int = 1024, height: int = 1024):
+    """Volumetric marcher: 200k Gaussians in U[-5,5]^3, SH degree 3, 1024x1024 rays."""
     scene = make_scene(n, seed=3, extent=5.0, sh_degree=3)
     t = math.tan(math.radians(30.0))
     cam = look_at_camera([0.0, 0.0, -6.0], [0.0, 0.0, 0.0], width, height, t * width / height, t)
@@ -204,7 +204,7 @@ def sphere_cameras(count: int, seed: int, radius: float = 6.0, resolutions=RESOL
     return cams
 
 
-def config4(n: int = 1_000_000, n_cams: int = 64):
+def multiview_batch(n: int = 1_000_000, n_cams: int = 64):
     scene = make_scene(n, seed=4, extent=5.0, sh_degree=3)
     return scene, sphere_cameras(n_cams, seed=4)
 

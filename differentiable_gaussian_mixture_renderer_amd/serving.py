@@ -63,7 +63,7 @@ class ResidentScene:
 
 
 def make_resident_scenes(count: int, seed: int, n_min: int = 50_000, n_max: int = 3_000_000, device="cuda:0") -> List[ResidentScene]:
-    """Scenes with Gaussian counts log-uniform in [n_min, n_max] (config 2 distributions)."""
+    """Scenes with Gaussian counts log-uniform in [n_min, n_max] (the 1M-Gaussian training-frame distributions)."""
     st = scenes.Stream(seed, 11)
     u = st.uniform(count)
     out = []
