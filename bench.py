@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--tile", default="16x16")
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame in the host sample (0: the whole frame)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -104,7 +104,7 @@ def main():
         loss = (img - target).abs().mean()
         return torch.autograd.grad(loss, params)
 
-    if tile != (16, 16) or args.batch != 256:
+    if tile != (16, 16) or args.batch != 64:
         splatter.DEFAULT_TILING = (tile, args.batch)
 
     for _ in range(args.warmup):

@@ -1,5 +1,5 @@
 """Autotuner for the batched splat: picks the tile shape {8x8, 16x16, 32x8}
-and the number of Gaussians staged in LDS per round {128, 256, 512} by timing
+and the number of Gaussians staged in LDS per round {64, 128, 256} by timing
 forward+backward on the actual workload.
 
 The LDS batch size does not change results; the tile shape does (binning is
@@ -19,7 +19,7 @@ import torch
 from .batching import BatchSplatter
 
 TILES = ((8, 8), (16, 16), (32, 8))
-LDS_BATCHES = (128, 256, 512)
+LDS_BATCHES = (64, 128, 256)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "dgmr_mi355x", "autotune.json")
 
 

@@ -29,7 +29,7 @@ def _cam_struct(cam, bg):
 class BatchSplatter:
     """Forward and backward of one scene over a list of cameras (scenes.Camera)."""
 
-    def __init__(self, tile=(16, 16), batch: int = 256, device="cuda:0"):
+    def __init__(self, tile=(16, 16), batch: int = 64, device="cuda:0"):
         self.tiling = C.tiling_struct(tile, batch)
         self.device = torch.device(device)
         self.capacity = 0
@@ -111,7 +111,7 @@ class BatchSplatter:
 
 
 def render_plain(shs, opacity, means, scales, rotations, cameras: Sequence, bg=(0.0, 0.0, 0.0), sh_degree: int = 3,
-                 grad_fn=None, tile=(16, 16), batch: int = 256):
+                 grad_fn=None, tile=(16, 16), batch: int = 64):
     """One Splatter forward (and backward when grad_fn maps image -> dL/dimage)
     per camera; gradients summed over views."""
     from .python_bindings import splatter

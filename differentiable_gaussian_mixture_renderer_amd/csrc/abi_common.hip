@@ -22,7 +22,7 @@ bool valid_tiling(const dgmr_tiling* t)
     if (!t)
         return false;
     const bool shape = (t->tile_w == 16 && t->tile_h == 16) || (t->tile_w == 8 && t->tile_h == 8) || (t->tile_w == 32 && t->tile_h == 8);
-    const bool batch = t->batch == 128 || t->batch == 256 || t->batch == 512;
+    const bool batch = t->batch == 64 || t->batch == 128 || t->batch == 256;
     return shape && batch;
 }
 

@@ -58,7 +58,7 @@ __device__ __forceinline__ int find_view(const ViewBlend* views, int n_views, in
 // Persistent blend over the global tile queue.  The queue counter starts at 0;
 // every workgroup exits once the counter has passed the last tile.
 template <int TW, int TH, int B, bool BACKWARD>
-__global__ __launch_bounds__(TW* TH) void k_batch_blend(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
+__global__ __launch_bounds__(64) void k_batch_blend(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, const ViewBlend* __restrict__ views,
     int n_views, int total_tiles, uint32_t* __restrict__ queue, float* __restrict__ screen_grads)
 {
@@ -185,26 +185,25 @@ void launch_batch_blend(bool backward, int batch, int grid, hipStream_t st, cons
     const int tt = int(L.total_tiles);
 #define DGMR_BATCH_BLEND(B)                                                                                                            \
     if (backward)                                                                                                                      \
-        k_batch_blend<TW, TH, B, true><<<grid, TW * TH, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt,  \
+        k_batch_blend<TW, TH, B, true><<<grid, 64, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt,  \
             L.queue, L.screen_grads);                                                                                                  \
     else                                                                                                                               \
-        k_batch_blend<TW, TH, B, false><<<grid, TW * TH, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt, \
+        k_batch_blend<TW, TH, B, false><<<grid, 64, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt, \
             L.queue, L.screen_grads);
-    if (batch == 128) {
+    if (batch == 64) {
+        DGMR_BATCH_BLEND(64)
+    } else if (batch == 128) {
         DGMR_BATCH_BLEND(128)
-    } else if (batch == 256) {
-        DGMR_BATCH_BLEND(256)
     } else {
-        DGMR_BATCH_BLEND(512)
+        DGMR_BATCH_BLEND(256)
     }
 #undef DGMR_BATCH_BLEND
 }
 
 void batch_blend(bool backward, const dgmr_tiling& t, hipStream_t st, const BatchLayout& L, int n_views)
 {
-    // persistent grid: a few workgroups per CU, never more than there are tiles
-    const int per_cu = (t.tile_w * t.tile_h == 64) ? 16 : 8;
-    const int grid = int(std::min<int64_t>(L.total_tiles, 256 * per_cu));
+    // persistent grid: one-wave workgroups, 16 per CU, never more than there are tiles
+    const int grid = int(std::min<int64_t>(L.total_tiles, 256 * 16));
     if (t.tile_w == 16)
         launch_batch_blend<16, 16>(backward, t.batch, grid, st, L, n_views);
     else if (t.tile_w == 8)

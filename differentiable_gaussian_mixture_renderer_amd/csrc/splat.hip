@@ -226,7 +226,7 @@ __global__ void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t*
 }
 
 template <int TW, int TH, int B>
-__global__ __launch_bounds__(TW* TH) void k_blend_forward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
+__global__ __launch_bounds__(64) void k_blend_forward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, int width, int height, int grid_x,
     int n_tiles, float bg0, float bg1, float bg2, float* __restrict__ framebuffer, float* __restrict__ final_T)
 {
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(TW* TH) void k_blend_forward(const uint32_t* __rest
 }
 
 template <int TW, int TH, int B>
-__global__ __launch_bounds__(TW* TH) void k_blend_backward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
+__global__ __launch_bounds__(64) void k_blend_backward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, int width, int height, int grid_x,
     int n_tiles, float bg0, float bg1, float bg2, const float* __restrict__ framebuffer, const float* __restrict__ final_T,
     const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
@@ -350,28 +350,28 @@ template <int TW, int TH>
 static void launch_blend_fwd(int batch, int n_tiles, hipStream_t st, const uint32_t* ranges, const uint32_t* pl, const float2* xy,
     const float4* co, const float* rgb, int W, int H, int gx, const double* bg, float* fb, float* T)
 {
-    const dim3 grid(n_tiles), block(TW * TH);
+    const dim3 grid(n_tiles), block(64);
     const float b0 = float(bg[0]), b1 = float(bg[1]), b2 = float(bg[2]);
-    if (batch == 128)
+    if (batch == 64)
+        k_blend_forward<TW, TH, 64><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T);
+    else if (batch == 128)
         k_blend_forward<TW, TH, 128><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T);
-    else if (batch == 256)
-        k_blend_forward<TW, TH, 256><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T);
     else
-        k_blend_forward<TW, TH, 512><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T);
+        k_blend_forward<TW, TH, 256><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T);
 }
 
 template <int TW, int TH>
 static void launch_blend_bwd(int batch, int n_tiles, hipStream_t st, const uint32_t* ranges, const uint32_t* pl, const float2* xy,
     const float4* co, const float* rgb, int W, int H, int gx, const double* bg, const float* fb, const float* T, const float* gfb, float* sg)
 {
-    const dim3 grid(n_tiles), block(TW * TH);
+    const dim3 grid(n_tiles), block(64);
     const float b0 = float(bg[0]), b1 = float(bg[1]), b2 = float(bg[2]);
-    if (batch == 128)
+    if (batch == 64)
+        k_blend_backward<TW, TH, 64><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);
+    else if (batch == 128)
         k_blend_backward<TW, TH, 128><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);
-    else if (batch == 256)
-        k_blend_backward<TW, TH, 256><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);
     else
-        k_blend_backward<TW, TH, 512><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);
+        k_blend_backward<TW, TH, 256><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);
 }
 
 void splat_blend_forward(const dgmr_tiling& t, int n_tiles, hipStream_t st, const uint32_t* ranges, const uint32_t* pl, const float* xy,

@@ -130,213 +130,292 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     out.conic_opacity[o] = make_float4(cov_f[2] / det2, -cov_f[1] / det2, cov_f[0] / det2, weight);
 }
 
+// ------------------------------------------------------------------ blending
+// One wave (64 lanes) blends one tile.  Lane l owns the NP = TW*TH/64 pixels
+// l, l + 64, ... of the tile in row-major order, so each Gaussian read from
+// LDS feeds NP independent pixel chains.  The tile's Gaussians go through LDS
+// in batches of B (B/64 per lane); the next batch is fetched into registers
+// while the current one is blended.
 template <int B>
 struct BlendSmem {
-    float4 geo[B];  // x, y, conic a, conic b (forward: pre-scaled -a/2, -b)
-    float4 col[B];  // conic c (forward: -c/2), opacity, r, g
+    float4 geo[B];  // x, y, -a/2, -b (conic a, b, c)
+    float4 col[B];  // -c/2, opacity, r, g
     float blue[B];
-    uint32_t id[B];
 };
 
-// One tile of the front-to-back blend (splat_forward.cu:242-335): pixels at
-// integer coordinates, alpha = min(0.99, o G), skip alpha < 1/255, stop before
-// T (1 - alpha) < 1e-4.  Must be called by all threads of the block.
+template <int B>
+struct BlendFetch {
+    static constexpr int NS = B / 64;
+    uint32_t id[NS];
+    float2 xy[NS];
+    float4 co[NS];
+    float r[NS], g[NS], b[NS];
+
+    __device__ __forceinline__ void load(uint32_t base, uint32_t last, const uint32_t* __restrict__ point_list, const float2* __restrict__ xy_in,
+        const float4* __restrict__ conic_opacity, const float* __restrict__ rgb)
+    {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const uint32_t k = base + uint32_t(lane_id() + 64 * s);
+            if (k < last) {
+                const uint32_t i = point_list[k];
+                id[s] = i;
+                xy[s] = xy_in[i];
+                co[s] = conic_opacity[i];
+                r[s] = rgb[3 * i];
+                g[s] = rgb[3 * i + 1];
+                b[s] = rgb[3 * i + 2];
+            }
+        }
+    }
+};
+
 template <int TW, int TH, int B>
 __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t first, uint32_t last, int px0, int py0, int width, int height,
     float bg0, float bg1, float bg2, const uint32_t* __restrict__ point_list, const float2* __restrict__ xy,
     const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, float* __restrict__ framebuffer, float* __restrict__ final_T)
 {
-    constexpr int NT = TW * TH;
-    const int px = px0 + int(threadIdx.x) % TW;
-    const int py = py0 + int(threadIdx.x) / TW;
-    const bool inside = px < width && py < height;
-    const float pxf = float(px), pyf = float(py);
-    float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
-    bool done = !inside;
+    constexpr int NP = TW * TH / 64;
+    const int lane = int(lane_id());
+    float pxf[NP], pyf[NP], T[NP], C0[NP], C1[NP], C2[NP];
+    bool live[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int q = lane + 64 * k;
+        const int px = px0 + q % TW, py = py0 + q / TW;
+        pxf[k] = float(px);
+        pyf[k] = float(py);
+        live[k] = px < width && py < height;
+        T[k] = 1.f;
+        C0[k] = C1[k] = C2[k] = 0.f;
+    }
+    BlendFetch<B> f;
+    if (first < last)
+        f.load(first, last, point_list, xy, conic_opacity, rgb);
     for (uint32_t base = first; base < last; base += B) {
-        if (__syncthreads_count(!done) == 0)
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+            any |= live[k];
+        if (__ballot(any) == 0)
             break;
-        for (int i = threadIdx.x; i < B; i += NT) {
-            const uint32_t k = base + uint32_t(i);
-            if (k < last) {
-                const uint32_t id = point_list[k];
-                const float2 p = xy[id];
-                const float4 co = conic_opacity[id];
-                sm.geo[i] = make_float4(p.x, p.y, -0.5f * co.x, -co.y);
-                sm.col[i] = make_float4(-0.5f * co.z, co.w, rgb[3 * id], rgb[3 * id + 1]);
-                sm.blue[i] = rgb[3 * id + 2];
-            }
+        __syncthreads(); // the previous batch has been read
+#pragma unroll
+        for (int s = 0; s < BlendFetch<B>::NS; ++s) {
+            const int i = lane + 64 * s;
+            sm.geo[i] = make_float4(f.xy[s].x, f.xy[s].y, -0.5f * f.co[s].x, -f.co[s].y);
+            sm.col[i] = make_float4(-0.5f * f.co[s].z, f.co[s].w, f.r[s], f.g[s]);
+            sm.blue[i] = f.b[s];
         }
         __syncthreads();
-        if (!done) {
-            const int cnt = int(min<uint32_t>(B, last - base));
-            for (int j = 0; j < cnt; ++j) {
-                const float4 g = sm.geo[j];
-                const float4 c = sm.col[j];
-                const float dx = pxf - g.x, dy = pyf - g.y;
-                const float power = fmaf(g.z * dx, dx, fmaf(c.x * dy, dy, g.w * dx * dy));
-                const float alpha = fminf(0.99f, c.y * __expf(power));
-                if (alpha < 1.0f / 255.0f)
-                    continue;
-                const float test_T = T * (1.f - alpha);
-                if (test_T < 0.0001f) {
-                    done = true;
-                    break;
+        if (base + B < last)
+            f.load(base + B, last, point_list, xy, conic_opacity, rgb);
+        const int cnt = int(min<uint32_t>(B, last - base));
+        for (int j = 0; j < cnt; ++j) {
+            const float4 g = sm.geo[j];
+            const float4 c = sm.col[j];
+            const float blue = sm.blue[j];
+            bool still = false;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                if (live[k]) {
+                    const float dx = pxf[k] - g.x, dy = pyf[k] - g.y;
+                    const float power = fmaf(g.z * dx, dx, fmaf(c.x * dy, dy, g.w * dx * dy));
+                    const float alpha = fminf(0.99f, c.y * __expf(power));
+                    if (alpha >= 1.0f / 255.0f) {
+                        const float test_T = T[k] * (1.f - alpha);
+                        if (test_T < 0.0001f) {
+                            live[k] = false;
+                        } else {
+                            const float wgt = alpha * T[k];
+                            C0[k] = fmaf(c.z, wgt, C0[k]);
+                            C1[k] = fmaf(c.w, wgt, C1[k]);
+                            C2[k] = fmaf(blue, wgt, C2[k]);
+                            T[k] = test_T;
+                        }
+                    }
                 }
-                const float wgt = alpha * T;
-                C0 = fmaf(c.z, wgt, C0);
-                C1 = fmaf(c.w, wgt, C1);
-                C2 = fmaf(sm.blue[j], wgt, C2);
-                T = test_T;
+                still |= live[k];
             }
+            if (__ballot(still) == 0)
+                break;
         }
     }
-    if (!inside)
-        return;
-    const int64_t p = int64_t(py) * width + px;
     const int64_t plane = int64_t(width) * height;
-    framebuffer[p] = fmaf(T, bg0, C0);
-    framebuffer[plane + p] = fmaf(T, bg1, C1);
-    framebuffer[2 * plane + p] = fmaf(T, bg2, C2);
-    final_T[p] = T;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int q = lane + 64 * k;
+        const int px = px0 + q % TW, py = py0 + q / TW;
+        if (px < width && py < height) {
+            const int64_t p = int64_t(py) * width + px;
+            framebuffer[p] = fmaf(T[k], bg0, C0[k]);
+            framebuffer[plane + p] = fmaf(T[k], bg1, C1[k]);
+            framebuffer[2 * plane + p] = fmaf(T[k], bg2, C2[k]);
+            final_T[p] = T[k];
+        }
+    }
 }
 
 template <int B>
 struct BlendSmemBwd {
-    float4 geo[B];  // x, y, conic a, conic b
-    float4 col[B];  // conic c, opacity, r, g
+    float4 geo[B];   // x, y, conic a, conic b
+    float4 col[B];   // conic c, opacity, r, g
     float blue[B];
-    uint32_t id[B];
-    float acc[9][B]; // per-batch gradient sums: xy 2, conic 3, opacity 1, rgb 3
+    float acc[9][B]; // per-batch gradient sums of the tile: xy 2, conic 3, opacity 1, rgb 3
 };
 
 // Blend adjoint of one tile, front to back.  The remaining colour is carried
 // unnormalised (C_final minus what has been composited so far), which is the
 // reference prototype's remaining colour (grad_inria_splatter_render.cu:100-156)
 // times the current transmittance, without its repeated division by (1 - alpha).
-// Per-Gaussian gradients (xy 2, conic 3, opacity 1, rgb 3) are summed over the
-// 64 pixels of a wave with DPP, over the waves of the tile with LDS atomics,
-// and leave the tile as one global atomic per value and Gaussian.
+// Per-Gaussian gradients (xy 2, conic 3, opacity 1, rgb 3) are summed over a
+// lane's pixels in registers, over the wave with DPP, kept per batch in LDS
+// and leave the tile as one global atomic per nonzero value and Gaussian.
 template <int TW, int TH, int B>
 __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_t first, uint32_t last, int px0, int py0, int width,
     int height, float bg0, float bg1, float bg2, const uint32_t* __restrict__ point_list, const float2* __restrict__ xy,
     const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, const float* __restrict__ framebuffer,
     const float* __restrict__ final_T, const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
 {
-    constexpr int NT = TW * TH;
-    const int px = px0 + int(threadIdx.x) % TW;
-    const int py = py0 + int(threadIdx.x) / TW;
-    const bool inside = px < width && py < height;
-    const float pxf = float(px), pyf = float(py);
-    float gC0 = 0.f, gC1 = 0.f, gC2 = 0.f, Tf = 1.f, S0 = 0.f, S1 = 0.f, S2 = 0.f;
-    if (inside) {
-        const int64_t p = int64_t(py) * width + px;
-        const int64_t plane = int64_t(width) * height;
-        gC0 = grad_fb[p];
-        gC1 = grad_fb[plane + p];
-        gC2 = grad_fb[2 * plane + p];
-        Tf = final_T[p];
-        S0 = framebuffer[p] - Tf * bg0;
-        S1 = framebuffer[plane + p] - Tf * bg1;
-        S2 = framebuffer[2 * plane + p] - Tf * bg2;
+    constexpr int NP = TW * TH / 64;
+    constexpr int NS = B / 64;
+    const int lane = int(lane_id());
+    const int64_t plane = int64_t(width) * height;
+    float pxf[NP], pyf[NP], gC0[NP], gC1[NP], gC2[NP], S0[NP], S1[NP], S2[NP], gT_Tf[NP], T[NP];
+    bool live[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int q = lane + 64 * k;
+        const int px = px0 + q % TW, py = py0 + q / TW;
+        pxf[k] = float(px);
+        pyf[k] = float(py);
+        gC0[k] = gC1[k] = gC2[k] = 0.f;
+        S0[k] = S1[k] = S2[k] = 0.f;
+        float Tf = 1.f;
+        if (px < width && py < height) {
+            const int64_t p = int64_t(py) * width + px;
+            gC0[k] = grad_fb[p];
+            gC1[k] = grad_fb[plane + p];
+            gC2[k] = grad_fb[2 * plane + p];
+            Tf = final_T[p];
+            S0[k] = framebuffer[p] - Tf * bg0;
+            S1[k] = framebuffer[plane + p] - Tf * bg1;
+            S2[k] = framebuffer[2 * plane + p] - Tf * bg2;
+        }
+        gT_Tf[k] = (gC0[k] * bg0 + gC1[k] * bg1 + gC2[k] * bg2) * Tf;
+        T[k] = 1.f;
+        live[k] = px < width && py < height && !(gC0[k] == 0.f && gC1[k] == 0.f && gC2[k] == 0.f);
     }
-    const float gT_Tf = (gC0 * bg0 + gC1 * bg1 + gC2 * bg2) * Tf;
-    float T = 1.f;
-    bool done = !inside || (gC0 == 0.f && gC1 == 0.f && gC2 == 0.f);
+    BlendFetch<B> f;
+    if (first < last)
+        f.load(first, last, point_list, xy, conic_opacity, rgb);
     for (uint32_t base = first; base < last; base += B) {
-        if (__syncthreads_count(!done) == 0)
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+            any |= live[k];
+        if (__ballot(any) == 0)
             break;
-        const int cnt = int(min<uint32_t>(B, last - base));
-        for (int i = threadIdx.x; i < B; i += NT) {
-            const uint32_t k = base + uint32_t(i);
-            if (k < last) {
-                const uint32_t id = point_list[k];
-                const float2 p = xy[id];
-                const float4 co = conic_opacity[id];
-                sm.geo[i] = make_float4(p.x, p.y, co.x, co.y);
-                sm.col[i] = make_float4(co.z, co.w, rgb[3 * id], rgb[3 * id + 1]);
-                sm.blue[i] = rgb[3 * id + 2];
-                sm.id[i] = id;
-            }
+        __syncthreads(); // the previous batch has been read and flushed
+        uint32_t ids[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int i = lane + 64 * s;
+            ids[s] = f.id[s];
+            sm.geo[i] = make_float4(f.xy[s].x, f.xy[s].y, f.co[s].x, f.co[s].y);
+            sm.col[i] = make_float4(f.co[s].z, f.co[s].w, f.r[s], f.g[s]);
+            sm.blue[i] = f.b[s];
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 sm.acc[q][i] = 0.f;
         }
         __syncthreads();
+        if (base + B < last)
+            f.load(base + B, last, point_list, xy, conic_opacity, rgb);
+        const int cnt = int(min<uint32_t>(B, last - base));
         for (int j = 0; j < cnt; ++j) {
-            if (__ballot(!done) == 0)
-                break; // the whole wave has finished
             const float4 g = sm.geo[j];
             const float4 c = sm.col[j];
             const float blue = sm.blue[j];
-            float gx = 0.f, gy = 0.f, ga = 0.f, gb = 0.f, gc = 0.f, go = 0.f, gr = 0.f, gg = 0.f, gbl = 0.f;
-            bool contrib = false;
-            if (!done) {
-                const float dx = pxf - g.x, dy = pyf - g.y;
-                const float power = fmaf(-0.5f * g.z * dx, dx, fmaf(-0.5f * c.x * dy, dy, -g.w * dx * dy));
-                const float G = __expf(power);
-                const float raw = c.y * G;
-                const float alpha = fminf(0.99f, raw);
-                if (alpha >= 1.0f / 255.0f) {
-                    const float tk = 1.f - alpha;
-                    const float test_T = T * tk;
-                    if (test_T < 0.0001f) {
-                        done = true;
-                    } else {
-                        contrib = true;
-                        const float wgt = alpha * T;
-                        S0 = fmaf(-c.z, wgt, S0);
-                        S1 = fmaf(-c.w, wgt, S1);
-                        S2 = fmaf(-blue, wgt, S2);
-                        gr = gC0 * wgt;
-                        gg = gC1 * wgt;
-                        gbl = gC2 * wgt;
-                        // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
-                        const float inv_tk = __builtin_amdgcn_rcpf(tk);
-                        const float g_alpha = (gC0 * c.z + gC1 * c.w + gC2 * blue) * T - (gC0 * S0 + gC1 * S1 + gC2 * S2 + gT_Tf) * inv_tk;
-                        const float g_raw = raw >= 0.99f ? 0.f : g_alpha;
-                        go = G * g_raw;
-                        const float gp = c.y * g_raw * G;
-                        ga = gp * (-0.5f * dx * dx);
-                        gb = gp * (-dx * dy);
-                        gc = gp * (-0.5f * dy * dy);
-                        gx = gp * (g.z * dx + g.w * dy);
-                        gy = gp * (c.x * dy + g.w * dx);
-                        T = test_T;
+            float sx = 0.f, sy = 0.f, sa = 0.f, sb = 0.f, sc = 0.f, so = 0.f, sr = 0.f, sg = 0.f, sbl = 0.f;
+            bool contrib = false, still = false;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                if (live[k]) {
+                    const float dx = pxf[k] - g.x, dy = pyf[k] - g.y;
+                    const float power = fmaf(-0.5f * g.z * dx, dx, fmaf(-0.5f * c.x * dy, dy, -g.w * dx * dy));
+                    const float G = __expf(power);
+                    const float raw = c.y * G;
+                    const float alpha = fminf(0.99f, raw);
+                    if (alpha >= 1.0f / 255.0f) {
+                        const float tk = 1.f - alpha;
+                        const float test_T = T[k] * tk;
+                        if (test_T < 0.0001f) {
+                            live[k] = false;
+                        } else {
+                            contrib = true;
+                            const float wgt = alpha * T[k];
+                            S0[k] = fmaf(-c.z, wgt, S0[k]);
+                            S1[k] = fmaf(-c.w, wgt, S1[k]);
+                            S2[k] = fmaf(-blue, wgt, S2[k]);
+                            sr += gC0[k] * wgt;
+                            sg += gC1[k] * wgt;
+                            sbl += gC2[k] * wgt;
+                            // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
+                            const float inv_tk = __builtin_amdgcn_rcpf(tk);
+                            const float g_alpha = (gC0[k] * c.z + gC1[k] * c.w + gC2[k] * blue) * T[k]
+                                - (gC0[k] * S0[k] + gC1[k] * S1[k] + gC2[k] * S2[k] + gT_Tf[k]) * inv_tk;
+                            const float g_raw = raw >= 0.99f ? 0.f : g_alpha;
+                            so += G * g_raw;
+                            const float gp = c.y * g_raw * G;
+                            sa += gp * (-0.5f * dx * dx);
+                            sb += gp * (-dx * dy);
+                            sc += gp * (-0.5f * dy * dy);
+                            sx += gp * (g.z * dx + g.w * dy);
+                            sy += gp * (c.x * dy + g.w * dx);
+                            T[k] = test_T;
+                        }
                     }
                 }
+                still |= live[k];
             }
             if (__ballot(contrib) != 0) {
-                gx = wave_sum_to_last(gx);
-                gy = wave_sum_to_last(gy);
-                ga = wave_sum_to_last(ga);
-                gb = wave_sum_to_last(gb);
-                gc = wave_sum_to_last(gc);
-                go = wave_sum_to_last(go);
-                gr = wave_sum_to_last(gr);
-                gg = wave_sum_to_last(gg);
-                gbl = wave_sum_to_last(gbl);
-                if (lane_id() == 63) {
-                    atomicAdd(&sm.acc[0][j], gx);
-                    atomicAdd(&sm.acc[1][j], gy);
-                    atomicAdd(&sm.acc[2][j], ga);
-                    atomicAdd(&sm.acc[3][j], gb);
-                    atomicAdd(&sm.acc[4][j], gc);
-                    atomicAdd(&sm.acc[5][j], go);
-                    atomicAdd(&sm.acc[6][j], gr);
-                    atomicAdd(&sm.acc[7][j], gg);
-                    atomicAdd(&sm.acc[8][j], gbl);
+                sx = wave_sum_to_last(sx);
+                sy = wave_sum_to_last(sy);
+                sa = wave_sum_to_last(sa);
+                sb = wave_sum_to_last(sb);
+                sc = wave_sum_to_last(sc);
+                so = wave_sum_to_last(so);
+                sr = wave_sum_to_last(sr);
+                sg = wave_sum_to_last(sg);
+                sbl = wave_sum_to_last(sbl);
+                if (lane == 63) {
+                    sm.acc[0][j] = sx;
+                    sm.acc[1][j] = sy;
+                    sm.acc[2][j] = sa;
+                    sm.acc[3][j] = sb;
+                    sm.acc[4][j] = sc;
+                    sm.acc[5][j] = so;
+                    sm.acc[6][j] = sr;
+                    sm.acc[7][j] = sg;
+                    sm.acc[8][j] = sbl;
                 }
             }
+            if (__ballot(still) == 0)
+                break;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < cnt; i += NT) {
-            float* dst = screen_grads + int64_t(sm.id[i]) * 9;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                const float v = sm.acc[q][i];
-                if (v != 0.f)
-                    unsafeAtomicAdd(dst + q, v);
+        for (int s = 0; s < NS; ++s) {
+            const int i = lane + 64 * s;
+            if (i < cnt) {
+                float* dst = screen_grads + int64_t(ids[s]) * 9;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    const float v = sm.acc[q][i];
+                    if (v != 0.f)
+                        unsafeAtomicAdd(dst + q, v);
+                }
             }
         }
     }

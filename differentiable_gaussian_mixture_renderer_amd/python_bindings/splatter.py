@@ -25,7 +25,7 @@ from .. import _common as C
 from .. import _native as N
 
 # tile shape and LDS batch used when the caller passes none
-DEFAULT_TILING = ((16, 16), 256)
+DEFAULT_TILING = ((16, 16), 64)
 
 # cache layout returned by forward (index: content)
 CACHE_FIELDS = ("framebuffer", "rects", "rgb", "rgb_sh_clamped", "depths", "points_xy_image", "conic_opacity",

@@ -131,7 +131,7 @@ class ServerStats:
 
 class ContinuousBatchingServer:
     def __init__(self, resident: List[ResidentScene], hbm_budget_bytes: int, workers: int = 3, max_batch_views: int = 32,
-                 tile=(16, 16), lds_batch: int = 256, device="cuda:0"):
+                 tile=(16, 16), lds_batch: int = 64, device="cuda:0"):
         self.resident = resident
         self.budget = int(hbm_budget_bytes)
         self.max_views = max_batch_views
@@ -238,7 +238,7 @@ class ContinuousBatchingServer:
 class FifoServer:
     """One request at a time, in arrival order, one camera per launch."""
 
-    def __init__(self, resident: List[ResidentScene], tile=(16, 16), lds_batch: int = 256):
+    def __init__(self, resident: List[ResidentScene], tile=(16, 16), lds_batch: int = 64):
         self.resident = resident
         self.tile, self.lds_batch = tile, lds_batch
         self.stats = ServerStats()

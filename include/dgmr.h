@@ -66,7 +66,7 @@ typedef struct dgmr_camera {
  * (constants.h:45-46) and 256-Gaussian batches (splat_forward.cu:268-270). */
 typedef struct dgmr_tiling {
     int32_t tile_w, tile_h;   /* 8x8, 16x16 or 32x8 */
-    int32_t batch;            /* Gaussians staged in LDS per round: 128, 256 or 512 */
+    int32_t batch;            /* Gaussians staged in LDS per round: 64, 128 or 256 */
 } dgmr_tiling;
 
 typedef struct dgmr_stats {

@@ -43,6 +43,12 @@ def test_argument_validation_without_gpu():
     bad = N.Tiling(12, 12, 256)
     assert lib.dgmr_workspace_size(1000, 64, 64, ctypes.byref(bad), 10000, ctypes.byref(size)) == N.DGMR_ERR_ARGUMENT
     assert b"bad argument" in lib.dgmr_last_error()
+    for batch, ok in ((64, True), (128, True), (256, True), (32, False), (512, False)):
+        tb = N.Tiling(16, 16, batch)
+        rc = lib.dgmr_workspace_size(1000, 64, 64, ctypes.byref(tb), 10000, ctypes.byref(size))
+        assert (rc == N.DGMR_OK) == ok, batch
+    # image sides beyond 32768 are rejected (16-bit tile grid)
+    assert lib.dgmr_workspace_size(1000, 40000, 64, ctypes.byref(t), 10000, ctypes.byref(size)) == N.DGMR_ERR_ARGUMENT
     # null pointers are rejected before any device work
     rc = lib.dgmr_splat_forward(None, None, ctypes.byref(t), None, None, None, None, None, 0, None, None)
     assert rc == N.DGMR_ERR_ARGUMENT

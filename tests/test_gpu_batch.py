@@ -24,7 +24,7 @@ def scene_and_cams(n=20_000, n_cams=6, seed=4):
     return scene, cams
 
 
-@pytest.mark.parametrize("tile,batch", [((16, 16), 256), ((8, 8), 128), ((32, 8), 512)])
+@pytest.mark.parametrize("tile,batch", [((16, 16), 64), ((8, 8), 128), ((32, 8), 256)])
 def test_batch_matches_single_views(tile, batch):
     torch = _torch()
     from differentiable_gaussian_mixture_renderer_amd.batching import BatchSplatter, render_plain
