@@ -139,6 +139,7 @@ SIGNATURES = {
     "dgmr_profile_read": (ctypes.c_int, [_P(StageTime), ctypes.c_int32, _P(ctypes.c_int32)]),
     "dgmr_last_error": (ctypes.c_char_p, []),
     "dgmr_version": (ctypes.c_char_p, []),
+    "dgmr_selftest_wave_sum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 _lib = None

@@ -408,6 +408,38 @@ __device__ __forceinline__ float wave_sum_to_last(float v)
     return v;
 }
 
+// Sums eight per-lane values over the wave with three transposing halvings
+// (lanes l / l+32 through v_permlane32_swap, rows 0,2 / 1,3 through
+// v_permlane16_swap, lanes l / l^8 through row_ror:8) and a 3-step DPP sum in
+// groups of 8 lanes.  Lane 8m then holds the total of value kSum8Order[m].
+constexpr int kSum8Order[8] = { 0, 4, 2, 6, 1, 5, 3, 7 };
+
+__device__ __forceinline__ float swap_sum32(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ float swap_sum16(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ float wave_sum8(float v0, float v1, float v2, float v3, float v4, float v5, float v6, float v7)
+{
+    const float w01 = swap_sum32(v0, v1), w23 = swap_sum32(v2, v3), w45 = swap_sum32(v4, v5), w67 = swap_sum32(v6, v7);
+    const float u0 = swap_sum16(w01, w23); // rows hold v0, v2, v1, v3
+    const float u1 = swap_sum16(w45, w67); // rows hold v4, v6, v5, v7
+    const bool hi = (lane_id() & 8u) != 0;
+    const float keep = hi ? u1 : u0, send = hi ? u0 : u1;
+    float x = keep + dpp_f32<0x128>(send); // row_ror:8 pairs lane l with l^8
+    x += dpp_f32<0x141>(x);                // row_half_mirror: l <-> 7-l in groups of 8
+    x += dpp_f32<0x4e>(x);                 // quad_perm [2,3,0,1]
+    x += dpp_f32<0xb1>(x);                 // quad_perm [1,0,3,2]
+    return x;
+}
+
 __device__ __forceinline__ float wave_sum(float v)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wave_sum_to_last(v)), 63));

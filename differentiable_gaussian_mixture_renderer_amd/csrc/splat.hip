@@ -296,6 +296,14 @@ SplatParams make_params(const dgmr_gaussians* g)
 } // namespace
 
 namespace {
+// one wave: out[l] = wave_sum8 of in[q * 64 + l] (q < 8), out[64 + l] = wave_sum_to_last of in[512 + l]
+__global__ __launch_bounds__(64) void k_selftest_wave_sum(const float* __restrict__ in, float* __restrict__ out)
+{
+    const int l = int(threadIdx.x);
+    out[l] = wave_sum8(in[l], in[64 + l], in[128 + l], in[192 + l], in[256 + l], in[320 + l], in[384 + l], in[448 + l]);
+    out[64 + l] = wave_sum_to_last(in[512 + l]);
+}
+
 __global__ void k_view_keys(int64_t n, const uint32_t* __restrict__ ids, int64_t n_per_view, uint32_t* __restrict__ keys)
 {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -584,6 +592,15 @@ int dgmr_splat_backward(const dgmr_gaussians* g, const dgmr_camera* cam, const d
     k_splat_preprocess_backward<<<unsigned(ceil_div64(n, kPreThreads)), kPreThreads, 0, st>>>(n, make_params(g), dc, geom->tiles_touched,
         reinterpret_cast<const float4*>(geom->conic_opacity), geom->rgb_clamped, screen_grads, grad_means, grad_scales, grad_rotations,
         grad_opacity, grad_sh);
+    DGMR_HIP_CHECK(hipGetLastError());
+    return DGMR_OK;
+}
+
+int dgmr_selftest_wave_sum(const float* in, float* out, dgmr_stream stream)
+{
+    DGMR_ARG_CHECK(in && out, "dgmr_selftest_wave_sum: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    k_selftest_wave_sum<<<1, 64, 0, st>>>(in, out);
     DGMR_HIP_CHECK(hipGetLastError());
     return DGMR_OK;
 }

@@ -132,14 +132,18 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
 
 // ------------------------------------------------------------------ blending
 // One wave (64 lanes) blends one tile.  Lane l owns the NP = TW*TH/64 pixels
-// l, l + 64, ... of the tile in row-major order, so each Gaussian read from
-// LDS feeds NP independent pixel chains.  The tile's Gaussians go through LDS
-// in batches of B (B/64 per lane); the next batch is fetched into registers
-// while the current one is blended.
+// l, l + 64, ... of the tile in row-major order: one pixel column, rows
+// RS = 64/TW apart, so the x offset to a Gaussian is shared by the lane's
+// pixels.  The tile's Gaussians go through LDS in batches of B (B/64 per
+// lane); the next batch is fetched into registers while the current one is
+// blended.  The conic is staged pre-scaled by -log2(e)/2 (diagonal) and
+// -log2(e) (off-diagonal), so G = exp2(A dx^2 + B dx dy + C dy^2).
+constexpr float kLog2e = 1.4426950408889634f;
+
 template <int B>
 struct BlendSmem {
-    float4 geo[B];  // x, y, -a/2, -b (conic a, b, c)
-    float4 col[B];  // -c/2, opacity, r, g
+    float4 geo[B];  // x, y, A, B
+    float4 col[B];  // C, opacity, r, g
     float blue[B];
 };
 
@@ -168,7 +172,34 @@ struct BlendFetch {
             }
         }
     }
+
+    // slot s of this lane into LDS, conic pre-scaled
+    __device__ __forceinline__ void stage(float4* geo, float4* col, float* blue, int s) const
+    {
+        const int i = int(lane_id()) + 64 * s;
+        geo[i] = make_float4(xy[s].x, xy[s].y, (-0.5f * kLog2e) * co[s].x, -kLog2e * co[s].y);
+        col[i] = make_float4((-0.5f * kLog2e) * co[s].z, co[s].w, r[s], g[s]);
+        blue[i] = b[s];
+    }
 };
+
+// per-Gaussian terms shared by a lane's pixels (dx is common to them)
+struct LaneTerms {
+    float dx, adx2, bdx;
+    __device__ __forceinline__ LaneTerms(float pxf, const float4& g)
+    {
+        dx = pxf - g.x;
+        adx2 = g.z * dx * dx;
+        bdx = g.w * dx;
+    }
+};
+
+// G of one pixel: exp2 of the pre-scaled quadratic form; the forward and the
+// backward evaluate it with this one sequence of operations
+__device__ __forceinline__ float gauss_at(const LaneTerms& t, float c_scaled, float dy)
+{
+    return __builtin_amdgcn_exp2f(fmaf(fmaf(c_scaled, dy, t.bdx), dy, t.adx2));
+}
 
 template <int TW, int TH, int B>
 __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t first, uint32_t last, int px0, int py0, int width, int height,
@@ -176,14 +207,15 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
     const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, float* __restrict__ framebuffer, float* __restrict__ final_T)
 {
     constexpr int NP = TW * TH / 64;
+    constexpr int RS = 64 / TW;
     const int lane = int(lane_id());
-    float pxf[NP], pyf[NP], T[NP], C0[NP], C1[NP], C2[NP];
+    const int px = px0 + lane % TW;
+    const float pxf = float(px);
+    float pyf[NP], T[NP], C0[NP], C1[NP], C2[NP];
     bool live[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-        const int q = lane + 64 * k;
-        const int px = px0 + q % TW, py = py0 + q / TW;
-        pxf[k] = float(px);
+        const int py = py0 + lane / TW + RS * k;
         pyf[k] = float(py);
         live[k] = px < width && py < height;
         T[k] = 1.f;
@@ -201,12 +233,8 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
             break;
         __syncthreads(); // the previous batch has been read
 #pragma unroll
-        for (int s = 0; s < BlendFetch<B>::NS; ++s) {
-            const int i = lane + 64 * s;
-            sm.geo[i] = make_float4(f.xy[s].x, f.xy[s].y, -0.5f * f.co[s].x, -f.co[s].y);
-            sm.col[i] = make_float4(-0.5f * f.co[s].z, f.co[s].w, f.r[s], f.g[s]);
-            sm.blue[i] = f.b[s];
-        }
+        for (int s = 0; s < BlendFetch<B>::NS; ++s)
+            f.stage(sm.geo, sm.col, sm.blue, s);
         __syncthreads();
         if (base + B < last)
             f.load(base + B, last, point_list, xy, conic_opacity, rgb);
@@ -215,13 +243,12 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
             const float4 g = sm.geo[j];
             const float4 c = sm.col[j];
             const float blue = sm.blue[j];
+            const LaneTerms lt(pxf, g);
             bool still = false;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
                 if (live[k]) {
-                    const float dx = pxf[k] - g.x, dy = pyf[k] - g.y;
-                    const float power = fmaf(g.z * dx, dx, fmaf(c.x * dy, dy, g.w * dx * dy));
-                    const float alpha = fminf(0.99f, c.y * __expf(power));
+                    const float alpha = fminf(0.99f, c.y * gauss_at(lt, c.x, pyf[k] - g.y));
                     if (alpha >= 1.0f / 255.0f) {
                         const float test_T = T[k] * (1.f - alpha);
                         if (test_T < 0.0001f) {
@@ -244,8 +271,7 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
     const int64_t plane = int64_t(width) * height;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-        const int q = lane + 64 * k;
-        const int px = px0 + q % TW, py = py0 + q / TW;
+        const int py = py0 + lane / TW + RS * k;
         if (px < width && py < height) {
             const int64_t p = int64_t(py) * width + px;
             framebuffer[p] = fmaf(T[k], bg0, C0[k]);
@@ -258,8 +284,8 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
 
 template <int B>
 struct BlendSmemBwd {
-    float4 geo[B];   // x, y, conic a, conic b
-    float4 col[B];   // conic c, opacity, r, g
+    float4 geo[B];   // x, y, A, B
+    float4 col[B];   // C, opacity, r, g
     float blue[B];
     float acc[9][B]; // per-batch gradient sums of the tile: xy 2, conic 3, opacity 1, rgb 3
 };
@@ -268,9 +294,11 @@ struct BlendSmemBwd {
 // unnormalised (C_final minus what has been composited so far), which is the
 // reference prototype's remaining colour (grad_inria_splatter_render.cu:100-156)
 // times the current transmittance, without its repeated division by (1 - alpha).
-// Per-Gaussian gradients (xy 2, conic 3, opacity 1, rgb 3) are summed over a
-// lane's pixels in registers, over the wave with DPP, kept per batch in LDS
-// and leave the tile as one global atomic per nonzero value and Gaussian.
+// Per Gaussian, a lane sums over its pixels the moments sum(gp), sum(gp dy),
+// sum(gp dy^2) of dL/dpower and turns them into the conic and position
+// gradients with its shared dx; the nine per-Gaussian sums (xy 2, conic 3,
+// opacity 1, rgb 3) are reduced over the wave, kept per batch in LDS and leave
+// the tile as one global atomic per nonzero value and Gaussian.
 template <int TW, int TH, int B>
 __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_t first, uint32_t last, int px0, int py0, int width,
     int height, float bg0, float bg1, float bg2, const uint32_t* __restrict__ point_list, const float2* __restrict__ xy,
@@ -278,21 +306,25 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
     const float* __restrict__ final_T, const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
 {
     constexpr int NP = TW * TH / 64;
+    constexpr int RS = 64 / TW;
     constexpr int NS = B / 64;
+    // d(power)/d(conic) in the pre-scaled units: conic = scaled * kUnscale
+    constexpr float kUnscaleDiag = -2.f / kLog2e, kUnscaleOff = -1.f / kLog2e;
     const int lane = int(lane_id());
     const int64_t plane = int64_t(width) * height;
-    float pxf[NP], pyf[NP], gC0[NP], gC1[NP], gC2[NP], S0[NP], S1[NP], S2[NP], gT_Tf[NP], T[NP];
+    const int px = px0 + lane % TW;
+    const float pxf = float(px);
+    float pyf[NP], gC0[NP], gC1[NP], gC2[NP], S0[NP], S1[NP], S2[NP], gT_Tf[NP], T[NP];
     bool live[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-        const int q = lane + 64 * k;
-        const int px = px0 + q % TW, py = py0 + q / TW;
-        pxf[k] = float(px);
+        const int py = py0 + lane / TW + RS * k;
         pyf[k] = float(py);
         gC0[k] = gC1[k] = gC2[k] = 0.f;
         S0[k] = S1[k] = S2[k] = 0.f;
         float Tf = 1.f;
-        if (px < width && py < height) {
+        const bool inside = px < width && py < height;
+        if (inside) {
             const int64_t p = int64_t(py) * width + px;
             gC0[k] = grad_fb[p];
             gC1[k] = grad_fb[plane + p];
@@ -304,11 +336,13 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
         }
         gT_Tf[k] = (gC0[k] * bg0 + gC1[k] * bg1 + gC2[k] * bg2) * Tf;
         T[k] = 1.f;
-        live[k] = px < width && py < height && !(gC0[k] == 0.f && gC1[k] == 0.f && gC2[k] == 0.f);
+        live[k] = inside && !(gC0[k] == 0.f && gC1[k] == 0.f && gC2[k] == 0.f);
     }
     BlendFetch<B> f;
     if (first < last)
         f.load(first, last, point_list, xy, conic_opacity, rgb);
+    // lane 8m stores wave_sum8's total of value kSum8Order[m]; lane 63 the ninth
+    const int slot8 = (lane & 7) == 0 ? kSum8Order[lane >> 3] : (lane == 63 ? 8 : -1);
     for (uint32_t base = first; base < last; base += B) {
         bool any = false;
 #pragma unroll
@@ -322,9 +356,7 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
         for (int s = 0; s < NS; ++s) {
             const int i = lane + 64 * s;
             ids[s] = f.id[s];
-            sm.geo[i] = make_float4(f.xy[s].x, f.xy[s].y, f.co[s].x, f.co[s].y);
-            sm.col[i] = make_float4(f.co[s].z, f.co[s].w, f.r[s], f.g[s]);
-            sm.blue[i] = f.b[s];
+            f.stage(sm.geo, sm.col, sm.blue, s);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 sm.acc[q][i] = 0.f;
@@ -337,14 +369,14 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             const float4 g = sm.geo[j];
             const float4 c = sm.col[j];
             const float blue = sm.blue[j];
-            float sx = 0.f, sy = 0.f, sa = 0.f, sb = 0.f, sc = 0.f, so = 0.f, sr = 0.f, sg = 0.f, sbl = 0.f;
+            const LaneTerms lt(pxf, g);
+            float m0 = 0.f, m1 = 0.f, m2 = 0.f, so = 0.f, sr = 0.f, sg = 0.f, sbl = 0.f;
             bool contrib = false, still = false;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
                 if (live[k]) {
-                    const float dx = pxf[k] - g.x, dy = pyf[k] - g.y;
-                    const float power = fmaf(-0.5f * g.z * dx, dx, fmaf(-0.5f * c.x * dy, dy, -g.w * dx * dy));
-                    const float G = __expf(power);
+                    const float dy = pyf[k] - g.y;
+                    const float G = gauss_at(lt, c.x, dy);
                     const float raw = c.y * G;
                     const float alpha = fminf(0.99f, raw);
                     if (alpha >= 1.0f / 255.0f) {
@@ -358,21 +390,19 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                             S0[k] = fmaf(-c.z, wgt, S0[k]);
                             S1[k] = fmaf(-c.w, wgt, S1[k]);
                             S2[k] = fmaf(-blue, wgt, S2[k]);
-                            sr += gC0[k] * wgt;
-                            sg += gC1[k] * wgt;
-                            sbl += gC2[k] * wgt;
+                            sr = fmaf(gC0[k], wgt, sr);
+                            sg = fmaf(gC1[k], wgt, sg);
+                            sbl = fmaf(gC2[k], wgt, sbl);
                             // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
                             const float inv_tk = __builtin_amdgcn_rcpf(tk);
                             const float g_alpha = (gC0[k] * c.z + gC1[k] * c.w + gC2[k] * blue) * T[k]
                                 - (gC0[k] * S0[k] + gC1[k] * S1[k] + gC2[k] * S2[k] + gT_Tf[k]) * inv_tk;
                             const float g_raw = raw >= 0.99f ? 0.f : g_alpha;
-                            so += G * g_raw;
-                            const float gp = c.y * g_raw * G;
-                            sa += gp * (-0.5f * dx * dx);
-                            sb += gp * (-dx * dy);
-                            sc += gp * (-0.5f * dy * dy);
-                            sx += gp * (g.z * dx + g.w * dy);
-                            sy += gp * (c.x * dy + g.w * dx);
+                            so = fmaf(G, g_raw, so);
+                            const float gp = raw * g_raw; // dL/dpower
+                            m0 += gp;
+                            m1 = fmaf(gp, dy, m1);
+                            m2 = fmaf(gp * dy, dy, m2);
                             T[k] = test_T;
                         }
                     }
@@ -380,26 +410,18 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                 still |= live[k];
             }
             if (__ballot(contrib) != 0) {
-                sx = wave_sum_to_last(sx);
-                sy = wave_sum_to_last(sy);
-                sa = wave_sum_to_last(sa);
-                sb = wave_sum_to_last(sb);
-                sc = wave_sum_to_last(sc);
-                so = wave_sum_to_last(so);
-                sr = wave_sum_to_last(sr);
-                sg = wave_sum_to_last(sg);
-                sbl = wave_sum_to_last(sbl);
-                if (lane == 63) {
-                    sm.acc[0][j] = sx;
-                    sm.acc[1][j] = sy;
-                    sm.acc[2][j] = sa;
-                    sm.acc[3][j] = sb;
-                    sm.acc[4][j] = sc;
-                    sm.acc[5][j] = so;
-                    sm.acc[6][j] = sr;
-                    sm.acc[7][j] = sg;
-                    sm.acc[8][j] = sbl;
-                }
+                // power = -a/2 dx^2 - b dx dy - c/2 dy^2 with dx = px - x, dy = py - y
+                const float a = g.z * kUnscaleDiag, b = g.w * kUnscaleOff, cc = c.x * kUnscaleDiag;
+                const float dx = lt.dx;
+                const float ga = -0.5f * dx * dx * m0;
+                const float gb = -dx * m1;
+                const float gc = -0.5f * m2;
+                const float gx = fmaf(a * dx, m0, b * m1);
+                const float gy = fmaf(cc, m1, b * dx * m0);
+                const float t8 = wave_sum8(gx, gy, ga, gb, gc, so, sr, sg);
+                const float t9 = wave_sum_to_last(sbl);
+                if (slot8 >= 0)
+                    sm.acc[slot8][j] = slot8 == 8 ? t9 : t8;
             }
             if (__ballot(still) == 0)
                 break;

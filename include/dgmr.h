@@ -196,6 +196,13 @@ typedef struct dgmr_stage_time {
 int dgmr_profile_enable(int on);
 int dgmr_profile_read(dgmr_stage_time* out, int32_t max_stages, int32_t* n_stages);
 
+/* ------------------------------------------------------------ self-test
+ * Runs the blend backward's wave reductions on one wave: in holds 9 x 64
+ * floats (device), out receives 128 (device): out[8m] = sum over lanes of
+ * in[q*64 + lane] for q = {0,4,2,6,1,5,3,7}[m], out[64 + 63] = sum of in[512..575].
+ * Diagnostic only; no counterpart in the reference. */
+int dgmr_selftest_wave_sum(const float* in, float* out, dgmr_stream stream);
+
 /* ------------------------------------------------------------ misc */
 const char* dgmr_last_error(void);
 const char* dgmr_version(void);
