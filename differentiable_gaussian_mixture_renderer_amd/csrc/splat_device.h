@@ -244,24 +244,21 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
             const float4 c = sm.col[j];
             const float blue = sm.blue[j];
             const LaneTerms lt(pxf, g);
+            // branch-free: every pixel evaluates, selects apply the blend rule
             bool still = false;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
-                if (live[k]) {
-                    const float alpha = fminf(0.99f, c.y * gauss_at(lt, c.x, pyf[k] - g.y));
-                    if (alpha >= 1.0f / 255.0f) {
-                        const float test_T = T[k] * (1.f - alpha);
-                        if (test_T < 0.0001f) {
-                            live[k] = false;
-                        } else {
-                            const float wgt = alpha * T[k];
-                            C0[k] = fmaf(c.z, wgt, C0[k]);
-                            C1[k] = fmaf(c.w, wgt, C1[k]);
-                            C2[k] = fmaf(blue, wgt, C2[k]);
-                            T[k] = test_T;
-                        }
-                    }
-                }
+                const float alpha = fminf(0.99f, c.y * gauss_at(lt, c.x, pyf[k] - g.y));
+                const float test_T = T[k] * (1.f - alpha);
+                const bool blend = live[k] && alpha >= 1.0f / 255.0f;
+                const bool stop = blend && test_T < 0.0001f;
+                const bool use = blend && !stop;
+                const float wgt = alpha * T[k];
+                C0[k] = use ? fmaf(c.z, wgt, C0[k]) : C0[k];
+                C1[k] = use ? fmaf(c.w, wgt, C1[k]) : C1[k];
+                C2[k] = use ? fmaf(blue, wgt, C2[k]) : C2[k];
+                T[k] = use ? test_T : T[k];
+                live[k] = live[k] && !stop;
                 still |= live[k];
             }
             if (__ballot(still) == 0)
@@ -370,46 +367,49 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             const float4 c = sm.col[j];
             const float blue = sm.blue[j];
             const LaneTerms lt(pxf, g);
-            float m0 = 0.f, m1 = 0.f, m2 = 0.f, so = 0.f, sr = 0.f, sg = 0.f, sbl = 0.f;
-            bool contrib = false, still = false;
+            // cheap part for every pixel: the blend decisions
+            float Gk[NP], rawk[NP], alphak[NP], tkk[NP], testk[NP], dyk[NP];
+            bool use[NP];
+            bool any_use = false, still = false;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
-                if (live[k]) {
-                    const float dy = pyf[k] - g.y;
-                    const float G = gauss_at(lt, c.x, dy);
-                    const float raw = c.y * G;
-                    const float alpha = fminf(0.99f, raw);
-                    if (alpha >= 1.0f / 255.0f) {
-                        const float tk = 1.f - alpha;
-                        const float test_T = T[k] * tk;
-                        if (test_T < 0.0001f) {
-                            live[k] = false;
-                        } else {
-                            contrib = true;
-                            const float wgt = alpha * T[k];
-                            S0[k] = fmaf(-c.z, wgt, S0[k]);
-                            S1[k] = fmaf(-c.w, wgt, S1[k]);
-                            S2[k] = fmaf(-blue, wgt, S2[k]);
-                            sr = fmaf(gC0[k], wgt, sr);
-                            sg = fmaf(gC1[k], wgt, sg);
-                            sbl = fmaf(gC2[k], wgt, sbl);
-                            // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
-                            const float inv_tk = __builtin_amdgcn_rcpf(tk);
-                            const float g_alpha = (gC0[k] * c.z + gC1[k] * c.w + gC2[k] * blue) * T[k]
-                                - (gC0[k] * S0[k] + gC1[k] * S1[k] + gC2[k] * S2[k] + gT_Tf[k]) * inv_tk;
-                            const float g_raw = raw >= 0.99f ? 0.f : g_alpha;
-                            so = fmaf(G, g_raw, so);
-                            const float gp = raw * g_raw; // dL/dpower
-                            m0 += gp;
-                            m1 = fmaf(gp, dy, m1);
-                            m2 = fmaf(gp * dy, dy, m2);
-                            T[k] = test_T;
-                        }
-                    }
-                }
+                dyk[k] = pyf[k] - g.y;
+                Gk[k] = gauss_at(lt, c.x, dyk[k]);
+                rawk[k] = c.y * Gk[k];
+                alphak[k] = fminf(0.99f, rawk[k]);
+                tkk[k] = 1.f - alphak[k];
+                testk[k] = T[k] * tkk[k];
+                const bool blend = live[k] && alphak[k] >= 1.0f / 255.0f;
+                const bool stop = blend && testk[k] < 0.0001f;
+                use[k] = blend && !stop;
+                any_use |= use[k];
+                live[k] = live[k] && !stop;
                 still |= live[k];
             }
-            if (__ballot(contrib) != 0) {
+            if (__ballot(any_use) != 0) {
+                float m0 = 0.f, m1 = 0.f, m2 = 0.f, so = 0.f, sr = 0.f, sg = 0.f, sbl = 0.f;
+                // selects keep the sums exact for pixels that do not blend this Gaussian
+#pragma unroll
+                for (int k = 0; k < NP; ++k) {
+                    const float wgt = alphak[k] * T[k];
+                    S0[k] = use[k] ? fmaf(-c.z, wgt, S0[k]) : S0[k];
+                    S1[k] = use[k] ? fmaf(-c.w, wgt, S1[k]) : S1[k];
+                    S2[k] = use[k] ? fmaf(-blue, wgt, S2[k]) : S2[k];
+                    sr = use[k] ? fmaf(gC0[k], wgt, sr) : sr;
+                    sg = use[k] ? fmaf(gC1[k], wgt, sg) : sg;
+                    sbl = use[k] ? fmaf(gC2[k], wgt, sbl) : sbl;
+                    // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
+                    const float inv_tk = __builtin_amdgcn_rcpf(tkk[k]);
+                    const float g_alpha = (gC0[k] * c.z + gC1[k] * c.w + gC2[k] * blue) * T[k]
+                        - (gC0[k] * S0[k] + gC1[k] * S1[k] + gC2[k] * S2[k] + gT_Tf[k]) * inv_tk;
+                    const float g_raw = (use[k] && rawk[k] < 0.99f) ? g_alpha : 0.f;
+                    so = fmaf(Gk[k], g_raw, so);
+                    const float gp = rawk[k] * g_raw; // dL/dpower
+                    m0 += gp;
+                    m1 = fmaf(gp, dyk[k], m1);
+                    m2 = fmaf(gp * dyk[k], dyk[k], m2);
+                    T[k] = use[k] ? testk[k] : T[k];
+                }
                 // power = -a/2 dx^2 - b dx dy - c/2 dy^2 with dx = px - x, dy = py - y
                 const float a = g.z * kUnscaleDiag, b = g.w * kUnscaleOff, cc = c.x * kUnscaleDiag;
                 const float dx = lt.dx;

@@ -169,6 +169,60 @@ void orc_splat_render(void* p, const void* bg, int x0, int y0, int x1, int y1, v
     }
 }
 
+// Work profile of the blend (float restatement), per tile: `traversed` = list
+// entries visited before every pixel of the tile has stopped (or the list
+// ends), `blended` = entries that reach alpha >= 1/255 at some live pixel.
+// Used to size launches; not a parity reference.
+void orc_splat_tile_work(void* p, uint32_t* traversed, uint32_t* blended)
+{
+    auto* h = static_cast<SplatHandle*>(p);
+    const auto& g = h->gf;
+    const auto& b = h->bins;
+    const int W = h->sf.cam.width, H = h->sf.cam.height;
+    const int n_tiles = b.grid_x * b.grid_y;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int tile = 0; tile < n_tiles; ++tile) {
+        const uint32_t first = b.ranges[2 * tile], last = b.ranges[2 * tile + 1];
+        const int px0 = (tile % b.grid_x) * b.tile_w, py0 = (tile / b.grid_x) * b.tile_h;
+        const int np = b.tile_w * b.tile_h;
+        std::vector<float> T(np, 1.f);
+        std::vector<uint8_t> live(np, 0);
+        int n_live = 0;
+        for (int q = 0; q < np; ++q) {
+            const int px = px0 + q % b.tile_w, py = py0 + q / b.tile_w;
+            live[q] = px < W && py < H;
+            n_live += live[q];
+        }
+        uint32_t trav = 0, blend = 0;
+        for (uint32_t k = first; k < last && n_live > 0; ++k) {
+            ++trav;
+            const uint32_t id = b.point_list[k];
+            bool any = false;
+            for (int q = 0; q < np; ++q) {
+                if (!live[q])
+                    continue;
+                const float dx = float(px0 + q % b.tile_w) - g.xy[2 * id], dy = float(py0 + q / b.tile_w) - g.xy[2 * id + 1];
+                const float* c = &g.conic[3 * id];
+                const float power = -0.5f * (c[0] * dx * dx + c[2] * dy * dy) - c[1] * dx * dy;
+                const float alpha = std::min(0.99f, g.weight2d[id] * std::exp(power));
+                if (alpha < 1.0f / 255.0f)
+                    continue;
+                any = true;
+                const float test_T = T[q] * (1.f - alpha);
+                if (test_T < 0.0001f) {
+                    live[q] = 0;
+                    --n_live;
+                    continue;
+                }
+                T[q] = test_T;
+            }
+            blend += any;
+        }
+        traversed[tile] = trav;
+        blended[tile] = blend;
+    }
+}
+
 // grads of the screen-space quantities (optional, may be null) and of the parameters
 void orc_splat_backward(void* p, const void* bg, int x0, int y0, int x1, int y1, const void* fb, const void* T_final,
     const void* grad_fb, void* g_means, void* g_scales, void* g_rotations, void* g_opacity, void* g_sh,
