@@ -211,12 +211,12 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
     const int lane = int(lane_id());
     const int px = px0 + lane % TW;
     const float pxf = float(px);
-    float pyf[NP], T[NP], C0[NP], C1[NP], C2[NP];
+    const float pyf0 = float(py0 + lane / TW);
+    float T[NP], C0[NP], C1[NP], C2[NP];
     bool live[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int py = py0 + lane / TW + RS * k;
-        pyf[k] = float(py);
         live[k] = px < width && py < height;
         T[k] = 1.f;
         C0[k] = C1[k] = C2[k] = 0.f;
@@ -248,7 +248,7 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
             bool still = false;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
-                const float alpha = fminf(0.99f, c.y * gauss_at(lt, c.x, pyf[k] - g.y));
+                const float alpha = fminf(0.99f, c.y * gauss_at(lt, c.x, pyf0 + float(RS * k) - g.y));
                 const float test_T = T[k] * (1.f - alpha);
                 const bool blend = live[k] && alpha >= 1.0f / 255.0f;
                 const bool stop = blend && test_T < 0.0001f;
@@ -311,27 +311,27 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
     const int64_t plane = int64_t(width) * height;
     const int px = px0 + lane % TW;
     const float pxf = float(px);
-    float pyf[NP], gC0[NP], gC1[NP], gC2[NP], S0[NP], S1[NP], S2[NP], gT_Tf[NP], T[NP];
+    const float pyf0 = float(py0 + lane / TW);
+    // S: the framebuffer minus what has been composited so far, i.e. the
+    // remaining colour plus T_final * background, whose dot with dL/dC is the
+    // remaining-colour term plus dL/dT_final * T_final of the alpha adjoint
+    float gC0[NP], gC1[NP], gC2[NP], S0[NP], S1[NP], S2[NP], T[NP];
     bool live[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int py = py0 + lane / TW + RS * k;
-        pyf[k] = float(py);
         gC0[k] = gC1[k] = gC2[k] = 0.f;
         S0[k] = S1[k] = S2[k] = 0.f;
-        float Tf = 1.f;
         const bool inside = px < width && py < height;
         if (inside) {
             const int64_t p = int64_t(py) * width + px;
             gC0[k] = grad_fb[p];
             gC1[k] = grad_fb[plane + p];
             gC2[k] = grad_fb[2 * plane + p];
-            Tf = final_T[p];
-            S0[k] = framebuffer[p] - Tf * bg0;
-            S1[k] = framebuffer[plane + p] - Tf * bg1;
-            S2[k] = framebuffer[2 * plane + p] - Tf * bg2;
+            S0[k] = framebuffer[p];
+            S1[k] = framebuffer[plane + p];
+            S2[k] = framebuffer[2 * plane + p];
         }
-        gT_Tf[k] = (gC0[k] * bg0 + gC1[k] * bg1 + gC2[k] * bg2) * Tf;
         T[k] = 1.f;
         live[k] = inside && !(gC0[k] == 0.f && gC1[k] == 0.f && gC2[k] == 0.f);
     }
@@ -368,19 +368,15 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             const float blue = sm.blue[j];
             const LaneTerms lt(pxf, g);
             // cheap part for every pixel: the blend decisions
-            float Gk[NP], rawk[NP], alphak[NP], tkk[NP], testk[NP], dyk[NP];
+            float Gk[NP], alphak[NP];
             bool use[NP];
             bool any_use = false, still = false;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
-                dyk[k] = pyf[k] - g.y;
-                Gk[k] = gauss_at(lt, c.x, dyk[k]);
-                rawk[k] = c.y * Gk[k];
-                alphak[k] = fminf(0.99f, rawk[k]);
-                tkk[k] = 1.f - alphak[k];
-                testk[k] = T[k] * tkk[k];
+                Gk[k] = gauss_at(lt, c.x, pyf0 + float(RS * k) - g.y);
+                alphak[k] = fminf(0.99f, c.y * Gk[k]);
                 const bool blend = live[k] && alphak[k] >= 1.0f / 255.0f;
-                const bool stop = blend && testk[k] < 0.0001f;
+                const bool stop = blend && T[k] * (1.f - alphak[k]) < 0.0001f;
                 use[k] = blend && !stop;
                 any_use |= use[k];
                 live[k] = live[k] && !stop;
@@ -391,6 +387,11 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                 // selects keep the sums exact for pixels that do not blend this Gaussian
 #pragma unroll
                 for (int k = 0; k < NP; ++k) {
+                    // the same operations as the cheap part, so the same values
+                    const float dy = pyf0 + float(RS * k) - g.y;
+                    const float raw = c.y * Gk[k];
+                    const float tk = 1.f - alphak[k];
+                    const float test_T = T[k] * tk;
                     const float wgt = alphak[k] * T[k];
                     S0[k] = use[k] ? fmaf(-c.z, wgt, S0[k]) : S0[k];
                     S1[k] = use[k] ? fmaf(-c.w, wgt, S1[k]) : S1[k];
@@ -399,16 +400,16 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                     sg = use[k] ? fmaf(gC1[k], wgt, sg) : sg;
                     sbl = use[k] ? fmaf(gC2[k], wgt, sbl) : sbl;
                     // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
-                    const float inv_tk = __builtin_amdgcn_rcpf(tkk[k]);
+                    const float inv_tk = __builtin_amdgcn_rcpf(tk);
                     const float g_alpha = (gC0[k] * c.z + gC1[k] * c.w + gC2[k] * blue) * T[k]
-                        - (gC0[k] * S0[k] + gC1[k] * S1[k] + gC2[k] * S2[k] + gT_Tf[k]) * inv_tk;
-                    const float g_raw = (use[k] && rawk[k] < 0.99f) ? g_alpha : 0.f;
+                        - (gC0[k] * S0[k] + gC1[k] * S1[k] + gC2[k] * S2[k]) * inv_tk;
+                    const float g_raw = (use[k] && raw < 0.99f) ? g_alpha : 0.f;
                     so = fmaf(Gk[k], g_raw, so);
-                    const float gp = rawk[k] * g_raw; // dL/dpower
+                    const float gp = raw * g_raw; // dL/dpower
                     m0 += gp;
-                    m1 = fmaf(gp, dyk[k], m1);
-                    m2 = fmaf(gp * dyk[k], dyk[k], m2);
-                    T[k] = use[k] ? testk[k] : T[k];
+                    m1 = fmaf(gp, dy, m1);
+                    m2 = fmaf(gp * dy, dy, m2);
+                    T[k] = use[k] ? test_T : T[k];
                 }
                 // power = -a/2 dx^2 - b dx dy - c/2 dy^2 with dx = px - x, dy = py - y
                 const float a = g.z * kUnscaleDiag, b = g.w * kUnscaleOff, cc = c.x * kUnscaleDiag;
