@@ -55,33 +55,34 @@ __device__ __forceinline__ int find_view(const ViewBlend* views, int n_views, in
     return lo;
 }
 
-// Persistent blend over the global tile queue.  The queue counter starts at 0;
-// every workgroup exits once the counter has passed the last tile.
+// Persistent blend over the global tile queue: kBlendWaves independent waves
+// per workgroup, each popping tiles for itself.  The queue counter starts at
+// 0; a wave exits once the counter has passed the last tile.
 template <int TW, int TH, int B, bool BACKWARD>
-__global__ __launch_bounds__(64) void k_batch_blend(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
+__global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, const ViewBlend* __restrict__ views,
     int n_views, int total_tiles, uint32_t* __restrict__ queue, float* __restrict__ screen_grads)
 {
-    __shared__ typename std::conditional<BACKWARD, BlendSmemBwd<B>, BlendSmem<B>>::type sm;
-    __shared__ int s_tile;
+    using Smem = typename std::conditional<BACKWARD, BlendSmemBwd<B>, BlendSmem<B>>::type;
+    __shared__ Smem sm_all[kBlendWaves];
+    Smem& sm = sm_all[threadIdx.x / 64];
     while (true) {
-        if (threadIdx.x == 0)
-            s_tile = int(atomicAdd(queue, 1u));
-        __syncthreads();
-        const int tile = s_tile;
-        __syncthreads();
+        uint32_t t = 0;
+        if (lane_id() == 0)
+            t = atomicAdd(queue, 1u);
+        const int tile = __shfl(int(t), 0, 64);
         if (tile >= total_tiles)
             break;
         const ViewBlend& vb = views[find_view(views, n_views, tile)];
         const int local = tile - vb.tile_base;
         const int px0 = (local % vb.grid_x) * TW, py0 = (local / vb.grid_x) * TH;
         if constexpr (BACKWARD)
-            blend_tile_backward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
+            blend_tile_backward<TW, TH, B>(sm, uniform(ranges[2 * tile]), uniform(ranges[2 * tile + 1]), px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
                 vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T, vb.grad_fb, screen_grads);
         else
-            blend_tile_forward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
+            blend_tile_forward<TW, TH, B>(sm, uniform(ranges[2 * tile]), uniform(ranges[2 * tile + 1]), px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
                 vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T);
-        __syncthreads();
+        wave_lds_sync();
     }
 }
 
@@ -185,10 +186,10 @@ void launch_batch_blend(bool backward, int batch, int grid, hipStream_t st, cons
     const int tt = int(L.total_tiles);
 #define DGMR_BATCH_BLEND(B)                                                                                                            \
     if (backward)                                                                                                                      \
-        k_batch_blend<TW, TH, B, true><<<grid, 64, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt,  \
+        k_batch_blend<TW, TH, B, true><<<grid, 64 * kBlendWaves, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt,  \
             L.queue, L.screen_grads);                                                                                                  \
     else                                                                                                                               \
-        k_batch_blend<TW, TH, B, false><<<grid, 64, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt, \
+        k_batch_blend<TW, TH, B, false><<<grid, 64 * kBlendWaves, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt, \
             L.queue, L.screen_grads);
     if (batch == 64) {
         DGMR_BATCH_BLEND(64)
@@ -202,8 +203,8 @@ void launch_batch_blend(bool backward, int batch, int grid, hipStream_t st, cons
 
 void batch_blend(bool backward, const dgmr_tiling& t, hipStream_t st, const BatchLayout& L, int n_views)
 {
-    // persistent grid: one-wave workgroups, 16 per CU, never more than there are tiles
-    const int grid = int(std::min<int64_t>(L.total_tiles, 256 * 16));
+    // persistent grid: 32 waves per CU, never more waves than there are tiles
+    const int grid = int(std::min<int64_t>(ceil_div64(L.total_tiles, kBlendWaves), 256 * 32 / kBlendWaves));
     if (t.tile_w == 16)
         launch_batch_blend<16, 16>(backward, t.batch, grid, st, L, n_views);
     else if (t.tile_w == 8)

@@ -389,6 +389,9 @@ __device__ __forceinline__ double portable_log(double x)
 // ------------------------------------------------------------- wave64 helpers
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 
+// a value the caller knows to be equal on all lanes, moved to a scalar register
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf>
 __device__ __forceinline__ float dpp_f32(float v)
 {

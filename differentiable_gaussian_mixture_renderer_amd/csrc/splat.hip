@@ -225,26 +225,37 @@ __global__ void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t*
         ranges[2 * tile + 1] = k + 1;
 }
 
+// kBlendWaves independent waves per workgroup, one tile each, LDS private per wave
 template <int TW, int TH, int B>
-__global__ __launch_bounds__(64) void k_blend_forward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
+__global__ __launch_bounds__(64 * kBlendWaves) void k_blend_forward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, int width, int height, int grid_x,
     int n_tiles, float bg0, float bg1, float bg2, float* __restrict__ framebuffer, float* __restrict__ final_T)
 {
-    __shared__ BlendSmem<B> sm;
-    const int tile = xcd_tile(blockIdx.x, n_tiles);
-    blend_tile_forward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], (tile % grid_x) * TW, (tile / grid_x) * TH, width, height, bg0, bg1,
+    __shared__ BlendSmem<B> sm_all[kBlendWaves];
+    const int wave = int(uniform(threadIdx.x / 64));
+    const int tile = xcd_tile(blockIdx.x, gridDim.x) * kBlendWaves + wave;
+    if (tile >= n_tiles)
+        return;
+    BlendSmem<B>& sm = sm_all[wave];
+    const uint32_t first = uniform(ranges[2 * tile]), last = uniform(ranges[2 * tile + 1]);
+    blend_tile_forward<TW, TH, B>(sm, first, last, (tile % grid_x) * TW, (tile / grid_x) * TH, width, height, bg0, bg1,
         bg2, point_list, xy, conic_opacity, rgb, framebuffer, final_T);
 }
 
 template <int TW, int TH, int B>
-__global__ __launch_bounds__(64) void k_blend_backward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
+__global__ __launch_bounds__(64 * kBlendWaves) void k_blend_backward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, int width, int height, int grid_x,
     int n_tiles, float bg0, float bg1, float bg2, const float* __restrict__ framebuffer, const float* __restrict__ final_T,
     const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
 {
-    __shared__ BlendSmemBwd<B> sm;
-    const int tile = xcd_tile(blockIdx.x, n_tiles);
-    blend_tile_backward<TW, TH, B>(sm, ranges[2 * tile], ranges[2 * tile + 1], (tile % grid_x) * TW, (tile / grid_x) * TH, width, height, bg0, bg1,
+    __shared__ BlendSmemBwd<B> sm_all[kBlendWaves];
+    const int wave = int(uniform(threadIdx.x / 64));
+    const int tile = xcd_tile(blockIdx.x, gridDim.x) * kBlendWaves + wave;
+    if (tile >= n_tiles)
+        return;
+    BlendSmemBwd<B>& sm = sm_all[wave];
+    const uint32_t first = uniform(ranges[2 * tile]), last = uniform(ranges[2 * tile + 1]);
+    blend_tile_backward<TW, TH, B>(sm, first, last, (tile % grid_x) * TW, (tile / grid_x) * TH, width, height, bg0, bg1,
         bg2, point_list, xy, conic_opacity, rgb, framebuffer, final_T, grad_fb, screen_grads);
 }
 
@@ -358,7 +369,7 @@ template <int TW, int TH>
 static void launch_blend_fwd(int batch, int n_tiles, hipStream_t st, const uint32_t* ranges, const uint32_t* pl, const float2* xy,
     const float4* co, const float* rgb, int W, int H, int gx, const double* bg, float* fb, float* T)
 {
-    const dim3 grid(n_tiles), block(64);
+    const dim3 grid(unsigned(ceil_div64(n_tiles, kBlendWaves))), block(64 * kBlendWaves);
     const float b0 = float(bg[0]), b1 = float(bg[1]), b2 = float(bg[2]);
     if (batch == 64)
         k_blend_forward<TW, TH, 64><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T);
@@ -372,7 +383,7 @@ template <int TW, int TH>
 static void launch_blend_bwd(int batch, int n_tiles, hipStream_t st, const uint32_t* ranges, const uint32_t* pl, const float2* xy,
     const float4* co, const float* rgb, int W, int H, int gx, const double* bg, const float* fb, const float* T, const float* gfb, float* sg)
 {
-    const dim3 grid(n_tiles), block(64);
+    const dim3 grid(unsigned(ceil_div64(n_tiles, kBlendWaves))), block(64 * kBlendWaves);
     const float b0 = float(bg[0]), b1 = float(bg[1]), b2 = float(bg[2]);
     if (batch == 64)
         k_blend_backward<TW, TH, 64><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);

@@ -139,6 +139,7 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
 // blended.  The conic is staged pre-scaled by -log2(e)/2 (diagonal) and
 // -log2(e) (off-diagonal), so G = exp2(A dx^2 + B dx dy + C dy^2).
 constexpr float kLog2e = 1.4426950408889634f;
+constexpr int kBlendWaves = 4; // tile waves per workgroup
 
 template <int B>
 struct BlendSmem {
@@ -147,29 +148,44 @@ struct BlendSmem {
     float blue[B];
 };
 
+// Two-stage fetch of a tile's list: the ids of batch b+2 and the Gaussian data
+// of batch b+1 are in flight while batch b is blended.  Every load is issued
+// unconditionally at an index clamped to the list (the list is non-empty), so
+// no value has to wait for a branch; slots past the end are never read.
 template <int B>
 struct BlendFetch {
     static constexpr int NS = B / 64;
+    uint32_t id_next[NS]; // ids of the batch after the one whose data is in flight
     uint32_t id[NS];
     float2 xy[NS];
     float4 co[NS];
     float r[NS], g[NS], b[NS];
 
-    __device__ __forceinline__ void load(uint32_t base, uint32_t last, const uint32_t* __restrict__ point_list, const float2* __restrict__ xy_in,
-        const float4* __restrict__ conic_opacity, const float* __restrict__ rgb)
+    __device__ __forceinline__ static uint32_t clamp_k(uint32_t base, int s, uint32_t last)
+    {
+        return min(base + uint32_t(lane_id() + 64 * s), last - 1);
+    }
+
+    __device__ __forceinline__ void load_ids(uint32_t base, uint32_t last, const uint32_t* __restrict__ point_list)
+    {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            id_next[s] = point_list[clamp_k(base, s, last)];
+    }
+
+    // data of the batch whose ids are in id_next
+    __device__ __forceinline__ void load_data(const float2* __restrict__ xy_in, const float4* __restrict__ conic_opacity,
+        const float* __restrict__ rgb)
     {
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const uint32_t k = base + uint32_t(lane_id() + 64 * s);
-            if (k < last) {
-                const uint32_t i = point_list[k];
-                id[s] = i;
-                xy[s] = xy_in[i];
-                co[s] = conic_opacity[i];
-                r[s] = rgb[3 * i];
-                g[s] = rgb[3 * i + 1];
-                b[s] = rgb[3 * i + 2];
-            }
+            const uint32_t i = id_next[s];
+            id[s] = i;
+            xy[s] = xy_in[i];
+            co[s] = conic_opacity[i];
+            r[s] = rgb[3 * i];
+            g[s] = rgb[3 * i + 1];
+            b[s] = rgb[3 * i + 2];
         }
     }
 
@@ -182,6 +198,10 @@ struct BlendFetch {
         blue[i] = b[s];
     }
 };
+
+// LDS of one wave is private to it: LDS operations of a wave execute in
+// order, so a compiler barrier orders its staging writes before its reads
+__device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
 
 // per-Gaussian terms shared by a lane's pixels (dx is common to them)
 struct LaneTerms {
@@ -221,48 +241,52 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
         T[k] = 1.f;
         C0[k] = C1[k] = C2[k] = 0.f;
     }
-    BlendFetch<B> f;
-    if (first < last)
-        f.load(first, last, point_list, xy, conic_opacity, rgb);
-    for (uint32_t base = first; base < last; base += B) {
-        bool any = false;
+    if (first < last) {
+        BlendFetch<B> f;
+        f.load_ids(first, last, point_list);
+        f.load_data(xy, conic_opacity, rgb);
+        f.load_ids(first + B, last, point_list);
+        for (uint32_t base = first; base < last; base += B) {
+            bool any = false;
 #pragma unroll
-        for (int k = 0; k < NP; ++k)
-            any |= live[k];
-        if (__ballot(any) == 0)
-            break;
-        __syncthreads(); // the previous batch has been read
-#pragma unroll
-        for (int s = 0; s < BlendFetch<B>::NS; ++s)
-            f.stage(sm.geo, sm.col, sm.blue, s);
-        __syncthreads();
-        if (base + B < last)
-            f.load(base + B, last, point_list, xy, conic_opacity, rgb);
-        const int cnt = int(min<uint32_t>(B, last - base));
-        for (int j = 0; j < cnt; ++j) {
-            const float4 g = sm.geo[j];
-            const float4 c = sm.col[j];
-            const float blue = sm.blue[j];
-            const LaneTerms lt(pxf, g);
-            // branch-free: every pixel evaluates, selects apply the blend rule
-            bool still = false;
-#pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                const float alpha = fminf(0.99f, c.y * gauss_at(lt, c.x, pyf0 + float(RS * k) - g.y));
-                const float test_T = T[k] * (1.f - alpha);
-                const bool blend = live[k] && alpha >= 1.0f / 255.0f;
-                const bool stop = blend && test_T < 0.0001f;
-                const bool use = blend && !stop;
-                const float wgt = alpha * T[k];
-                C0[k] = use ? fmaf(c.z, wgt, C0[k]) : C0[k];
-                C1[k] = use ? fmaf(c.w, wgt, C1[k]) : C1[k];
-                C2[k] = use ? fmaf(blue, wgt, C2[k]) : C2[k];
-                T[k] = use ? test_T : T[k];
-                live[k] = live[k] && !stop;
-                still |= live[k];
-            }
-            if (__ballot(still) == 0)
+            for (int k = 0; k < NP; ++k)
+                any |= live[k];
+            if (__ballot(any) == 0)
                 break;
+            wave_lds_sync(); // the previous batch has been read
+#pragma unroll
+            for (int s = 0; s < BlendFetch<B>::NS; ++s)
+                f.stage(sm.geo, sm.col, sm.blue, s);
+            wave_lds_sync();
+            // unconditional (clamped) so the loaded registers need no merge after a branch
+            f.load_data(xy, conic_opacity, rgb);
+            f.load_ids(base + 2 * B, last, point_list);
+            const int cnt = int(min<uint32_t>(B, last - base));
+            for (int j = 0; j < cnt; ++j) {
+                const float4 g = sm.geo[j];
+                const float4 c = sm.col[j];
+                const float blue = sm.blue[j];
+                const LaneTerms lt(pxf, g);
+                // branch-free: every pixel evaluates, selects apply the blend rule
+                bool still = false;
+#pragma unroll
+                for (int k = 0; k < NP; ++k) {
+                    const float alpha = fminf(0.99f, c.y * gauss_at(lt, c.x, pyf0 + float(RS * k) - g.y));
+                    const float test_T = T[k] * (1.f - alpha);
+                    const bool blend = live[k] && alpha >= 1.0f / 255.0f;
+                    const bool stop = blend && test_T < 0.0001f;
+                    const bool use = blend && !stop;
+                    const float wgt = alpha * T[k];
+                    C0[k] = use ? fmaf(c.z, wgt, C0[k]) : C0[k];
+                    C1[k] = use ? fmaf(c.w, wgt, C1[k]) : C1[k];
+                    C2[k] = use ? fmaf(blue, wgt, C2[k]) : C2[k];
+                    T[k] = use ? test_T : T[k];
+                    live[k] = live[k] && !stop;
+                    still |= live[k];
+                }
+                if (__ballot(still) == 0)
+                    break;
+            }
         }
     }
     const int64_t plane = int64_t(width) * height;
@@ -335,11 +359,33 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
         T[k] = 1.f;
         live[k] = inside && !(gC0[k] == 0.f && gC1[k] == 0.f && gC2[k] == 0.f);
     }
+    if (first >= last)
+        return;
     BlendFetch<B> f;
-    if (first < last)
-        f.load(first, last, point_list, xy, conic_opacity, rgb);
+    f.load_ids(first, last, point_list);
+    f.load_data(xy, conic_opacity, rgb);
+    f.load_ids(first + B, last, point_list);
     // lane 8m stores wave_sum8's total of value kSum8Order[m]; lane 63 the ninth
     const int slot8 = (lane & 7) == 0 ? kSum8Order[lane >> 3] : (lane == 63 ? 8 : -1);
+    // a batch's sums leave for global memory at the top of the next batch,
+    // after its data has arrived, so no wait on the loads covers the atomics
+    uint32_t ids[NS];
+    int cnt = 0;
+    const auto flush = [&]() {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int i = lane + 64 * s;
+            if (i < cnt) {
+                float* dst = screen_grads + int64_t(ids[s]) * 9;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    const float v = sm.acc[q][i];
+                    if (v != 0.f)
+                        unsafeAtomicAdd(dst + q, v);
+                }
+            }
+        }
+    };
     for (uint32_t base = first; base < last; base += B) {
         bool any = false;
 #pragma unroll
@@ -347,8 +393,8 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             any |= live[k];
         if (__ballot(any) == 0)
             break;
-        __syncthreads(); // the previous batch has been read and flushed
-        uint32_t ids[NS];
+        wave_lds_sync(); // the previous batch has been read
+        flush();
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int i = lane + 64 * s;
@@ -358,10 +404,11 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             for (int q = 0; q < 9; ++q)
                 sm.acc[q][i] = 0.f;
         }
-        __syncthreads();
-        if (base + B < last)
-            f.load(base + B, last, point_list, xy, conic_opacity, rgb);
-        const int cnt = int(min<uint32_t>(B, last - base));
+        wave_lds_sync();
+        // unconditional (clamped) so the loaded registers need no merge after a branch
+        f.load_data(xy, conic_opacity, rgb);
+        f.load_ids(base + 2 * B, last, point_list);
+        cnt = int(min<uint32_t>(B, last - base));
         for (int j = 0; j < cnt; ++j) {
             const float4 g = sm.geo[j];
             const float4 c = sm.col[j];
@@ -427,21 +474,9 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             if (__ballot(still) == 0)
                 break;
         }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int i = lane + 64 * s;
-            if (i < cnt) {
-                float* dst = screen_grads + int64_t(ids[s]) * 9;
-#pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    const float v = sm.acc[q][i];
-                    if (v != 0.f)
-                        unsafeAtomicAdd(dst + q, v);
-                }
-            }
-        }
     }
+    wave_lds_sync();
+    flush();
 }
 
 // Per-Gaussian adjoint of the preprocess for one camera: conic -> covariance
