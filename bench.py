@@ -2,7 +2,8 @@
 
 A step is one training iteration of the hot path on one frame: parameter
 activations (exp of log-scales, quaternion normalisation, sigmoid of opacity
-logits), splat forward, L1 loss against a seeded uniform-noise target, and the
+logits; fused into the splatter's preprocess unless --torch-activations),
+splat forward, L1 loss against a seeded uniform-noise target, and the
 backward to every parameter (means, log-scales, quaternions, opacity logits,
 SH).  Inputs are resident in HBM before the timed region.
 
@@ -37,6 +38,8 @@ def parse():
     ap.add_argument("--tile", default="16x16")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
+    ap.add_argument("--torch-activations", action="store_true",
+                    help="activate the parameters with torch ops before the splatter instead of inside its kernels")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame in the host sample (0: the whole frame)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (calibrates HBM byte counters) and exit")
@@ -96,11 +99,17 @@ def main():
     campos = torch.tensor(cam.camera_center)
     bg = torch.zeros(3)
 
-    def render(sc, op, rot):
-        return splatter.Splatter.apply(sh, op, means, sc, rot, view, proj, campos, bg, W, H, cam.tan_fovx, cam.tan_fovy, 3)
+    def render():
+        if args.torch_activations:
+            return splatter.Splatter.apply(sh, torch.sigmoid(logits), means, torch.exp(log_scales),
+                                           torch.nn.functional.normalize(quats, dim=-1), view, proj, campos, bg, W, H, cam.tan_fovx,
+                                           cam.tan_fovy, 3)
+        # exp / normalise / sigmoid inside the preprocess and its backward
+        return splatter.SplatterActivated.apply(sh, logits, means, log_scales, quats, view, proj, campos, bg, W, H, cam.tan_fovx,
+                                                cam.tan_fovy, 3)
 
     def step():
-        img = render(torch.exp(log_scales), torch.sigmoid(logits), torch.nn.functional.normalize(quats, dim=-1))
+        img = render()
         loss = (img - target).abs().mean()
         return torch.autograd.grad(loss, params)
 
@@ -169,7 +178,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 scene: means U[-5,5]^3, log-scales N(-3,0.5), SH degree 3; uniform L1 target)",
         "config": {"workload": f"tile splat fwd+bwd, {args.n} Gaussians SH3, {W}x{H}, L1 loss", "tile": f"{tile[0]}x{tile[1]}",
-                   "lds_batch": args.batch, "pairs": R, "visible": n_visible},
+                   "lds_batch": args.batch, "pairs": R, "visible": n_visible,
+                   "activations": "torch" if args.torch_activations else "fused"},
         "roofline": {"bound": "hbm", "kernel": d_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(d_avg_ms, 4), "algorithmic_bytes_per_launch": d_bytes / max(d_launches, 1)},

@@ -51,7 +51,8 @@ def prepare_inputs(shs, opacity, means3D, scales, rotations, float32_only: bool)
     return [t.contiguous() for t in tensors]
 
 
-def gaussians_struct(shs, opacity, means3D, scales, rotations, sh_degree: int, scale_multiplier: float = 1.0) -> N.Gaussians:
+def gaussians_struct(shs, opacity, means3D, scales, rotations, sh_degree: int, scale_multiplier: float = 1.0,
+                     activation: int = N.DGMR_ACT_NONE) -> N.Gaussians:
     g = N.Gaussians()
     g.n = means3D.shape[0]
     g.dtype = dtype_code(means3D)
@@ -63,6 +64,7 @@ def gaussians_struct(shs, opacity, means3D, scales, rotations, sh_degree: int, s
     g.sh_coeffs = shs.shape[1]
     g.sh_degree = int(sh_degree)
     g.scale_multiplier = float(scale_multiplier)
+    g.activation = int(activation)
     return g
 
 

@@ -19,6 +19,8 @@ DGMR_ERR_WORKSPACE = 3
 DGMR_ERR_DEVICE = 4
 DGMR_F32 = 0
 DGMR_F64 = 1
+DGMR_ACT_NONE = 0
+DGMR_ACT_LOG_NORM_SIGMOID = 1
 
 c_void_p = ctypes.c_void_p
 
@@ -35,6 +37,7 @@ class Gaussians(ctypes.Structure):
         ("sh_coeffs", ctypes.c_int32),
         ("sh_degree", ctypes.c_int32),
         ("scale_multiplier", ctypes.c_double),
+        ("activation", ctypes.c_int32),
     ]
 
 

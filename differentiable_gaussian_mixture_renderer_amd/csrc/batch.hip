@@ -111,6 +111,7 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
                 preprocess_backward_gaussian(i, p, row, cams[v], conic_opacity[o], clamped + 3 * o, screen_grads + o * 9, g_mu, g_s, g_q, &g_w,
                     g_shv);
         }
+        activation_backward(p, i, g_s, g_q, &g_w);
         for (int a = 0; a < 3; ++a) {
             g_means[3 * i + a] = g_mu[a];
             g_scales[3 * i + a] = g_s[a];
@@ -276,7 +277,7 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     if (n > 0) {
         StageScope sc("batch.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 67.0);
         const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
-            static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier) };
+            static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), g->activation };
         k_batch_preprocess<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams,
             SplatOut { L.rects, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, L.depths, dkeys, dids });
         DGMR_HIP_CHECK(hipGetLastError());
@@ -339,7 +340,7 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
     }
     StageScope sc("batch.preprocess_backward", st, double(n) * 2.0 * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 40.0);
     const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
-        static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier) };
+        static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), g->activation };
     k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams, L.tiles, L.conic_opacity, L.clamped,
         L.screen_grads, grad_means, grad_scales, grad_rotations, grad_opacity, grad_sh);
     DGMR_HIP_CHECK(hipGetLastError());

@@ -157,24 +157,26 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     hist[int64_t(threadIdx.x) * n_blocks + blockIdx.x] = c;
 }
 
-// Stable scatter.  The block's 4096 items are first sorted by digit in LDS:
-// chunks of 256 consecutive items are ranked with wave ballots (lanes holding
-// the same digit), waves are ordered through LDS, chunks are processed in
-// order, so equal digits keep their input order.  The LDS copy is then written
-// out in digit order, so each digit's items leave the block as one contiguous
-// run instead of 4-byte scattered stores.
+// Stable scatter.  Wave w of the block owns items [1024w, 1024w + 1024) of
+// the block's 4096 and walks them in 16 rounds of 64: lanes holding the same
+// digit find each other with 8 ballots, the lowest of them advances the
+// wave's running count of that digit, and every item keeps its rank inside
+// (wave, digit).  One block scan over (digit, wave) then places every item in
+// LDS in digit order (stable: digit, wave, round, lane = input order), and the
+// LDS copy leaves the block one contiguous run per digit.  Four barriers per
+// block.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int n_blocks,
     const uint32_t* __restrict__ offsets)
 {
+    constexpr int kWaves = kSortThreads / 64;
+    constexpr int kRounds = kSortTile / kSortThreads; // rounds of 64 items per wave
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
-    __shared__ uint32_t s_local[kRadix];    // block-local start of each digit
-    __shared__ uint32_t s_global[kRadix];   // global start of each digit for this block
-    __shared__ uint32_t s_run[kRadix];      // running local position per digit
-    __shared__ uint32_t s_wave_cnt[4][kRadix];
-    __shared__ uint32_t s_wave_off[4][kRadix];
-    __shared__ uint32_t s_scan[4];
+    __shared__ uint32_t s_count[kWaves][kRadix]; // per-wave digit counts, then per-wave starts in LDS order
+    __shared__ uint32_t s_local[kRadix];         // block-local start of each digit
+    __shared__ uint32_t s_global[kRadix];        // global start of each digit for this block
+    __shared__ uint32_t s_scan[kWaves];
     const int64_t n = sort_count(n_max, n_dev);
     const int64_t base = int64_t(blockIdx.x) * kSortTile;
     if (base >= n)
@@ -182,51 +184,23 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     const int count = int(min<int64_t>(kSortTile, n - base));
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / 64;
-    // block histogram
-    s_run[threadIdx.x] = 0;
-    for (int w = 0; w < 4; ++w)
-        s_wave_cnt[w][threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t keys[kSortItems];
-#pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        const int i = j * kSortThreads + threadIdx.x;
-        keys[j] = i < count ? keys_in[base + i] : 0u;
-        if (i < count)
-            atomicAdd(&s_run[(keys[j] >> shift) & 0xffu], 1u);
-    }
-    __syncthreads();
-    // exclusive scan of the 256 digit counts
-    {
-        const uint32_t c = s_run[threadIdx.x];
-        uint32_t v = c;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(v, d, 64);
-            if (lane >= unsigned(d))
-                v += o;
-        }
-        if (lane == 63)
-            s_scan[wave] = v;
-        __syncthreads();
-        uint32_t prefix = 0;
-        for (unsigned w = 0; w < wave; ++w)
-            prefix += s_scan[w];
-        const uint32_t excl = prefix + v - c;
-        s_local[threadIdx.x] = excl;
-        s_run[threadIdx.x] = excl;
-        s_global[threadIdx.x] = offsets[int64_t(threadIdx.x) * n_blocks + blockIdx.x];
-    }
-    __syncthreads();
     const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
-    for (int j = 0; j < kSortItems; ++j) {
-        if (j * kSortThreads >= count)
-            break; // uniform across the block
-        const int i = j * kSortThreads + threadIdx.x;
+    for (int w = 0; w < kWaves; ++w)
+        s_count[w][threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t keys[kRounds], vals[kRounds], rank[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
         const bool valid = i < count;
-        const uint32_t key = keys[j];
-        const uint32_t val = valid ? vals_in[base + i] : 0u;
-        const uint32_t digit = (key >> shift) & 0xffu;
+        keys[r] = valid ? keys_in[base + i] : 0u;
+        vals[r] = valid ? vals_in[base + i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
+        const bool valid = i < count;
+        const uint32_t digit = (keys[r] >> shift) & 0xffu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -234,26 +208,53 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        const uint32_t rank = __popcll(peers & lt_mask);
+        // the wave's LDS operations run in order: every lane reads the running
+        // count before the lowest lane of its digit group writes it back
+        const uint32_t before = s_count[wave][digit];
+        rank[r] = before + uint32_t(__popcll(peers & lt_mask));
         if (valid && (peers & lt_mask) == 0)
-            s_wave_cnt[wave][digit] = __popcll(peers);
-        __syncthreads();
-        {
-            uint32_t run = s_run[threadIdx.x];
+            s_count[wave][digit] = before + uint32_t(__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {
+        // thread d: starts of digit d per wave, block-exclusive over digits
+        const uint32_t d = threadIdx.x;
+        uint32_t c[kWaves], tot = 0;
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                const uint32_t c = s_wave_cnt[w][threadIdx.x];
-                s_wave_off[w][threadIdx.x] = run;
-                run += c;
-                s_wave_cnt[w][threadIdx.x] = 0;
-            }
-            s_run[threadIdx.x] = run;
+        for (int w = 0; w < kWaves; ++w) {
+            c[w] = s_count[w][d];
+            tot += c[w];
         }
+        uint32_t v = tot;
+#pragma unroll
+        for (int k = 1; k < 64; k <<= 1) {
+            const uint32_t o = __shfl_up(v, k, 64);
+            if (lane >= unsigned(k))
+                v += o;
+        }
+        if (lane == 63)
+            s_scan[wave] = v;
         __syncthreads();
-        if (valid) {
-            const uint32_t pos = s_wave_off[wave][digit] + rank;
-            s_keys[pos] = key;
-            s_vals[pos] = val;
+        uint32_t start = v - tot;
+        for (unsigned w = 0; w < wave; ++w)
+            start += s_scan[w];
+        s_local[d] = start;
+        s_global[d] = offsets[int64_t(d) * n_blocks + blockIdx.x];
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            s_count[w][d] = start;
+            start += c[w];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
+        if (i < count) {
+            const uint32_t pos = s_count[wave][(keys[r] >> shift) & 0xffu] + rank[r];
+            s_keys[pos] = keys[r];
+            s_vals[pos] = vals[r];
         }
     }
     __syncthreads();

@@ -281,6 +281,7 @@ __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64
     if (i < n && tiles[i] != 0)
         preprocess_backward_gaussian(i, p, row, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9, g_mu, g_s, g_q, &g_w, g_shv);
     if (i < n) {
+        activation_backward(p, i, g_s, g_q, &g_w);
         for (int a = 0; a < 3; ++a) {
             g_means[3 * i + a] = g_mu[a];
             g_scales[3 * i + a] = g_s[a];
@@ -301,7 +302,8 @@ __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64
 SplatParams make_params(const dgmr_gaussians* g)
 {
     return SplatParams { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
-        static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier) };
+        static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier),
+        g->activation };
 }
 
 } // namespace

@@ -16,7 +16,60 @@ struct SplatParams {
     const float* sh;
     int sh_coeffs, sh_degree;
     float scale_multiplier;
+    int activation; // DGMR_ACT_*: 1 = log-scales, unnormalised quaternions, opacity logits
 };
+
+// scales (times the multiplier), rotation and opacity of Gaussian i as the
+// renderer uses them; with activation the stored values are log-scales, an
+// unnormalised quaternion and an opacity logit (exp, q/|q|, sigmoid as
+// torch.exp, torch.nn.functional.normalize and torch.sigmoid compute them)
+__device__ __forceinline__ void gaussian_params(const SplatParams& p, int64_t i, float* s, float* q, float* w)
+{
+    const float m = p.scale_multiplier;
+    if (p.activation) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            s[a] = expf(p.scales[3 * i + a]) * m;
+        const float r[4] = { p.rotations[4 * i], p.rotations[4 * i + 1], p.rotations[4 * i + 2], p.rotations[4 * i + 3] };
+        const float nrm = fmaxf(sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]), 1e-12f);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            q[a] = r[a] / nrm;
+        *w = 1.f / (1.f + expf(-p.opacity[i]));
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            s[a] = p.scales[3 * i + a] * m;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            q[a] = p.rotations[4 * i + a];
+        *w = p.opacity[i];
+    }
+}
+
+// turns gradients w.r.t. the used scale (before the multiplier), rotation and
+// opacity into gradients w.r.t. the stored log-scale, quaternion and logit
+__device__ __forceinline__ void activation_backward(const SplatParams& p, int64_t i, float* g_s, float* g_q, float* g_w)
+{
+    if (!p.activation)
+        return;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        g_s[a] *= expf(p.scales[3 * i + a]);
+    const float r[4] = { p.rotations[4 * i], p.rotations[4 * i + 1], p.rotations[4 * i + 2], p.rotations[4 * i + 3] };
+    const float nrm = fmaxf(sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]), 1e-12f);
+    float qn[4], d = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        qn[a] = r[a] / nrm;
+        d += g_q[a] * qn[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        g_q[a] = (g_q[a] - d * qn[a]) / nrm;
+    const float o = 1.f / (1.f + expf(-p.opacity[i]));
+    *g_w *= o * (1.f - o);
+}
 
 // per-Gaussian outputs of the preprocess; index o = view * n + i
 struct SplatOut {
@@ -73,10 +126,8 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     transform_point(cam.view, mu, t, 3);
     if (t[2] < 0.2f)
         return;
-    const float m = p.scale_multiplier;
-    const float s[3] = { p.scales[3 * i] * m, p.scales[3 * i + 1] * m, p.scales[3 * i + 2] * m };
-    const float q[4] = { p.rotations[4 * i], p.rotations[4 * i + 1], p.rotations[4 * i + 2], p.rotations[4 * i + 3] };
-    const float w = p.opacity[i];
+    float s[3], q[4], w;
+    gaussian_params(p, i, s, q, &w);
     SplatIn<float> in;
     splat_internals(mu, s, q, cam, &in);
     const float cov_f[3] = { in.cov2[0] + 0.3f, in.cov2[1], in.cov2[2] + 0.3f };
@@ -489,8 +540,8 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
 {
     const float mu[3] = { p.means[3 * i], p.means[3 * i + 1], p.means[3 * i + 2] };
     const float m = p.scale_multiplier;
-    const float s[3] = { p.scales[3 * i] * m, p.scales[3 * i + 1] * m, p.scales[3 * i + 2] * m };
-    const float q[4] = { p.rotations[4 * i], p.rotations[4 * i + 1], p.rotations[4 * i + 2], p.rotations[4 * i + 3] };
+    float s[3], q[4], w;
+    gaussian_params(p, i, s, q, &w);
     // conic = cov^-1: dL/dcov = -A G A
     const float G00 = sg[2], G01 = 0.5f * sg[3], G11 = sg[4];
     const float t00 = A.x * G00 + A.y * G01, t01 = A.x * G01 + A.y * G11;
@@ -499,7 +550,7 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
     const float r10 = t10 * A.x + t11 * A.y, r11 = t10 * A.y + t11 * A.z;
     const float g_cov2[3] = { -r00, -(r01 + r10), -r11 };
     float g_w, g_mu[3], g_s[3], g_q[4];
-    splat_ots_grad(p.opacity[i], mu, s, q, cam, 0.3f, sg[5], sg, g_cov2, &g_w, g_mu, g_s, g_q);
+    splat_ots_grad(w, mu, s, q, cam, 0.3f, sg[5], sg, g_cov2, &g_w, g_mu, g_s, g_q);
     const float v[3] = { mu[0] - cam.pos[0], mu[1] - cam.pos[1], mu[2] - cam.pos[2] };
     const float len = sqrtf(dot3(v, v));
     const float dir[3] = { v[0] / len, v[1] / len, v[2] / len };

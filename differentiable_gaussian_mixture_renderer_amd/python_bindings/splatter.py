@@ -34,7 +34,7 @@ CACHE_FIELDS = ("framebuffer", "rects", "rgb", "rgb_sh_clamped", "depths", "poin
 
 def _forward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, proj_matrix, cam_position, background,
              render_width: int, render_height: int, tan_fovx: float, tan_fovy: float, sh_degree: int,
-             tile=None, batch: int = 0, scale_multiplier: float = 1.0):
+             tile=None, batch: int = 0, scale_multiplier: float = 1.0, activation: int = N.DGMR_ACT_NONE):
     tile = tile or DEFAULT_TILING[0]
     batch = batch or DEFAULT_TILING[1]
     shs, weights, centroids, cov_scales, cov_rotations = C.prepare_inputs(shs, weights, centroids, cov_scales, cov_rotations, True)
@@ -44,7 +44,7 @@ def _forward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, pr
     lib = N.load()
     tiling = C.tiling_struct(tile, batch)
     cam = C.camera_struct(view_matrix, proj_matrix, cam_position, background, W, H, tan_fovx, tan_fovy)
-    g = C.gaussians_struct(shs, weights, centroids, cov_scales, cov_rotations, sh_degree, scale_multiplier)
+    g = C.gaussians_struct(shs, weights, centroids, cov_scales, cov_rotations, sh_degree, scale_multiplier, activation)
     gx, gy = (W + tiling.tile_w - 1) // tiling.tile_w, (H + tiling.tile_h - 1) // tiling.tile_h
     f32 = dict(dtype=torch.float32, device=dev)
     with torch.cuda.device(dev):
@@ -87,7 +87,7 @@ def _forward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, pr
 
 def _backward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, proj_matrix, cam_position, background,
               render_width: int, render_height: int, tan_fovx: float, tan_fovy: float, sh_degree: int, grad_framebuffer, cache,
-              tile=None, batch: int = 0, scale_multiplier: float = 1.0):
+              tile=None, batch: int = 0, scale_multiplier: float = 1.0, activation: int = N.DGMR_ACT_NONE):
     tile = tile or DEFAULT_TILING[0]
     batch = batch or DEFAULT_TILING[1]
     shs, weights, centroids, cov_scales, cov_rotations = C.prepare_inputs(shs, weights, centroids, cov_scales, cov_rotations, True)
@@ -98,7 +98,7 @@ def _backward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, p
     lib = N.load()
     tiling = C.tiling_struct(tile, batch)
     cam = C.camera_struct(view_matrix, proj_matrix, cam_position, background, W, H, tan_fovx, tan_fovy)
-    g = C.gaussians_struct(shs, weights, centroids, cov_scales, cov_rotations, sh_degree, scale_multiplier)
+    g = C.gaussians_struct(shs, weights, centroids, cov_scales, cov_rotations, sh_degree, scale_multiplier, activation)
     grad_framebuffer = grad_framebuffer.to(torch.float32).contiguous()
     f32 = dict(dtype=torch.float32, device=dev)
     with torch.cuda.device(dev):
@@ -147,3 +147,33 @@ class Splatter(torch.autograd.Function):
 
 
 apply = Splatter.apply
+
+
+class SplatterActivated(torch.autograd.Function):
+    """Splatter.apply's argument list, with the Gaussians given the way a 3DGS
+    trainer stores them: opacity logits, log-scales and unnormalised
+    quaternions.  exp, q/|q| and sigmoid run inside the preprocess and their
+    derivatives inside the preprocess backward (the same values as
+    torch.exp / F.normalize / torch.sigmoid before Splatter.apply, without the
+    extra kernels); gradients are w.r.t. the stored values."""
+
+    @staticmethod
+    def forward(ctx, shs, opacity_logits, means3D, log_scales, rotations, world_view_transform, full_proj_transform, camera_center,
+                bg_color, image_width: int, image_height: int, tanfovx: float, tanfovy: float, active_sh_degree: int):
+        cache = cpp_binding.forward(shs, opacity_logits, means3D, log_scales, rotations, world_view_transform, full_proj_transform,
+                                    camera_center, bg_color, image_width, image_height, tanfovx, tanfovy, active_sh_degree,
+                                    activation=N.DGMR_ACT_LOG_NORM_SIGMOID)
+        ctx.scalars = (int(image_width), int(image_height), float(tanfovx), float(tanfovy), int(active_sh_degree))
+        ctx.save_for_backward(shs, opacity_logits, means3D, log_scales, rotations, world_view_transform, full_proj_transform,
+                              camera_center, bg_color, *cache)
+        return cache[0]
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        if not grad_output.is_contiguous():
+            grad_output = grad_output.contiguous()
+        shs, logits, means3D, log_scales, rotations, wv, fp, cc, bg, *cache = ctx.saved_tensors
+        W, H, tx, ty, deg = ctx.scalars
+        g = cpp_binding.backward(shs, logits, means3D, log_scales, rotations, wv, fp, cc, bg, W, H, tx, ty, deg, grad_output, cache,
+                                 activation=N.DGMR_ACT_LOG_NORM_SIGMOID)
+        return (*g, None, None, None, None, None, None, None, None, None)

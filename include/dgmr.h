@@ -37,6 +37,14 @@ enum dgmr_status {
 
 enum dgmr_dtype { DGMR_F32 = 0, DGMR_F64 = 1 };
 
+/* DGMR_ACT_NONE: scales, rotations and opacity are used as given (the
+ * reference's convention).  DGMR_ACT_LOG_NORM_SIGMOID (splatter only; no
+ * counterpart in the reference, which leaves activations to the caller):
+ * they are log-scales, unnormalised quaternions and opacity logits, activated
+ * inside the preprocess (exp, q/|q|, sigmoid) and differentiated in the
+ * backward, so the gradients are w.r.t. the stored values. */
+enum dgmr_activation { DGMR_ACT_NONE = 0, DGMR_ACT_LOG_NORM_SIGMOID = 1 };
+
 /* Gaussian parameters as the caller holds them (AoS, contiguous). */
 typedef struct dgmr_gaussians {
     int64_t n;
@@ -49,6 +57,7 @@ typedef struct dgmr_gaussians {
     int32_t sh_coeffs;        /* K: coefficients stored per Gaussian, 1..16 */
     int32_t sh_degree;        /* active degree 0..3 */
     double scale_multiplier;  /* ForwardData::cov_scale_multiplier (1 in the reference binding) */
+    int32_t activation;       /* dgmr_activation: how scales, rotations and opacity are stored */
 } dgmr_gaussians;
 
 /* Pinhole camera, host values.  Replaces the .cpu() copies of

@@ -43,7 +43,7 @@ def lib() -> ctypes.CDLL:
         L.orc_splat_get_geometry.argtypes = [vp] * 9
         L.orc_splat_render.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]
         L.orc_splat_backward.argtypes = [vp, vp, i32, i32, i32, i32] + [vp] * 12
-        L.orc_splat_tile_work.argtypes = [vp, vp, vp]
+        L.orc_splat_tile_work.argtypes = [vp, vp, vp, vp]
         L.orc_march_get_geometry.argtypes = [vp] * 9
         L.orc_march_render.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp]
         L.orc_march_backward.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp] + [vp] * 9
@@ -162,14 +162,16 @@ class SplatOracle(_Handle):
                                  P(out["weight2d"]), P(out["rgb"]))
         return out
 
-    def tile_work(self):
-        """Per tile: list entries traversed before the tile saturates, and entries blended."""
+    def tile_work(self, reachable=False):
+        """Per tile: list entries traversed before the tile saturates, entries
+        blended, and (optionally) entries reaching alpha >= 1/255 anywhere in the tile."""
         gx = (self.W + self.tile[0] - 1) // self.tile[0]
         gy = (self.H + self.tile[1] - 1) // self.tile[1]
         trav = np.zeros(gx * gy, np.uint32)
         blend = np.zeros(gx * gy, np.uint32)
-        lib().orc_splat_tile_work(self.h, P(trav), P(blend))
-        return trav, blend
+        reach = np.zeros(gx * gy, np.uint32) if reachable else None
+        lib().orc_splat_tile_work(self.h, P(trav), P(blend), P(reach) if reachable else None)
+        return (trav, blend, reach) if reachable else (trav, blend)
 
     def time_fwd_bwd(self, bg, grad_fb, crop=None) -> float:
         """Wall seconds of preprocess + binning + blend + backward (float32 restatement)."""

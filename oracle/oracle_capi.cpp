@@ -171,9 +171,11 @@ void orc_splat_render(void* p, const void* bg, int x0, int y0, int x1, int y1, v
 
 // Work profile of the blend (float restatement), per tile: `traversed` = list
 // entries visited before every pixel of the tile has stopped (or the list
-// ends), `blended` = entries that reach alpha >= 1/255 at some live pixel.
-// Used to size launches; not a parity reference.
-void orc_splat_tile_work(void* p, uint32_t* traversed, uint32_t* blended)
+// ends), `blended` = entries that reach alpha >= 1/255 at some live pixel,
+// `reachable` (may be null) = entries that reach alpha >= 1/255 at some pixel
+// of the tile, ignoring termination.  Used to size launches; not a parity
+// reference.
+void orc_splat_tile_work(void* p, uint32_t* traversed, uint32_t* blended, uint32_t* reachable)
 {
     auto* h = static_cast<SplatHandle*>(p);
     const auto& g = h->gf;
@@ -220,6 +222,24 @@ void orc_splat_tile_work(void* p, uint32_t* traversed, uint32_t* blended)
         }
         traversed[tile] = trav;
         blended[tile] = blend;
+        if (reachable) {
+            uint32_t reach = 0;
+            for (uint32_t k = first; k < last; ++k) {
+                const uint32_t id = b.point_list[k];
+                bool any = false;
+                for (int q = 0; q < np && !any; ++q) {
+                    const int px = px0 + q % b.tile_w, py = py0 + q / b.tile_w;
+                    if (px >= W || py >= H)
+                        continue;
+                    const float dx = float(px) - g.xy[2 * id], dy = float(py) - g.xy[2 * id + 1];
+                    const float* c = &g.conic[3 * id];
+                    const float power = -0.5f * (c[0] * dx * dx + c[2] * dy * dy) - c[1] * dx * dy;
+                    any = std::min(0.99f, g.weight2d[id] * std::exp(power)) >= 1.0f / 255.0f;
+                }
+                reach += any;
+            }
+            reachable[tile] = reach;
+        }
     }
 }
 
