@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--torch-activations", action="store_true",
                     help="activate the parameters with torch ops before the splatter instead of inside its kernels")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame in the host sample (0: the whole frame)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-camera batched workload")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--batched-cams", type=int, default=64)
+    ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (calibrates HBM byte counters) and exit")
     return ap.parse_args()
@@ -163,6 +167,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, cam, args.cpu_rows)
+    del means, log_scales, quats, logits, sh, params, target
+    torch.cuda.empty_cache()
+    batched = None if args.no_batched else batched_workload(args.n, args.batched_cams)
+    served = None if args.no_serving else serving_workload(args.serving_requests)
 
     line = {
         "metric": METRIC,
@@ -186,8 +194,107 @@ def main():
         "frame_hbm_fraction": round(frame_hbm_frac, 4),
         "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0])},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": served,
     }
     print(json.dumps(line))
+
+
+def batched_workload(n: int, n_cams: int, reps: int = 3):
+    """One scene seen by n_cams cameras of uneven resolution (256^2 .. 1080p) on
+    the radius-6 sphere: forward + L1 backward of all views through the batched
+    path (autotuned tile shape and LDS batch) against the one-camera-per-launch
+    plain path, same inputs, same per-view gradients."""
+    import numpy as np
+    import torch
+
+    from differentiable_gaussian_mixture_renderer_amd import autotune, scenes
+    from differentiable_gaussian_mixture_renderer_amd.batching import BatchSplatter, render_plain
+
+    scene, cams = scenes.multiview_batch(n=n, n_cams=n_cams)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    params = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations)]
+    gen = torch.Generator(device=dev).manual_seed(4)
+    targets = [torch.rand((3, c.height, c.width), generator=gen, device=dev) for c in cams]
+    l1 = lambda img, v: torch.sign(img - targets[v]) / img.numel()
+    px = sum(c.width * c.height for c in cams)
+
+    (tile, lds), table = autotune.autotune(params, cams, reps=1, use_cache=False)
+    bs = BatchSplatter(tile=tile, batch=lds)
+
+    def batched_step():
+        fbs, _ = bs.forward(*params, cams)
+        return fbs, bs.backward([l1(img, v) for v, img in enumerate(fbs)])
+
+    batched_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fbs, gb = batched_step()
+    torch.cuda.synchronize()
+    ms_b = (time.perf_counter() - t0) / reps * 1e3
+
+    def plain_step():
+        it = iter(range(len(cams)))
+        return render_plain(*params, cams, grad_fn=lambda img: l1(img, next(it)))
+
+    plain_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        imgs, gp = plain_step()
+    torch.cuda.synchronize()
+    ms_p = (time.perf_counter() - t0) / reps * 1e3
+    img_diff = max(float((a - b).abs().max()) for a, b in zip(fbs, imgs))
+    grad_rel = max(float((a - b).norm() / b.norm()) for a, b in zip(gb, gp))
+    return {"workload": f"{n_cams} cameras of one {n}-Gaussian SH3 scene, resolutions 256^2..1920x1080, fwd+L1 bwd",
+            "mpixels_per_batch": round(px / 1e6, 3), "pairs": bs.n_pairs, "tile": f"{tile[0]}x{tile[1]}", "lds_batch": lds,
+            "autotune_ms": table, "batched_ms": round(ms_b, 3), "plain_ms": round(ms_p, 3),
+            "batched_mpx_s": round(px / ms_b / 1e3, 1), "plain_mpx_s": round(px / ms_p / 1e3, 1),
+            "speedup_vs_plain": round(ms_p / ms_b, 3), "max_image_diff_vs_plain": img_diff, "max_grad_rel_l2_vs_plain": grad_rel}
+
+
+def serving_workload(n_requests: int, rate_hz: float = 200.0):
+    """Poisson arrivals at rate_hz over 8 resident scenes (50k..3M Gaussians,
+    log-uniform), cameras 256^2..1080p, 30 % forward+backward: continuous
+    batching against a FIFO server that renders one request at a time through
+    the plain path.  Also both servers with every request present at t=0
+    (saturated), which measures sustainable throughput."""
+    import copy
+
+    import numpy as np
+    import torch
+
+    from differentiable_gaussian_mixture_renderer_amd import serving
+
+    resident = serving.make_resident_scenes(8, seed=5)
+    trace = serving.make_trace(n_requests, rate_hz, 8, seed=5)
+    out = {"workload": f"{n_requests} requests, Poisson {rate_hz:g}/s, 8 resident scenes of 50k..3M Gaussians, 256^2..1080p, "
+                       "30% fwd+bwd", "scenes_gaussians": [sc.n for sc in resident]}
+    budget = 48 << 30
+
+    def run(server_cls, reqs, **kw):
+        srv = server_cls(resident, **kw)
+        st = srv.run(reqs)
+        torch.cuda.synchronize()
+        return st.summary()
+
+    # warm-up of both paths (workspaces, capacities)
+    warm = copy.deepcopy(trace[:8])
+    for r in warm:
+        r.arrival = 0.0
+    run(serving.ContinuousBatchingServer, warm, hbm_budget_bytes=budget)
+    run(serving.FifoServer, copy.deepcopy(warm))
+    out["continuous"] = run(serving.ContinuousBatchingServer, copy.deepcopy(trace), hbm_budget_bytes=budget)
+    out["fifo_plain"] = run(serving.FifoServer, copy.deepcopy(trace))
+    sat = copy.deepcopy(trace)
+    for r in sat:
+        r.arrival = 0.0
+    out["continuous_saturated"] = run(serving.ContinuousBatchingServer, copy.deepcopy(sat), hbm_budget_bytes=budget)
+    out["fifo_plain_saturated"] = run(serving.FifoServer, copy.deepcopy(sat))
+    out["saturated_throughput_ratio"] = round(out["continuous_saturated"]["mpixel_per_s"] / out["fifo_plain_saturated"]["mpixel_per_s"], 3)
+    return out
 
 
 def cpu_baseline(scene, cam, rows: int):

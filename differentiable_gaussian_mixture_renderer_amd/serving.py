@@ -166,6 +166,7 @@ class ContinuousBatchingServer:
                 imgs, _ = splatter.forward(*sc.params, [r.camera for r in batch], sh_degree=sc.sh_degree)
                 grads = None
                 if any(r.backward for r in batch):
+                    # at most one backward request per batch: the summed gradients are its own
                     gimg = [l1_grad(img, r.target_seed) if r.backward else torch.zeros_like(img) for img, r in zip(imgs, batch)]
                     grads = splatter.backward(gimg)
                 stream.synchronize()
@@ -184,17 +185,21 @@ class ContinuousBatchingServer:
     # -------------------------------------------------------------- scheduler
     def _schedule(self):
         """Admits the waiting requests of the scene with the oldest waiting
-        request, as many as fit the budget and the batch limit."""
+        request, as many as fit the budget and the batch limit.  A batch holds
+        at most one forward+backward request: the batched backward sums the
+        gradients of its views, and forward-only views contribute zero."""
         if not self._waiting:
             return False
         scene_id = self._waiting[0].scene
         picked, rest = [], deque()
+        has_backward = False
         while self._waiting:
             r = self._waiting.popleft()
-            if r.scene == scene_id and len(picked) < self.max_views:
+            if r.scene == scene_id and len(picked) < self.max_views and not (r.backward and has_backward):
                 cost = estimate_bytes(self.resident[scene_id], picked + [r])
                 if self._in_flight + cost <= self.budget or (not picked and self._in_flight == 0):
                     picked.append(r)
+                    has_backward |= r.backward
                     continue
             rest.append(r)
         self._waiting = rest
