@@ -48,7 +48,7 @@ def lib() -> ctypes.CDLL:
         L.orc_march_render.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp]
         L.orc_march_backward.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp] + [vp] * 9
         L.orc_time_splat_fwd_bwd.restype = f64
-        L.orc_time_splat_fwd_bwd.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32]
+        L.orc_time_splat_fwd_bwd.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32]
         L.orc_portable_log_f.restype = ctypes.c_float
         L.orc_portable_log_f.argtypes = [ctypes.c_float]
         L.orc_portable_log_d.restype = f64
@@ -173,14 +173,14 @@ class SplatOracle(_Handle):
         lib().orc_splat_tile_work(self.h, P(trav), P(blend), P(reach) if reachable else None)
         return (trav, blend, reach) if reachable else (trav, blend)
 
-    def time_fwd_bwd(self, bg, grad_fb, crop=None) -> float:
-        """Wall seconds of preprocess + binning + blend + backward (float32 restatement)."""
+    def time_fwd_bwd(self, bg, grad_fb, crop=None, threads: int = 1) -> float:
+        """Wall seconds of preprocess + binning + blend + backward (float32 restatement) on `threads` host threads."""
         assert self.dt == np.float32
         x0, y0, x1, y1 = crop or (0, 0, self.W, self.H)
         fb = np.zeros((3, self.H, self.W), np.float32)
         T = np.ones((self.H, self.W), np.float32)
         bgc, gc = _c(bg, np.float32), _c(grad_fb, np.float32)
-        return float(lib().orc_time_splat_fwd_bwd(self.h, P(bgc), P(gc), P(fb), P(T), x0, y0, x1, y1))
+        return float(lib().orc_time_splat_fwd_bwd(self.h, P(bgc), P(gc), P(fb), P(T), x0, y0, x1, y1, int(threads)))
 
 
 class MarchOracle(_Handle):

@@ -4,6 +4,9 @@
 // array crosses as a plain pointer in the scalar type chosen at creation
 // (float32 when dbl == 0, float64 otherwise).
 #include <chrono>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cstdint>
 #include <cstdlib>
 #include <memory>
@@ -650,11 +653,19 @@ int orc_sections_randomised_check(int dbl)
 }
 
 // ------------------------------------------------------------ CPU baseline
-// Times splat forward+backward of the float restatement on the host.
-// Returns wall seconds; n_threads only affects the blend (OpenMP over rows).
-double orc_time_splat_fwd_bwd(void* p, const float* bg, const float* grad_fb, float* fb, float* T_final, int x0, int y0, int x1, int y1)
+// Times splat forward+backward of the float restatement on the host with
+// `threads` OpenMP threads (preprocess and its backward over Gaussians, pair
+// emission over Gaussians, a parallel stable sort, the blend and its adjoint
+// over image rows).  Returns wall seconds.
+double orc_time_splat_fwd_bwd(void* p, const float* bg, const float* grad_fb, float* fb, float* T_final, int x0, int y0, int x1, int y1,
+    int threads)
 {
     auto* h = static_cast<SplatHandle*>(p);
+    threads = std::max(1, threads);
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    omp_set_num_threads(threads);
+#endif
     const auto t0 = std::chrono::steady_clock::now();
     auto& st = h->sf;
     SplatGeometry<float> g;
@@ -663,10 +674,16 @@ double orc_time_splat_fwd_bwd(void* p, const float* bg, const float* grad_fb, fl
     bin(g.rect, g.tiles, g.xy, g.key_depth, st.cam.width, st.cam.height, h->tile_w, h->tile_h, &b);
     splat_blend(g, b, st.cam.width, st.cam.height, bg, x0, y0, x1, y1, fb, T_final, nullptr, nullptr);
     Splat2dGrads<float> gr;
-    splat_blend_backward(g, b, st.cam.width, st.cam.height, bg, x0, y0, x1, y1, fb, T_final, grad_fb, &gr);
+    if (threads > 1)
+        splat_blend_backward_threads(g, b, st.cam.width, st.cam.height, bg, x0, y0, x1, y1, fb, T_final, grad_fb, &gr, threads);
+    else
+        splat_blend_backward(g, b, st.cam.width, st.cam.height, bg, x0, y0, x1, y1, fb, T_final, grad_fb, &gr);
     ParamGrads<float> pg;
     splat_preprocess_backward(st.scene, st.cam, g, gr, &pg);
     const auto t1 = std::chrono::steady_clock::now();
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
     return std::chrono::duration<double>(t1 - t0).count();
 }
 

@@ -10,6 +10,9 @@
 #pragma once
 
 #include <algorithm>
+#ifdef _OPENMP
+#include <parallel/algorithm>
+#endif
 #include <cstdint>
 #include <vector>
 
@@ -69,6 +72,8 @@ inline void splat_preprocess(const Scene<S>& sc, const Camera<S>& cam, int tile_
     g->clamped.assign(3 * n, 0);
     g->depth.assign(n, 0);
     g->key_depth.assign(n, 0);
+    // Gaussians are independent: the thread count does not change any value
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         const S* mu = sc.means + 3 * i;
         S t[3];
@@ -132,8 +137,10 @@ inline void bin(const std::vector<int32_t>& rect, const std::vector<uint32_t>& t
         acc += tiles[i];
         b->offsets[i] = uint32_t(acc);
     }
-    std::vector<std::pair<uint64_t, uint32_t>> kv;
-    kv.reserve(acc);
+    // emitted in Gaussian order, row-major within a rectangle; slot offsets
+    // come from the scan, so the order does not depend on the thread count
+    std::vector<std::pair<uint64_t, uint32_t>> kv(acc);
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int64_t i = 0; i < n; ++i) {
         if (tiles[i] == 0)
             continue;
@@ -141,11 +148,18 @@ inline void bin(const std::vector<int32_t>& rect, const std::vector<uint32_t>& t
         get_rect(xy[2 * i], xy[2 * i + 1], rect[2 * i], rect[2 * i + 1], b->grid_x, b->grid_y, tile_w, tile_h, &x0, &y0, &x1, &y1);
         uint32_t bits;
         std::memcpy(&bits, &key_depth[i], 4);
+        size_t k = b->offsets[i] - tiles[i];
         for (int y = y0; y < y1; ++y)
             for (int x = x0; x < x1; ++x)
-                kv.emplace_back((uint64_t(uint32_t(y * b->grid_x + x)) << 32) | bits, uint32_t(i));
+                kv[k++] = { (uint64_t(uint32_t(y * b->grid_x + x)) << 32) | bits, uint32_t(i) };
     }
-    std::stable_sort(kv.begin(), kv.end(), [](const auto& a, const auto& c) { return a.first < c.first; });
+    // a stable sort has one result; the parallel one reaches it with all threads
+    const auto by_key = [](const auto& a, const auto& c) { return a.first < c.first; };
+#ifdef _OPENMP
+    __gnu_parallel::stable_sort(kv.begin(), kv.end(), by_key);
+#else
+    std::stable_sort(kv.begin(), kv.end(), by_key);
+#endif
     b->keys.resize(kv.size());
     b->point_list.resize(kv.size());
     for (size_t k = 0; k < kv.size(); ++k) {
@@ -179,6 +193,8 @@ inline void splat_blend(const SplatGeometry<S>& g, const Binning& b, int width, 
 {
     const BlendThresholds th;
     uint64_t evals = 0;
+    // pixels are independent: the thread count does not change any value
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : evals)
     for (int py = y0; py < y1; ++py) {
         for (int px = x0; px < x1; ++px) {
             const int tile = (py / b.tile_h) * b.grid_x + (px / b.tile_w);
@@ -296,6 +312,33 @@ inline void splat_blend_backward(const SplatGeometry<S>& g, const Binning& b, in
     }
 }
 
+// The blend adjoint with image rows split over threads, each summing into its
+// own gradient arrays, reduced at the end.  Same math as splat_blend_backward;
+// the sums differ from it by float reassociation only.  Used for host timing.
+template <class S>
+inline void splat_blend_backward_threads(const SplatGeometry<S>& g, const Binning& b, int width, int height, const S* bg,
+    int x0, int y0, int x1, int y1, const S* fb, const S* T_final, const S* grad_fb, Splat2dGrads<S>* gr, int threads)
+{
+    std::vector<Splat2dGrads<S>> part(threads);
+#pragma omp parallel for num_threads(threads) schedule(static, 1)
+    for (int t = 0; t < threads; ++t) {
+        const int rows = y1 - y0;
+        const int a = y0 + int(int64_t(rows) * t / threads), c = y0 + int(int64_t(rows) * (t + 1) / threads);
+        splat_blend_backward(g, b, width, height, bg, x0, a, x1, c, fb, T_final, grad_fb, &part[t]);
+    }
+    *gr = std::move(part[0]);
+    for (int t = 1; t < threads; ++t) {
+        for (size_t k = 0; k < gr->xy.size(); ++k)
+            gr->xy[k] += part[t].xy[k];
+        for (size_t k = 0; k < gr->conic.size(); ++k)
+            gr->conic[k] += part[t].conic[k];
+        for (size_t k = 0; k < gr->weight.size(); ++k)
+            gr->weight[k] += part[t].weight[k];
+        for (size_t k = 0; k < gr->rgb.size(); ++k)
+            gr->rgb[k] += part[t].rgb[k];
+    }
+}
+
 template <class S>
 struct ParamGrads {
     std::vector<S> means;     // [n,3]
@@ -317,6 +360,7 @@ inline void splat_preprocess_backward(const Scene<S>& sc, const Camera<S>& cam, 
     out->rotations.assign(4 * n, 0);
     out->opacity.assign(n, 0);
     out->sh.assign(48 * n, 0);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         if (g.tiles[i] == 0)
             continue;
