@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame in the host sample (0: the whole frame)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-camera batched workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-marcher", action="store_true", help="skip the volumetric marcher workload")
     ap.add_argument("--batched-cams", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -171,6 +172,7 @@ def main():
     torch.cuda.empty_cache()
     batched = None if args.no_batched else batched_workload(args.n, args.batched_cams)
     served = None if args.no_serving else serving_workload(args.serving_requests)
+    marcher = None if args.no_marcher else marcher_workload(lib, with_cpu=not args.no_cpu)
 
     line = {
         "metric": METRIC,
@@ -196,6 +198,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": served,
+        "marcher": marcher,
     }
     print(json.dumps(line))
 
@@ -253,6 +256,68 @@ def batched_workload(n: int, n_cams: int, reps: int = 3):
             "autotune_ms": table, "batched_ms": round(ms_b, 3), "plain_ms": round(ms_p, 3),
             "batched_mpx_s": round(px / ms_b / 1e3, 1), "plain_mpx_s": round(px / ms_p / 1e3, 1),
             "speedup_vs_plain": round(ms_p / ms_b, 3), "max_image_diff_vs_plain": img_diff, "max_grad_rel_l2_vs_plain": grad_rel}
+
+
+def marcher_workload(lib, with_cpu: bool, steps: int = 3, max_large_steps: int = 2):
+    """Volumetric ray marcher: 200k Gaussians, 1024x1024 rays, at most
+    max_large_steps x 127 bins per ray (256 marching steps), fp32 forward +
+    L1 backward through VolMarcher.apply.  Per-stage durations from the stage
+    profile; the host baseline is the float restatement on a band of rows."""
+    import numpy as np
+    import torch
+
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    from differentiable_gaussian_mixture_renderer_amd.python_bindings import vol_marcher
+
+    scene, cam = scenes.marcher_frame()
+    W, H = cam.width, cam.height
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    params = [t(scene.sh).requires_grad_(), t(scene.opacity).requires_grad_(), t(scene.means).requires_grad_(),
+              t(scene.scales).requires_grad_(), t(scene.rotations).requires_grad_()]
+    target = torch.rand((3, H, W), generator=torch.Generator(device=dev).manual_seed(3), device=dev)
+    cam_args = (torch.tensor(cam.world_view_transform), torch.tensor(cam.full_proj_transform), torch.tensor(cam.camera_center),
+                torch.zeros(3), W, H, cam.tan_fovx, cam.tan_fovy, scene.sh_degree)
+    saved = dict(vol_marcher.DEFAULT_OPTIONS)
+    vol_marcher.DEFAULT_OPTIONS["max_large_steps"] = max_large_steps
+    try:
+        def step():
+            img = vol_marcher.VolMarcher.apply(*params, *cam_args)
+            return torch.autograd.grad((img - target).abs().mean(), params)
+
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        lib.dgmr_profile_enable(1)
+        step()
+        torch.cuda.synchronize()
+        lib.dgmr_profile_enable(0)
+        stages = {k: round(v[0] / max(v[1], 1), 3) for k, v in stage_times(lib).items() if k.startswith("march")}
+    finally:
+        vol_marcher.DEFAULT_OPTIONS.update(saved)
+    out = {"workload": f"volumetric marcher fwd+bwd, {scene.n} Gaussians SH3, {W}x{H} rays, <= {max_large_steps * 127} bins per ray, L1",
+           "ms_per_step": round(ms, 3), "mrays_per_s": round(W * H / ms / 1e3, 2), "stages_ms": stages}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        rows = 16
+        y0 = (H - rows) // 2
+        bg = np.zeros(3, np.float32)
+        grad = np.full((3, H, W), 1.0 / (3 * H * W), np.float32)
+        t0 = time.perf_counter()
+        h = O.MarchOracle(O.scene_arrays(scene), cam, dbl=False)
+        fb, T = h.render(bg, crop=(0, y0, W, y0 + rows), max_large_steps=max_large_steps)[:2]
+        h.backward(bg, fb, T, grad, crop=(0, y0, W, y0 + rows), max_large_steps=max_large_steps)
+        secs = time.perf_counter() - t0
+        threads = os.environ.get("OMP_NUM_THREADS", str(os.cpu_count()))
+        out["cpu_baseline"] = {"value": round(W * rows / secs / 1e6, 5), "unit": "Mrays/s", "kind": "port",
+                               "sample": f"float restatement: preprocess+binning of all Gaussians, forward of {W}x{rows} rays on "
+                                         f"{threads} OpenMP threads, backward of the same rays on 1 thread, {secs:.1f} s"}
+    return out
 
 
 def serving_workload(n_requests: int, rate_hz: float = 200.0):

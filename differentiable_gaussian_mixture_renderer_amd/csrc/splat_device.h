@@ -387,25 +387,24 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
     const int px = px0 + lane % TW;
     const float pxf = float(px);
     const float pyf0 = float(py0 + lane / TW);
-    // S: the framebuffer minus what has been composited so far, i.e. the
-    // remaining colour plus T_final * background, whose dot with dL/dC is the
-    // remaining-colour term plus dL/dT_final * T_final of the alpha adjoint
-    float gC0[NP], gC1[NP], gC2[NP], S0[NP], S1[NP], S2[NP], T[NP];
+    // D = <dL/dC, S> with S the framebuffer minus what has been composited so
+    // far, i.e. the remaining colour plus T_final * background: D carries the
+    // remaining-colour term and dL/dT_final * T_final of the alpha adjoint in
+    // one scalar per pixel
+    float gC0[NP], gC1[NP], gC2[NP], D[NP], T[NP];
     bool live[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int py = py0 + lane / TW + RS * k;
         gC0[k] = gC1[k] = gC2[k] = 0.f;
-        S0[k] = S1[k] = S2[k] = 0.f;
+        D[k] = 0.f;
         const bool inside = px < width && py < height;
         if (inside) {
             const int64_t p = int64_t(py) * width + px;
             gC0[k] = grad_fb[p];
             gC1[k] = grad_fb[plane + p];
             gC2[k] = grad_fb[2 * plane + p];
-            S0[k] = framebuffer[p];
-            S1[k] = framebuffer[plane + p];
-            S2[k] = framebuffer[2 * plane + p];
+            D[k] = gC0[k] * framebuffer[p] + gC1[k] * framebuffer[plane + p] + gC2[k] * framebuffer[2 * plane + p];
         }
         T[k] = 1.f;
         live[k] = inside && !(gC0[k] == 0.f && gC1[k] == 0.f && gC2[k] == 0.f);
@@ -491,16 +490,14 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                     const float tk = 1.f - alphak[k];
                     const float test_T = T[k] * tk;
                     const float wgt = alphak[k] * T[k];
-                    S0[k] = use[k] ? fmaf(-c.z, wgt, S0[k]) : S0[k];
-                    S1[k] = use[k] ? fmaf(-c.w, wgt, S1[k]) : S1[k];
-                    S2[k] = use[k] ? fmaf(-blue, wgt, S2[k]) : S2[k];
+                    const float grgb = gC0[k] * c.z + gC1[k] * c.w + gC2[k] * blue; // <dL/dC, rgb>
+                    D[k] = use[k] ? fmaf(-grgb, wgt, D[k]) : D[k];
                     sr = use[k] ? fmaf(gC0[k], wgt, sr) : sr;
                     sg = use[k] ? fmaf(gC1[k], wgt, sg) : sg;
                     sbl = use[k] ? fmaf(gC2[k], wgt, sbl) : sbl;
                     // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
                     const float inv_tk = __builtin_amdgcn_rcpf(tk);
-                    const float g_alpha = (gC0[k] * c.z + gC1[k] * c.w + gC2[k] * blue) * T[k]
-                        - (gC0[k] * S0[k] + gC1[k] * S1[k] + gC2[k] * S2[k]) * inv_tk;
+                    const float g_alpha = grgb * T[k] - D[k] * inv_tk;
                     const float g_raw = (use[k] && raw < 0.99f) ? g_alpha : 0.f;
                     so = fmaf(Gk[k], g_raw, so);
                     const float gp = raw * g_raw; // dL/dpower
