@@ -157,8 +157,9 @@ __device__ __forceinline__ void splat_ots_grad(S w, const S* mu, const S* s, con
     const S cov_f[3] = { in.cov2[0] + filter, in.cov2[1], in.cov2[2] + filter };
     const S detJ = det3(in.J);
     int i0, i1;
-    larger2_idx(s, &i0, &i1);
-    const S num = S(2 * kPi) * s[i0] * s[i1];
+    S sa, sb;
+    larger2_values(s, &sa, &sb, &i0, &i1);
+    const S num = S(2 * kPi) * sa * sb;
     const S det2 = cov_f[0] * cov_f[2] - cov_f[1] * cov_f[1];
     const S sq = sqrt(det2);
     const S i2 = S(2 * kPi) * sq;
@@ -168,9 +169,8 @@ __device__ __forceinline__ void splat_ots_grad(S w, const S* mu, const S* s, con
     const S g_i2 = -g_weight * w * detJ * num / (i2 * i2);
     const S f = g_i2 * S(2 * kPi) / (S(2) * sq);
     S g_c2[3] = { g_cov2[0] + f * cov_f[2], g_cov2[1] + f * (S(-2) * cov_f[1]), g_cov2[2] + f * cov_f[0] };
-    S gs[3] = { 0, 0, 0 };
-    gs[i0] = g_num * S(2 * kPi) * s[i1];
-    gs[i1] = g_num * S(2 * kPi) * s[i0];
+    const S g_a = g_num * S(2 * kPi) * sb, g_b = g_num * S(2 * kPi) * sa;
+    const S gs[3] = { i0 == 0 ? g_a : S(0), i0 == 1 ? g_a : (i1 == 1 ? g_b : S(0)), i1 == 2 ? g_b : S(0) };
     S g_J[3][3];
     det3_grad(in.J, g_det, g_J);
     S g_cov3[6], g_T[3][3];

@@ -248,6 +248,16 @@ __device__ __forceinline__ void larger2_idx(const S* v, int* i0, int* i1)
     }
 }
 
+// v[i0], v[i1] of larger2_idx without indexing by a runtime value (which would
+// place v in scratch memory); (i0, i1) is one of (0,1), (1,2), (0,2)
+template <class S>
+__device__ __forceinline__ void larger2_values(const S* v, S* a, S* b, int* i0, int* i1)
+{
+    larger2_idx(v, i0, i1);
+    *a = *i0 == 0 ? v[0] : v[1];
+    *b = *i1 == 1 ? v[1] : v[2];
+}
+
 // SH colour (math.h:233-271) of one channel from basis values
 template <class S>
 __device__ __forceinline__ void sh_basis(int deg, const S* dir, S* Y)

@@ -266,6 +266,8 @@ __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64
     const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations,
     float* __restrict__ g_opacity, float* __restrict__ g_sh)
 {
+    // SH rows in and SH gradients out go through LDS (coalesced); a row is
+    // overwritten in place by its gradient
     __shared__ float s_sh[kPreThreads * kShStride];
     const int64_t first = int64_t(blockIdx.x) * blockDim.x;
     const int count = int(min<int64_t>(blockDim.x, n - first));
@@ -273,14 +275,14 @@ __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64
     __syncthreads();
     const int64_t i = first + threadIdx.x;
     float g_mu[3] = { 0.f, 0.f, 0.f }, g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f }, g_w = 0.f;
-    float g_shv[48];
-#pragma unroll
-    for (int k = 0; k < 48; ++k)
-        g_shv[k] = 0.f;
     float* row = s_sh + threadIdx.x * kShStride;
-    if (i < n && tiles[i] != 0)
-        preprocess_backward_gaussian(i, p, row, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9, g_mu, g_s, g_q, &g_w, g_shv);
     if (i < n) {
+        if (tiles[i] != 0)
+            preprocess_backward_gaussian<true>(i, p, row, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9, g_mu, g_s, g_q, &g_w,
+                nullptr);
+        else
+            for (int k = 0; k < 3 * p.sh_coeffs; ++k)
+                row[k] = 0.f;
         activation_backward(p, i, g_s, g_q, &g_w);
         for (int a = 0; a < 3; ++a) {
             g_means[3 * i + a] = g_mu[a];
@@ -290,11 +292,6 @@ __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64
             g_rotations[4 * i + a] = g_q[a];
         g_opacity[i] = g_w;
     }
-    __syncthreads(); // every row has been read; reuse the rows for the SH gradient
-#pragma unroll
-    for (int k = 0; k < 48; ++k)
-        if (k < p.sh_coeffs * 3)
-            row[k] = g_shv[k];
     __syncthreads();
     unstage_sh(g_sh, s_sh, first, count, p.sh_coeffs);
 }

@@ -173,8 +173,9 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     out.clamped[3 * o + 2] = r2 >= 0.f;
     // screen-space weight, Ots formulation (math.h:389-397)
     int i0, i1;
-    larger2_idx(s, &i0, &i1);
-    const float i2prime = float(2 * kPi) * s[i0] * s[i1];
+    float sa, sb;
+    larger2_values(s, &sa, &sb, &i0, &i1);
+    const float i2prime = float(2 * kPi) * sa * sb;
     const float det2 = cov_f[0] * cov_f[2] - cov_f[1] * cov_f[1];
     const float i2 = float(2 * kPi) * sqrt_rn(det2);
     const float weight = w * det3(in.J) * i2prime / i2;
@@ -532,7 +533,11 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
 // grad::sh_to_colour (grad/math.h:203-282) and normalize.  Adds into the
 // accumulators (g_sh has 48 slots; only k < sh_coeffs are meaningful).
 // c: this Gaussian's SH row.
-__device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const SplatParams& p, const float* c, const DevCamera<float>& cam,
+// IN_PLACE: c is the Gaussian's SH row in LDS and receives its SH gradient
+// (sh_coeffs x 3 values, read before written); otherwise g_sh_acc (48
+// accumulators) is added to.
+template <bool IN_PLACE = false>
+__device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const SplatParams& p, float* c, const DevCamera<float>& cam,
     const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc, float* g_q_acc, float* g_w_acc, float* g_sh_acc)
 {
     const float mu[3] = { p.means[3 * i], p.means[3 * i + 1], p.means[3 * i + 2] };
@@ -561,14 +566,24 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
     for (int k = 0; k < 16; ++k) {
         if (k < kmax) {
             const float c0 = c[3 * k], c1 = c[3 * k + 1], c2 = c[3 * k + 2];
-            g_sh_acc[3 * k + 0] += Y[k] * gr[0];
-            g_sh_acc[3 * k + 1] += Y[k] * gr[1];
-            g_sh_acc[3 * k + 2] += Y[k] * gr[2];
+            if constexpr (IN_PLACE) {
+                c[3 * k + 0] = Y[k] * gr[0];
+                c[3 * k + 1] = Y[k] * gr[1];
+                c[3 * k + 2] = Y[k] * gr[2];
+            } else {
+                g_sh_acc[3 * k + 0] += Y[k] * gr[0];
+                g_sh_acc[3 * k + 1] += Y[k] * gr[1];
+                g_sh_acc[3 * k + 2] += Y[k] * gr[2];
+            }
             const float dot_c = c0 * gr[0] + c1 * gr[1] + c2 * gr[2];
             g_dir[0] = fmaf(dY[k][0], dot_c, g_dir[0]);
             g_dir[1] = fmaf(dY[k][1], dot_c, g_dir[1]);
             g_dir[2] = fmaf(dY[k][2], dot_c, g_dir[2]);
         }
+    }
+    if constexpr (IN_PLACE) {
+        for (int k = kmax; k < p.sh_coeffs; ++k)
+            c[3 * k] = c[3 * k + 1] = c[3 * k + 2] = 0.f; // beyond the active degree
     }
     const float dd = dot3(g_dir, dir);
 #pragma unroll
