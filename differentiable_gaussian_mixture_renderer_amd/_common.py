@@ -11,6 +11,7 @@ from . import _native as N
 
 _state_lock = threading.Lock()
 _capacity = {}   # (device index, kind) -> pair capacity that fitted last time
+_ppp = {}        # device index -> largest (tile, Gaussian) pairs per pixel seen
 
 
 def stream_handle(device: torch.device) -> int:
@@ -90,6 +91,16 @@ def remember_capacity(device: torch.device, kind: str, cap: int) -> None:
     with _state_lock:
         key = (device.index, kind)
         _capacity[key] = max(_capacity.get(key, 0), int(cap))
+
+
+def pairs_per_pixel(device: torch.device) -> float:
+    with _state_lock:
+        return _ppp.get(device.index, 16.0)
+
+
+def remember_pairs_per_pixel(device: torch.device, ppp: float) -> None:
+    with _state_lock:
+        _ppp[device.index] = max(_ppp.get(device.index, 0.0), float(ppp))
 
 
 def workspace(device: torch.device, n: int, width: int, height: int, tiling: N.Tiling, cap: int) -> torch.Tensor:

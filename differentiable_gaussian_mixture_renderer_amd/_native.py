@@ -136,8 +136,8 @@ SIGNATURES = {
     "dgmr_splat_batch_forward": (ctypes.c_int, [_P(Gaussians), _P(BatchView), ctypes.c_int32, _P(Tiling), c_void_p, ctypes.c_size_t,
                                                 ctypes.c_int64, _P(Stats), c_void_p]),
     "dgmr_splat_batch_backward": (ctypes.c_int, [_P(Gaussians), _P(BatchView), ctypes.c_int32, _P(Tiling), c_void_p, ctypes.c_size_t,
-                                                 ctypes.c_int64, ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                                 c_void_p]),
+                                                 ctypes.c_int64, ctypes.c_int64, _P(ctypes.c_int32), ctypes.c_int32, c_void_p, c_void_p,
+                                                 c_void_p, c_void_p, c_void_p, c_void_p]),
     "dgmr_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "dgmr_profile_read": (ctypes.c_int, [_P(StageTime), ctypes.c_int32, _P(ctypes.c_int32)]),
     "dgmr_last_error": (ctypes.c_char_p, []),

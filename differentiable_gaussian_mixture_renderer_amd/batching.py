@@ -13,7 +13,7 @@ that the batched path is measured against.
 from __future__ import annotations
 
 import ctypes
-from typing import List, Sequence
+from typing import List, Optional, Sequence
 
 import torch
 
@@ -59,7 +59,9 @@ class BatchSplatter:
         fbs = [torch.empty((3, c.height, c.width), dtype=torch.float32, device=dev) for c in self.cameras]
         Ts = [torch.empty((c.height, c.width), dtype=torch.float32, device=dev) for c in self.cameras]
         views = self._view_array(self.cameras, self.bg, fbs, Ts)
-        cap = self.capacity or C.initial_capacity(dev, "batch", 4 * n * len(self.cameras))
+        # pair capacity from the largest pairs-per-pixel density seen so far
+        px = sum(c.width * c.height for c in self.cameras)
+        cap = max(int(C.pairs_per_pixel(dev) * px * 1.25) + 65536, 1 << 16)
         stats = N.Stats()
         stream = C.stream_handle(dev)
         with torch.cuda.device(dev):
@@ -80,8 +82,8 @@ class BatchSplatter:
             else:
                 raise N.DgmrError(N.DGMR_ERR_CAPACITY, "dgmr_splat_batch_forward", "pair capacity kept growing")
         self.capacity = cap
-        C.remember_capacity(dev, "batch", cap)
         self._n_pairs = int(stats.n_pairs)
+        C.remember_pairs_per_pixel(dev, self._n_pairs / max(px, 1))
         self._fbs, self._Ts = fbs, Ts
         return fbs, Ts
 
@@ -89,25 +91,40 @@ class BatchSplatter:
     def n_pairs(self) -> int:
         return self._n_pairs
 
-    def backward(self, grad_images: List[torch.Tensor]):
+    def backward(self, grad_images: List[Optional[torch.Tensor]], groups: Optional[Sequence[int]] = None):
+        """Gradients of the scene parameters.  groups=None: summed over all
+        views, returned as (g_sh, g_opacity, g_means, g_scales, g_rotations).
+        Otherwise groups[v] in {-1, 0..G-1} (a group's views consecutive, -1 =
+        no gradient, its grad image may be None) and a list of G such tuples is
+        returned, one per group."""
         shs, opacity, means, scales, rotations = self._keep
         dev = means.device
         n = means.shape[0]
-        grads = [g.to(torch.float32).contiguous() for g in grad_images]
+        G = 1 if groups is None else max(1, max(groups) + 1)
+        zero = None
+        grads = []
+        for v, g in enumerate(grad_images):
+            if g is None:
+                if zero is None or zero.shape != self._fbs[v].shape:
+                    zero = torch.zeros_like(self._fbs[v])
+                g = zero
+            grads.append(g.to(torch.float32).contiguous())
         views = self._view_array(self.cameras, self.bg, self._fbs, self._Ts, grads)
         f32 = dict(dtype=torch.float32, device=dev)
-        g_means = torch.empty((n, 3), **f32)
-        g_scales = torch.empty((n, 3), **f32)
-        g_rot = torch.empty((n, 4), **f32)
-        g_op = torch.empty(opacity.shape, **f32)
-        g_sh = torch.empty(shs.shape, **f32)
+        g_means = torch.empty((G, n, 3), **f32)
+        g_scales = torch.empty((G, n, 3), **f32)
+        g_rot = torch.empty((G, n, 4), **f32)
+        g_op = torch.empty((G, *opacity.shape), **f32)
+        g_sh = torch.empty((G, *shs.shape), **f32)
+        grp = None if groups is None else (ctypes.c_int32 * len(groups))(*[int(x) for x in groups])
         with torch.cuda.device(dev):
             rc = N.load().dgmr_splat_batch_backward(ctypes.byref(self._g), views, len(self.cameras), ctypes.byref(self.tiling),
-                                                    self._ws.data_ptr(), self._ws.numel(), self.capacity, self._n_pairs,
+                                                    self._ws.data_ptr(), self._ws.numel(), self.capacity, self._n_pairs, grp, G,
                                                     g_means.data_ptr(), g_scales.data_ptr(), g_rot.data_ptr(), g_op.data_ptr(),
                                                     g_sh.data_ptr(), C.stream_handle(dev))
             N.check(rc, "dgmr_splat_batch_backward")
-        return g_sh, g_op, g_means, g_scales, g_rot
+        out = [(g_sh[k], g_op[k], g_means[k], g_scales[k], g_rot[k]) for k in range(G)]
+        return out[0] if groups is None else out
 
 
 def render_plain(shs, opacity, means, scales, rotations, cameras: Sequence, bg=(0.0, 0.0, 0.0), sh_degree: int = 3,

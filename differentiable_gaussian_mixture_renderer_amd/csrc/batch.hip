@@ -86,11 +86,13 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t
     }
 }
 
-// thread per Gaussian, summing its adjoint over the views that see it
+// thread per Gaussian, summing its adjoint over the views of each gradient
+// group: group[v] is -1 (no gradient) or a group id whose views are
+// consecutive; group g's sums go to block g of the outputs ([n_groups][n][..]).
 __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, int n_views, SplatParams p, const DevCamera<float>* __restrict__ cams,
-    const uint32_t* __restrict__ tiles, const float4* __restrict__ conic_opacity, const uint8_t* __restrict__ clamped,
-    const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations,
-    float* __restrict__ g_opacity, float* __restrict__ g_sh)
+    const int32_t* __restrict__ group, const uint32_t* __restrict__ tiles, const float4* __restrict__ conic_opacity,
+    const uint8_t* __restrict__ clamped, const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales,
+    float* __restrict__ g_rotations, float* __restrict__ g_opacity, float* __restrict__ g_sh)
 {
     __shared__ float s_sh[256 * kShStride];
     const int64_t first = int64_t(blockIdx.x) * blockDim.x;
@@ -98,41 +100,55 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
     stage_sh(s_sh, p.sh, first, count, p.sh_coeffs);
     __syncthreads();
     const int64_t i = first + threadIdx.x;
-    float g_mu[3] = { 0.f, 0.f, 0.f }, g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f }, g_w = 0.f;
-    float g_shv[48];
+    if (i >= n)
+        return;
+    const float* row = s_sh + threadIdx.x * kShStride;
+    const int kc = 3 * p.sh_coeffs;
+    float g_mu[3], g_s[3], g_q[4], g_w, g_shv[48];
+    int cur = -1;
+    for (int v = 0; v <= n_views; ++v) {
+        const int gv = v < n_views ? group[v] : -1;
+        if (gv != cur) {
+            if (cur >= 0) {
+                // views of group cur done: write its block
+                activation_backward(p, i, g_s, g_q, &g_w);
+                const int64_t o = int64_t(cur) * n + i;
+                for (int a = 0; a < 3; ++a) {
+                    g_means[3 * o + a] = g_mu[a];
+                    g_scales[3 * o + a] = g_s[a];
+                }
+                for (int a = 0; a < 4; ++a)
+                    g_rotations[4 * o + a] = g_q[a];
+                g_opacity[o] = g_w;
 #pragma unroll
-    for (int k = 0; k < 48; ++k)
-        g_shv[k] = 0.f;
-    float* row = s_sh + threadIdx.x * kShStride;
-    if (i < n) {
-        for (int v = 0; v < n_views; ++v) {
-            const int64_t o = int64_t(v) * n + i;
-            if (tiles[o] != 0)
-                preprocess_backward_gaussian(i, p, row, cams[v], conic_opacity[o], clamped + 3 * o, screen_grads + o * 9, g_mu, g_s, g_q, &g_w,
-                    g_shv);
+                for (int k = 0; k < 48; ++k)
+                    if (k < kc)
+                        g_sh[o * kc + k] = g_shv[k];
+            }
+            cur = gv;
+            for (int a = 0; a < 3; ++a)
+                g_mu[a] = g_s[a] = 0.f;
+            for (int a = 0; a < 4; ++a)
+                g_q[a] = 0.f;
+            g_w = 0.f;
+#pragma unroll
+            for (int k = 0; k < 48; ++k)
+                g_shv[k] = 0.f;
         }
-        activation_backward(p, i, g_s, g_q, &g_w);
-        for (int a = 0; a < 3; ++a) {
-            g_means[3 * i + a] = g_mu[a];
-            g_scales[3 * i + a] = g_s[a];
-        }
-        for (int a = 0; a < 4; ++a)
-            g_rotations[4 * i + a] = g_q[a];
-        g_opacity[i] = g_w;
+        if (gv < 0)
+            continue;
+        const int64_t o = int64_t(v) * n + i;
+        if (tiles[o] != 0)
+            preprocess_backward_gaussian(i, p, const_cast<float*>(row), cams[v], conic_opacity[o], clamped + 3 * o, screen_grads + o * 9, g_mu,
+                g_s, g_q, &g_w, g_shv);
     }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 48; ++k)
-        if (k < p.sh_coeffs * 3)
-            row[k] = g_shv[k];
-    __syncthreads();
-    unstage_sh(g_sh, s_sh, first, count, p.sh_coeffs);
 }
 
 // Workspace layout shared by forward and backward (the forward's geometry and
 // binning must survive until the backward).
 struct BatchLayout {
     DevCamera<float>* cams;
+    int32_t* groups;
     ViewBlend* views;
     int4* grids;
     uint32_t* queue;
@@ -161,6 +177,7 @@ BatchLayout carve_batch(void* ws, int64_t n, const dgmr_batch_view* views, int n
     const int64_t m = std::max<int64_t>(1, n * n_views);
     L.total_tiles = total_tiles;
     L.cams = cv.take<DevCamera<float>>(n_views);
+    L.groups = cv.take<int32_t>(n_views);
     L.views = cv.take<ViewBlend>(n_views);
     L.grids = cv.take<int4>(n_views);
     L.queue = cv.take<uint32_t>(64);
@@ -303,15 +320,35 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
 }
 
 int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling, void* workspace,
-    size_t workspace_bytes, int64_t pair_capacity, int64_t n_pairs, float* grad_means, float* grad_scales, float* grad_rotations,
-    float* grad_opacity, float* grad_sh, dgmr_stream stream)
+    size_t workspace_bytes, int64_t pair_capacity, int64_t n_pairs, const int32_t* view_group, int32_t n_groups, float* grad_means,
+    float* grad_scales, float* grad_rotations, float* grad_opacity, float* grad_sh, dgmr_stream stream)
 {
     const int rc0 = check_batch_args(g, views, n_views, tiling);
     if (rc0 != DGMR_OK)
         return rc0;
     DGMR_ARG_CHECK(workspace && grad_means && grad_scales && grad_rotations && grad_opacity && grad_sh, "dgmr_splat_batch_backward: null argument");
+    // gradient groups: ids in [-1, n_groups), each group's views consecutive
+    std::vector<int32_t> grp(n_views, 0);
+    bool every_group_has_views = true;
+    if (view_group) {
+        DGMR_ARG_CHECK(n_groups >= 1, "dgmr_splat_batch_backward: n_groups must be >= 1 with view_group");
+        std::vector<char> seen(n_groups, 0);
+        for (int v = 0; v < n_views; ++v) {
+            const int32_t gv = view_group[v];
+            DGMR_ARG_CHECK(gv >= -1 && gv < n_groups, "dgmr_splat_batch_backward: view_group out of range");
+            if (gv >= 0 && (v == 0 || view_group[v - 1] != gv)) {
+                DGMR_ARG_CHECK(!seen[gv], "dgmr_splat_batch_backward: a group's views must be consecutive");
+                seen[gv] = 1;
+            }
+            grp[v] = gv;
+        }
+        for (int gi = 0; gi < n_groups; ++gi)
+            every_group_has_views = every_group_has_views && seen[gi];
+    } else {
+        n_groups = 1;
+    }
     for (int v = 0; v < n_views; ++v)
-        DGMR_ARG_CHECK(views[v].grad_framebuffer, "dgmr_splat_batch_backward: every view needs grad_framebuffer");
+        DGMR_ARG_CHECK(grp[v] < 0 || views[v].grad_framebuffer, "dgmr_splat_batch_backward: a view with a gradient group needs grad_framebuffer");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int64_t n = g->n;
     size_t need = 0;
@@ -330,6 +367,7 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
         base += int64_t(c.grid_x) * c.grid_y;
     }
     DGMR_HIP_CHECK(hipMemcpyAsync(L.views, vb.data(), sizeof(ViewBlend) * n_views, hipMemcpyHostToDevice, st));
+    DGMR_HIP_CHECK(hipMemcpyAsync(L.groups, grp.data(), sizeof(int32_t) * n_views, hipMemcpyHostToDevice, st));
     const int64_t m = n * n_views;
     DGMR_HIP_CHECK(hipMemsetAsync(L.screen_grads, 0, size_t(m) * 9 * sizeof(float), st));
     if (n_pairs > 0) {
@@ -341,8 +379,17 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
     StageScope sc("batch.preprocess_backward", st, double(n) * 2.0 * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 40.0);
     const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
         static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), g->activation };
-    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams, L.tiles, L.conic_opacity, L.clamped,
-        L.screen_grads, grad_means, grad_scales, grad_rotations, grad_opacity, grad_sh);
+    if (!every_group_has_views) {
+        // a group with no view is written by no thread: clear the outputs first
+        const size_t per = size_t(n) * n_groups;
+        DGMR_HIP_CHECK(hipMemsetAsync(grad_means, 0, per * 3 * sizeof(float), st));
+        DGMR_HIP_CHECK(hipMemsetAsync(grad_scales, 0, per * 3 * sizeof(float), st));
+        DGMR_HIP_CHECK(hipMemsetAsync(grad_rotations, 0, per * 4 * sizeof(float), st));
+        DGMR_HIP_CHECK(hipMemsetAsync(grad_opacity, 0, per * sizeof(float), st));
+        DGMR_HIP_CHECK(hipMemsetAsync(grad_sh, 0, per * 3 * g->sh_coeffs * sizeof(float), st));
+    }
+    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams, L.groups, L.tiles, L.conic_opacity,
+        L.clamped, L.screen_grads, grad_means, grad_scales, grad_rotations, grad_opacity, grad_sh);
     DGMR_HIP_CHECK(hipGetLastError());
     return DGMR_OK;
 }

@@ -161,21 +161,24 @@ class ContinuousBatchingServer:
                 if not self._work and self._stop:
                     return
                 scene_id, batch, cost = self._work.popleft()
+            # gradient requests first, one gradient group each; forward-only views after them
+            batch = [r for r in batch if r.backward] + [r for r in batch if not r.backward]
+            n_bwd = sum(r.backward for r in batch)
             with torch.cuda.stream(stream):
                 sc = self.resident[scene_id]
                 imgs, _ = splatter.forward(*sc.params, [r.camera for r in batch], sh_degree=sc.sh_degree)
-                grads = None
-                if any(r.backward for r in batch):
-                    # at most one backward request per batch: the summed gradients are its own
-                    gimg = [l1_grad(img, r.target_seed) if r.backward else torch.zeros_like(img) for img, r in zip(imgs, batch)]
-                    grads = splatter.backward(gimg)
+                grads = [None] * len(batch)
+                if n_bwd:
+                    gimg = [l1_grad(img, r.target_seed) if r.backward else None for img, r in zip(imgs, batch)]
+                    groups = [k if k < n_bwd else -1 for k in range(len(batch))]
+                    grads[:n_bwd] = splatter.backward(gimg, groups=groups)
                 stream.synchronize()
             t = self.now()
             with self._lock:
-                for r, img in zip(batch, imgs):
+                for r, img, g in zip(batch, imgs, grads):
                     r.done_at = t
                     r.image = img
-                    r.grads = grads if r.backward else None
+                    r.grads = g
                     self.stats.latencies.append(t - r.arrival)
                     self.stats.pixels += r.pixels
                 self.stats.batches += 1
@@ -185,21 +188,17 @@ class ContinuousBatchingServer:
     # -------------------------------------------------------------- scheduler
     def _schedule(self):
         """Admits the waiting requests of the scene with the oldest waiting
-        request, as many as fit the budget and the batch limit.  A batch holds
-        at most one forward+backward request: the batched backward sums the
-        gradients of its views, and forward-only views contribute zero."""
+        request, as many as fit the budget and the batch limit."""
         if not self._waiting:
             return False
         scene_id = self._waiting[0].scene
         picked, rest = [], deque()
-        has_backward = False
         while self._waiting:
             r = self._waiting.popleft()
-            if r.scene == scene_id and len(picked) < self.max_views and not (r.backward and has_backward):
+            if r.scene == scene_id and len(picked) < self.max_views:
                 cost = estimate_bytes(self.resident[scene_id], picked + [r])
                 if self._in_flight + cost <= self.budget or (not picked and self._in_flight == 0):
                     picked.append(r)
-                    has_backward |= r.backward
                     continue
             rest.append(r)
         self._waiting = rest

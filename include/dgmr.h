@@ -184,9 +184,14 @@ int dgmr_batch_workspace_size(int64_t n, const dgmr_batch_view* views, int32_t n
 int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,
     void* workspace, size_t workspace_bytes, int64_t pair_capacity, dgmr_stats* stats, dgmr_stream stream);
 
-/* Gradients of all views are summed into grad_* (overwritten). */
+/* Gradient groups: view_group NULL sums the gradients of all views into
+ * grad_* (one block, n_groups ignored).  Otherwise view_group (host, n_views
+ * entries) gives each view a group in [0, n_groups) or -1 (no gradient); a
+ * group's views are consecutive, and grad_* hold n_groups consecutive blocks
+ * ([n_groups][n,3] means, ...), each the sum over its group's views (zero for
+ * a group without views).  Outputs are overwritten. */
 int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,
-    void* workspace, size_t workspace_bytes, int64_t pair_capacity, int64_t n_pairs,
+    void* workspace, size_t workspace_bytes, int64_t pair_capacity, int64_t n_pairs, const int32_t* view_group, int32_t n_groups,
     float* grad_means, float* grad_scales, float* grad_rotations, float* grad_opacity, float* grad_sh, dgmr_stream stream);
 
 /* ------------------------------------------------------------ stage timing

@@ -50,3 +50,29 @@ def test_batch_matches_single_views(tile, batch):
     for a, b in zip(gb, gp):
         a, b = a.cpu().numpy(), b.cpu().numpy()
         assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b) + 1e-12
+
+
+def test_gradient_groups_match_per_group_plain_sums():
+    """view groups [0, 0, -1, 1, 1, 1]: group k's gradients equal the plain
+    path's gradients summed over its views (||diff|| <= 1e-4 ||ref||); the
+    view without a group contributes to none."""
+    torch = _torch()
+    from differentiable_gaussian_mixture_renderer_amd.batching import BatchSplatter, render_plain
+    scene, cams = scene_and_cams()
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    params = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations)]
+    print(f"[gpu batch] gradient groups over {len(cams)} views", flush=True)
+    bs = BatchSplatter()
+    fbs, _ = bs.forward(*params, cams)
+    rng = np.random.default_rng(2)
+    grads = [torch.tensor(rng.normal(size=(3, c.height, c.width)).astype(np.float32) / (3 * c.height * c.width), device=dev) for c in cams]
+    groups = [0, 0, -1, 1, 1, 1]
+    out = bs.backward([g if k >= 0 else None for g, k in zip(grads, groups)], groups=groups)
+    assert len(out) == 2
+    for k, members in ((0, [0, 1]), (1, [3, 4, 5])):
+        gi = iter([grads[v] for v in members])
+        _, ref = render_plain(*params, [cams[v] for v in members], grad_fn=lambda img: next(gi))
+        for a, b in zip(out[k], ref):
+            a, b = a.cpu().numpy(), b.cpu().numpy()
+            assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b) + 1e-12, k
