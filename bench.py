@@ -237,6 +237,14 @@ def batched_workload(n: int, n_cams: int, reps: int = 3):
         fbs, gb = batched_step()
     torch.cuda.synchronize()
     ms_b = (time.perf_counter() - t0) / reps * 1e3
+    from differentiable_gaussian_mixture_renderer_amd import _native as N
+    lib = N.load()
+    lib.dgmr_profile_read((N.StageTime * 64)(), 64, ctypes.byref(ctypes.c_int32(0)))  # drop older records
+    lib.dgmr_profile_enable(1)
+    batched_step()
+    torch.cuda.synchronize()
+    lib.dgmr_profile_enable(0)
+    stages = {k: round(v[0] / max(v[1], 1), 3) for k, v in stage_times(lib).items()}
 
     def plain_step():
         it = iter(range(len(cams)))
@@ -253,7 +261,7 @@ def batched_workload(n: int, n_cams: int, reps: int = 3):
     grad_rel = max(float((a - b).norm() / b.norm()) for a, b in zip(gb, gp))
     return {"workload": f"{n_cams} cameras of one {n}-Gaussian SH3 scene, resolutions 256^2..1920x1080, fwd+L1 bwd",
             "mpixels_per_batch": round(px / 1e6, 3), "pairs": bs.n_pairs, "tile": f"{tile[0]}x{tile[1]}", "lds_batch": lds,
-            "autotune_ms": table, "batched_ms": round(ms_b, 3), "plain_ms": round(ms_p, 3),
+            "autotune_ms": table, "batched_ms": round(ms_b, 3), "batched_stages_ms": stages, "plain_ms": round(ms_p, 3),
             "batched_mpx_s": round(px / ms_b / 1e3, 1), "plain_mpx_s": round(px / ms_p / 1e3, 1),
             "speedup_vs_plain": round(ms_p / ms_b, 3), "max_image_diff_vs_plain": img_diff, "max_grad_rel_l2_vs_plain": grad_rel}
 

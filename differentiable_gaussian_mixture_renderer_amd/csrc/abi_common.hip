@@ -145,6 +145,49 @@ StageScope::StageScope(const char* name, hipStream_t s, double bytes) : stream(s
     slot = int(g_prof.records.size()) - 1;
 }
 
+void stage_set_bytes(int slot, double bytes)
+{
+    if (slot < 0)
+        return;
+    std::lock_guard<std::mutex> lk(g_prof_mutex);
+    if (slot < int(g_prof.records.size()))
+        g_prof.records[slot].bytes = bytes;
+}
+
+int enqueue_count(const uint32_t* dev_count, hipStream_t st, PendingCount* pc)
+{
+    // one pinned word and one event per thread, kept for the life of the process
+    thread_local uint32_t* host = nullptr;
+    thread_local hipEvent_t done = nullptr;
+    if (!host) {
+        void* p = nullptr;
+        DGMR_HIP_CHECK(hipHostMalloc(&p, 256, hipHostMallocDefault));
+        host = static_cast<uint32_t*>(p);
+    }
+    if (!done)
+        DGMR_HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    DGMR_HIP_CHECK(hipMemcpyAsync(host, dev_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    DGMR_HIP_CHECK(hipEventRecord(done, st));
+    pc->device = dev_count;
+    pc->host = host;
+    pc->done = done;
+    return DGMR_OK;
+}
+
+int64_t PendingCount::wait()
+{
+    if (!done || !host)
+        return 0;
+    if (hipEventSynchronize(done) != hipSuccess) {
+        set_error("waiting for the pair count failed");
+        return -1;
+    }
+    const int64_t r = int64_t(*host);
+    for (int k = 0; k < n_slots; ++k)
+        stage_set_bytes(slot[k], per_pair[k] * double(r) + fixed[k]);
+    return r;
+}
+
 StageScope::~StageScope()
 {
     if (slot < 0)

@@ -302,20 +302,30 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     dgmr_binning bins { nullptr, L.point_list, L.ranges, pair_capacity };
     Carver cv(L.binning, L.binning_bytes);
     const BatchBinning bb { L.grids, std::max<int64_t>(n, 1), n_views, L.total_tiles };
-    int64_t R = 0;
+    PendingCount pending;
     if (m > 0) {
-        const int rc = bin_gaussians(m, L.rects, L.tiles, L.xy, false, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &R, st, &bb);
-        if (stats)
-            stats->n_pairs = R;
+        const int rc = bin_gaussians(m, L.rects, L.tiles, L.xy, false, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &pending, st, &bb);
         if (rc != DGMR_OK)
             return rc;
     } else {
         DGMR_HIP_CHECK(hipMemsetAsync(L.ranges, 0, size_t(L.total_tiles) * 2 * sizeof(uint32_t), st));
     }
-    StageScope sc("batch.blend_forward", st, 40.0 * double(R));
-    DGMR_HIP_CHECK(hipMemsetAsync(L.queue, 0, sizeof(uint32_t), st));
-    batch_blend(false, *tiling, st, L, n_views);
-    DGMR_HIP_CHECK(hipGetLastError());
+    {
+        StageScope sc("batch.blend_forward", st);
+        pending.add_stage(sc.slot, 40.0, 0.0);
+        DGMR_HIP_CHECK(hipMemsetAsync(L.queue, 0, sizeof(uint32_t), st));
+        batch_blend(false, *tiling, st, L, n_views);
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
+    const int64_t R = pending.wait();
+    if (R < 0)
+        return DGMR_ERR_DEVICE;
+    if (stats)
+        stats->n_pairs = R;
+    if (R > pair_capacity) {
+        set_error("pair capacity too small");
+        return DGMR_ERR_CAPACITY;
+    }
     return DGMR_OK;
 }
 

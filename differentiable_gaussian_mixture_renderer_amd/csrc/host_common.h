@@ -74,6 +74,37 @@ struct StageScope {
     int slot = -1;
     hipStream_t stream;
 };
+// sets the byte model of a recorded stage once it is known (no-op for slot < 0)
+void stage_set_bytes(int slot, double bytes);
+
+// The pair count R read back without stalling the stream: it is copied to
+// pinned host memory right after the scan that produces it, the kernels that
+// depend on it read it on the device (grids sized by the pair capacity), and
+// the host waits for the copy only when the value is needed, after every
+// launch of the entry point has been queued.  Per thread: one pending count
+// at a time.
+struct PendingCount {
+    const uint32_t* device = nullptr; // the count on the device, readable by later kernels
+    const volatile uint32_t* host = nullptr;
+    hipEvent_t done = nullptr;
+    // stages whose byte model depends on R: bytes = per_pair * R + fixed
+    int slot[4] = { -1, -1, -1, -1 };
+    double per_pair[4] = { 0, 0, 0, 0 }, fixed[4] = { 0, 0, 0, 0 };
+    int n_slots = 0;
+    void add_stage(int s, double pp, double fx)
+    {
+        if (n_slots < 4) {
+            slot[n_slots] = s;
+            per_pair[n_slots] = pp;
+            fixed[n_slots] = fx;
+            ++n_slots;
+        }
+    }
+    // R, or -1 when waiting failed; fills the stage byte models
+    int64_t wait();
+};
+// queues the copy of *dev_count; returns DGMR_OK or DGMR_ERR_DEVICE
+int enqueue_count(const uint32_t* dev_count, hipStream_t st, PendingCount* pc);
 
 // multi-view binning: Gaussian ids span n_views blocks of n_per_view; view v's
 // tiles occupy [views[v].z, views[v].z + views[v].x * views[v].y) of the
@@ -88,8 +119,11 @@ struct BatchBinning {
 // binning shared by both renderers (implemented with the splatter): depth sort,
 // scan, emission, pair sort, tile ranges.  The preprocess must have filled the
 // depth-key and id buffers returned by binning_key_buffers.
+// R comes back through *pending (wait() after the dependent launches); work
+// sized by R is bounded by bins->pair_capacity, and R > capacity leaves a
+// truncated, in-bounds binning that the caller reports as DGMR_ERR_CAPACITY.
 int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, int grid_x, int grid_y, int tile_w,
-    int tile_h, dgmr_binning* bins, Carver& cv, int64_t* n_pairs, hipStream_t st, const BatchBinning* batch);
+    int tile_h, dgmr_binning* bins, Carver& cv, PendingCount* pending, hipStream_t st, const BatchBinning* batch);
 void binning_key_buffers(void* workspace, int64_t n, uint32_t** keys, uint32_t** ids);
 size_t splat_workspace_bytes(int64_t n, int64_t cap);
 

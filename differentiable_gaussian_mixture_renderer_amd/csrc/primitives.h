@@ -22,7 +22,8 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t 
 // Sorts bits [begin_bit, end_bit) in 8-bit digits.  Pass p reads buffer
 // (p % 2 == 0 ? A : B) and writes the other one; the sorted result ends in A
 // when the number of passes is even, in B when it is odd.  n may be given on
-// the device (n_dev != nullptr, capacity n_max) so no host sync is needed.
+// the device (n_dev != nullptr, capacity n_max; min(*n_dev, n_max) items are
+// sorted) so no host sync is needed.
 struct SortBuffers {
     uint32_t* keys_a;
     uint32_t* vals_a;

@@ -130,7 +130,7 @@ constexpr int kRadix = 256;
 
 __device__ __forceinline__ int64_t sort_count(int64_t n_max, const uint32_t* n_dev)
 {
-    return n_dev ? int64_t(*n_dev) : n_max;
+    return n_dev ? min(int64_t(*n_dev), n_max) : n_max;
 }
 
 // per-block digit histogram, stored digit-major: hist[digit * n_blocks + block]

@@ -109,14 +109,18 @@ __device__ __forceinline__ void owner_rect(uint32_t id, const int32_t* __restric
 }
 
 template <class S>
-__global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, uint32_t R, const uint32_t* __restrict__ sorted_ids,
+__global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const uint32_t* __restrict__ n_pairs, uint32_t capacity,
+    const uint32_t* __restrict__ sorted_ids,
     const uint32_t* __restrict__ offsets, const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const S* __restrict__ xy,
     int grid_x, int grid_y, int tile_w, int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out)
 {
     __shared__ uint32_t s_off[kEmitPairs], s_id[kEmitPairs], s_start[kEmitPairs], s_shape[kEmitPairs];
     __shared__ uint32_t s_bounds[2];
+    const uint32_t R = min(*n_pairs, capacity); // pairs past the capacity are not written
     const uint32_t first = blockIdx.x * uint32_t(kEmitPairs);
+    if (first >= R)
+        return;
     const uint32_t end = min(R, first + uint32_t(kEmitPairs));
     const unsigned wave = threadIdx.x / 64;
     if (wave < 2) {
@@ -212,10 +216,11 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, uint32_t
     }
 }
 
-__global__ void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ n_pairs, uint32_t* __restrict__ ranges)
+__global__ void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ n_pairs, uint32_t capacity,
+    uint32_t* __restrict__ ranges)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t R = *n_pairs;
+    const uint32_t R = min(*n_pairs, capacity);
     if (k >= R)
         return;
     const uint32_t tile = keys[k];
@@ -422,7 +427,7 @@ void splat_blend_backward(const dgmr_tiling& t, int n_tiles, hipStream_t st, con
 // pair sort, ranges.  depth_keys/ids are the workspace's A buffers, filled by
 // the preprocess.  Returns the pair count through *n_pairs (host).
 int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, int grid_x, int grid_y, int tile_w,
-    int tile_h, dgmr_binning* bins, Carver& cv, int64_t* n_pairs, hipStream_t st, const BatchBinning* batch)
+    int tile_h, dgmr_binning* bins, Carver& cv, PendingCount* pending, hipStream_t st, const BatchBinning* batch)
 {
     const int4* views = batch ? batch->views : nullptr;
     const int64_t n_per_view = batch ? batch->n_per_view : 0;
@@ -454,18 +459,15 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
         if (bins->point_offsets)
             DGMR_HIP_CHECK(scan_u32(tiles, bins->point_offsets, nullptr, n, true, w.scan_scratch, st));
     }
-    uint32_t R = 0;
-    DGMR_HIP_CHECK(hipMemcpyAsync(&R, w.counter, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    DGMR_HIP_CHECK(hipStreamSynchronize(st));
-    *n_pairs = int64_t(R);
+    // R stays on the device: the copy to the host is awaited by the caller
+    const int rc_count = enqueue_count(w.counter, st, pending);
+    if (rc_count != DGMR_OK)
+        return rc_count;
     const int64_t n_tiles = batch ? batch->total_tiles : int64_t(grid_x) * grid_y;
     DGMR_HIP_CHECK(hipMemsetAsync(bins->ranges, 0, size_t(n_tiles) * 2 * sizeof(uint32_t), st));
-    if (R == 0)
+    const int64_t cap = std::min<int64_t>(bins->pair_capacity, int64_t(UINT32_MAX));
+    if (cap <= 0)
         return DGMR_OK;
-    if (int64_t(R) > bins->pair_capacity) {
-        set_error("pair capacity too small");
-        return DGMR_ERR_CAPACITY;
-    }
     // emit into the buffer pair that leaves the sorted ids in bins->point_list
     const int tile_bits = int(higher_msb(uint32_t(n_tiles)));
     const int passes = radix_passes(0, tile_bits);
@@ -474,25 +476,28 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
         pb = SortBuffers { w.pkeys_a, bins->point_list, w.pkeys_b, w.pvals_a };
     else
         pb = SortBuffers { w.pkeys_a, w.pvals_a, w.pkeys_b, bins->point_list };
-    const unsigned emit_blocks = unsigned(ceil_div64(R, kEmitPairs));
+    const unsigned emit_blocks = unsigned(ceil_div64(cap, kEmitPairs));
     {
-        StageScope sc("bin.emit", st, 8.0 * double(R) + 28.0 * double(n));
+        StageScope sc("bin.emit", st);
+        pending->add_stage(sc.slot, 8.0, 28.0 * double(n));
         if (xy_double)
-            k_emit_pairs<double><<<emit_blocks, kEmitThreads, 0, st>>>(n, R, sorted_ids, w.pair_offsets, rects, tiles,
+            k_emit_pairs<double><<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.pair_offsets, rects, tiles,
                 static_cast<const double*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
         else
-            k_emit_pairs<float><<<emit_blocks, kEmitThreads, 0, st>>>(n, R, sorted_ids, w.pair_offsets, rects, tiles,
+            k_emit_pairs<float><<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.pair_offsets, rects, tiles,
                 static_cast<const float*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
         DGMR_HIP_CHECK(hipGetLastError());
     }
     {
-        StageScope sc("bin.pair_sort", st, 20.0 * double(R) * passes);
-        DGMR_HIP_CHECK(radix_sort_pairs(pb, int64_t(R), nullptr, 0, tile_bits, w.sort_scratch, st));
+        StageScope sc("bin.pair_sort", st);
+        pending->add_stage(sc.slot, 20.0 * passes, 0.0);
+        DGMR_HIP_CHECK(radix_sort_pairs(pb, cap, w.counter, 0, tile_bits, w.sort_scratch, st));
     }
     const uint32_t* sorted_keys = (passes % 2 == 0) ? pb.keys_a : pb.keys_b;
     {
-        StageScope sc("bin.ranges", st, 4.0 * double(R) + 8.0 * double(n_tiles));
-        k_tile_ranges<<<unsigned(ceil_div64(R, 256)), 256, 0, st>>>(sorted_keys, w.counter, bins->ranges);
+        StageScope sc("bin.ranges", st);
+        pending->add_stage(sc.slot, 4.0, 8.0 * double(n_tiles));
+        k_tile_ranges<<<unsigned(ceil_div64(cap, 256)), 256, 0, st>>>(sorted_keys, w.counter, uint32_t(cap), bins->ranges);
         DGMR_HIP_CHECK(hipGetLastError());
     }
     return DGMR_OK;
@@ -528,7 +533,7 @@ int dgmr_splat_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dg
 {
     DGMR_ARG_CHECK(g && cam && geom && bins && framebuffer && final_transmittance && workspace, "dgmr_splat_forward: null argument");
     DGMR_ARG_CHECK(g->dtype == DGMR_F32, "dgmr_splat_forward: the splatter computes in float32");
-    DGMR_ARG_CHECK(valid_tiling(tiling), "dgmr_splat_forward: tiling must be 8x8, 16x16 or 32x8 with batch 128/256/512");
+    DGMR_ARG_CHECK(valid_tiling(tiling), "dgmr_splat_forward: tiling must be 8x8, 16x16 or 32x8 with batch 64/128/256");
     DGMR_ARG_CHECK(g->n >= 0 && g->n < (int64_t(1) << 31), "dgmr_splat_forward: bad Gaussian count");
     DGMR_ARG_CHECK(g->sh_coeffs >= 1 && g->sh_coeffs <= 16 && g->sh_degree >= 0 && g->sh_degree <= 3, "dgmr_splat_forward: bad SH layout");
     DGMR_ARG_CHECK(valid_image(cam->width, cam->height), "dgmr_splat_forward: bad image size");
@@ -560,18 +565,29 @@ int dgmr_splat_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dg
     DGMR_HIP_CHECK(hipGetLastError());
     }
     Carver cv(workspace, workspace_bytes);
-    int64_t R = 0;
-    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, false, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv, &R, st,
-        nullptr);
-    if (stats)
-        stats->n_pairs = R;
+    PendingCount pending;
+    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, false, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv,
+        &pending, st, nullptr);
     if (rc != DGMR_OK)
         return rc;
+    int blend_slot = -1;
     {
-        StageScope sc("splat.blend_forward", st, 40.0 * double(R) + 16.0 * double(plane) + 8.0 * double(n_tiles));
+        StageScope sc("splat.blend_forward", st);
+        blend_slot = sc.slot;
         splat_blend_forward(*tiling, int(n_tiles), st, bins->ranges, bins->point_list, geom->xy, geom->conic_opacity, geom->rgb, cam->width,
             cam->height, dc.grid_x, cam->background, framebuffer, final_transmittance);
         DGMR_HIP_CHECK(hipGetLastError());
+    }
+    pending.add_stage(blend_slot, 40.0, 16.0 * double(plane) + 8.0 * double(n_tiles));
+    // every launch is queued: now wait for the pair count
+    const int64_t R = pending.wait();
+    if (R < 0)
+        return DGMR_ERR_DEVICE;
+    if (stats)
+        stats->n_pairs = R;
+    if (R > bins->pair_capacity) {
+        set_error("pair capacity too small");
+        return DGMR_ERR_CAPACITY;
     }
     return DGMR_OK;
 }
