@@ -101,6 +101,7 @@ class MarchOptions(ctypes.Structure):
         ("max_large_steps", ctypes.c_int32),
         ("n_frozen", ctypes.c_int32),
         ("frozen", (ctypes.c_double * 3) * 8),
+        ("bins_per_chunk", ctypes.c_int32),
     ]
 
 

@@ -27,7 +27,7 @@ from .. import _native as N
 DEFAULT_TILING = ((16, 16), 256)
 # march options: large-step cap (0 = unbounded, as the reference) and frozen
 # sample sections for gradient checks (DGMR_TORCH_GRAD_CHECK_CONST_SAMPLES)
-DEFAULT_OPTIONS = {"max_large_steps": 0, "frozen": None}
+DEFAULT_OPTIONS = {"max_large_steps": 0, "frozen": None, "bins_per_chunk": 0}
 
 
 def _options(max_large_steps=None, frozen=None) -> N.MarchOptions:
@@ -38,6 +38,7 @@ def _options(max_large_steps=None, frozen=None) -> N.MarchOptions:
     if len(fr) > 8:
         raise ValueError("at most 8 frozen sections")
     o.n_frozen = len(fr)
+    o.bins_per_chunk = int(DEFAULT_OPTIONS.get("bins_per_chunk", 0))
     for i, (a, b, c) in enumerate(fr):
         o.frozen[i][0], o.frozen[i][1], o.frozen[i][2] = float(a), float(b), float(c)
     return o

@@ -144,6 +144,7 @@ typedef struct dgmr_march_options {
     int32_t n_frozen;         /* >0: fixed sample sections for gradient checks
                                  (DGMR_TORCH_GRAD_CHECK_CONST_SAMPLES, vol_marcher.h:25) */
     double frozen[8][3];      /* start, end, step of each frozen section */
+    int32_t bins_per_chunk;   /* float path: 16 or 32 bins kept in registers per pass (0 = 32); results do not depend on it */
 } dgmr_march_options;
 
 /* Marcher forward: replaces dgmr::vol_marcher::forward<T>

@@ -209,3 +209,34 @@ def test_volmarcher_autograd_and_large_step_cap():
     fb, _, nl = ref.render(bg, max_large_steps=2)
     assert nl.max() <= 2
     assert np.abs(out["framebuffer"] - fb).max() <= 1e-9
+
+
+def test_chunk_size_and_tiles_do_not_change_results():
+    """The float marcher keeps 16 or 32 bins in registers per pass; a chunk is
+    skipped for a Gaussian only where every bin would fall under the per-bin
+    threshold, so the image and remaining transparency are bit-identical."""
+    import torch
+    from conftest import gpu_available
+    if not gpu_available():
+        pytest.skip("no GPU")
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    from differentiable_gaussian_mixture_renderer_amd.python_bindings import vol_marcher
+    scene = scenes.make_scene(3_000, seed=21, extent=1.5, sh_degree=1)
+    cam = scenes.look_at_camera([0.0, 0.0, -4.0], [0.0, 0.0, 0.0], 96, 64, 0.6, 0.4)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    args = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations), t(cam.world_view_transform),
+            t(cam.full_proj_transform), t(cam.camera_center), t(np.zeros(3, np.float32)), cam.width, cam.height, cam.tan_fovx,
+            cam.tan_fovy, scene.sh_degree]
+    print("[gpu march] chunk sizes 16/32", flush=True)
+    saved = dict(vol_marcher.DEFAULT_OPTIONS)
+    try:
+        out = {}
+        for chunk in (16, 32):
+            vol_marcher.DEFAULT_OPTIONS["bins_per_chunk"] = chunk
+            cache = vol_marcher.cpp_binding.forward(*args, max_large_steps=2)
+            out[chunk] = (cache[0].cpu().numpy(), cache[-1].cpu().numpy())
+    finally:
+        vol_marcher.DEFAULT_OPTIONS.update(saved)
+    np.testing.assert_array_equal(out[16][0], out[32][0])
+    np.testing.assert_array_equal(out[16][1], out[32][1])
