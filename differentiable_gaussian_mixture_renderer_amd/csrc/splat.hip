@@ -271,34 +271,27 @@ __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64
     const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations,
     float* __restrict__ g_opacity, float* __restrict__ g_sh)
 {
-    // SH rows in and SH gradients out go through LDS (coalesced); a row is
-    // overwritten in place by its gradient
-    __shared__ float s_sh[kPreThreads * kShStride];
-    const int64_t first = int64_t(blockIdx.x) * blockDim.x;
-    const int count = int(min<int64_t>(blockDim.x, n - first));
-    stage_sh(s_sh, p.sh, first, count, p.sh_coeffs);
-    __syncthreads();
-    const int64_t i = first + threadIdx.x;
+    // SH rows are read from and their gradients written to global memory
+    // directly: no LDS, so occupancy is bounded by registers only
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int kc = 3 * p.sh_coeffs;
     float g_mu[3] = { 0.f, 0.f, 0.f }, g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f }, g_w = 0.f;
-    float* row = s_sh + threadIdx.x * kShStride;
-    if (i < n) {
-        if (tiles[i] != 0)
-            preprocess_backward_gaussian<true>(i, p, row, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9, g_mu, g_s, g_q, &g_w,
-                nullptr);
-        else
-            for (int k = 0; k < 3 * p.sh_coeffs; ++k)
-                row[k] = 0.f;
-        activation_backward(p, i, g_s, g_q, &g_w);
-        for (int a = 0; a < 3; ++a) {
-            g_means[3 * i + a] = g_mu[a];
-            g_scales[3 * i + a] = g_s[a];
-        }
-        for (int a = 0; a < 4; ++a)
-            g_rotations[4 * i + a] = g_q[a];
-        g_opacity[i] = g_w;
+    if (tiles[i] != 0)
+        preprocess_backward_gaussian<true>(i, p, p.sh + i * kc, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9, g_mu, g_s, g_q,
+            &g_w, g_sh + i * kc);
+    else
+        for (int k = 0; k < kc; ++k)
+            g_sh[i * kc + k] = 0.f;
+    activation_backward(p, i, g_s, g_q, &g_w);
+    for (int a = 0; a < 3; ++a) {
+        g_means[3 * i + a] = g_mu[a];
+        g_scales[3 * i + a] = g_s[a];
     }
-    __syncthreads();
-    unstage_sh(g_sh, s_sh, first, count, p.sh_coeffs);
+    for (int a = 0; a < 4; ++a)
+        g_rotations[4 * i + a] = g_q[a];
+    g_opacity[i] = g_w;
 }
 
 SplatParams make_params(const dgmr_gaussians* g)

@@ -485,6 +485,8 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                 // selects keep the sums exact for pixels that do not blend this Gaussian
 #pragma unroll
                 for (int k = 0; k < NP; ++k) {
+                    if (__ballot(use[k]) == 0)
+                        continue; // no pixel of this row group blends the Gaussian
                     // the same operations as the cheap part, so the same values
                     const float dy = pyf0 + float(RS * k) - g.y;
                     const float raw = c.y * Gk[k];
@@ -533,12 +535,12 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
 // grad::sh_to_colour (grad/math.h:203-282) and normalize.  Adds into the
 // accumulators (g_sh has 48 slots; only k < sh_coeffs are meaningful).
 // c: this Gaussian's SH row.
-// IN_PLACE: c is the Gaussian's SH row in LDS and receives its SH gradient
-// (sh_coeffs x 3 values, read before written); otherwise g_sh_acc (48
-// accumulators) is added to.
-template <bool IN_PLACE = false>
-__device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const SplatParams& p, float* c, const DevCamera<float>& cam,
-    const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc, float* g_q_acc, float* g_w_acc, float* g_sh_acc)
+// SH gradient: ASSIGN writes Y_k * dL/drgb to g_sh (sh_coeffs x 3 values,
+// zero beyond the active degree; g_sh may alias c, each value is read
+// before it is written); otherwise it is added to g_sh (48 accumulators).
+template <bool ASSIGN = false>
+__device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const SplatParams& p, const float* c, const DevCamera<float>& cam,
+    const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc, float* g_q_acc, float* g_w_acc, float* g_sh)
 {
     const float mu[3] = { p.means[3 * i], p.means[3 * i + 1], p.means[3 * i + 2] };
     const float m = p.scale_multiplier;
@@ -566,14 +568,14 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
     for (int k = 0; k < 16; ++k) {
         if (k < kmax) {
             const float c0 = c[3 * k], c1 = c[3 * k + 1], c2 = c[3 * k + 2];
-            if constexpr (IN_PLACE) {
-                c[3 * k + 0] = Y[k] * gr[0];
-                c[3 * k + 1] = Y[k] * gr[1];
-                c[3 * k + 2] = Y[k] * gr[2];
+            if constexpr (ASSIGN) {
+                g_sh[3 * k + 0] = Y[k] * gr[0];
+                g_sh[3 * k + 1] = Y[k] * gr[1];
+                g_sh[3 * k + 2] = Y[k] * gr[2];
             } else {
-                g_sh_acc[3 * k + 0] += Y[k] * gr[0];
-                g_sh_acc[3 * k + 1] += Y[k] * gr[1];
-                g_sh_acc[3 * k + 2] += Y[k] * gr[2];
+                g_sh[3 * k + 0] += Y[k] * gr[0];
+                g_sh[3 * k + 1] += Y[k] * gr[1];
+                g_sh[3 * k + 2] += Y[k] * gr[2];
             }
             const float dot_c = c0 * gr[0] + c1 * gr[1] + c2 * gr[2];
             g_dir[0] = fmaf(dY[k][0], dot_c, g_dir[0]);
@@ -581,9 +583,9 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
             g_dir[2] = fmaf(dY[k][2], dot_c, g_dir[2]);
         }
     }
-    if constexpr (IN_PLACE) {
+    if constexpr (ASSIGN) {
         for (int k = kmax; k < p.sh_coeffs; ++k)
-            c[3 * k] = c[3 * k + 1] = c[3 * k + 2] = 0.f; // beyond the active degree
+            g_sh[3 * k] = g_sh[3 * k + 1] = g_sh[3 * k + 2] = 0.f; // beyond the active degree
     }
     const float dd = dot3(g_dir, dir);
 #pragma unroll
