@@ -165,6 +165,7 @@ def main():
             tr = json.load(f)
         traffic = tr.get(d_name)
 
+    adam = adam_workload(params)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, cam, args.cpu_rows)
@@ -196,11 +197,40 @@ def main():
         "frame_hbm_fraction": round(frame_hbm_frac, 4),
         "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0])},
         "cpu_baseline": cpu,
+        "optimizer_step": adam,
         "batched": batched,
         "serving": served,
         "marcher": marcher,
     }
     print(json.dumps(line))
+
+
+def adam_workload(params, reps: int = 20):
+    """The optimiser step after the backward: Adam over the five parameter
+    tensors of the frame (59 floats per Gaussian), fused kernels against
+    torch.optim.Adam (foreach), same gradients."""
+    import torch
+
+    from differentiable_gaussian_mixture_renderer_amd.optim import FusedAdam
+
+    def timed(make):
+        ps = [p.detach().clone().requires_grad_() for p in params]
+        for p in ps:
+            p.grad = torch.full_like(p, 1e-3)
+        opt = make(ps)
+        opt.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            opt.step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    fused = timed(lambda ps: FusedAdam(ps, lr=1e-3))
+    ref = timed(lambda ps: torch.optim.Adam(ps, lr=1e-3, foreach=True))
+    n_el = sum(p.numel() for p in params)
+    return {"elements": n_el, "fused_ms": round(fused, 4), "torch_adam_ms": round(ref, 4),
+            "fused_gb_s": round(28.0 * n_el / fused / 1e6, 1)}
 
 
 def batched_workload(n: int, n_cams: int, reps: int = 3):

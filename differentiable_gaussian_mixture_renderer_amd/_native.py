@@ -96,6 +96,17 @@ class MarchGeometry(ctypes.Structure):
     ]
 
 
+class AdamTensor(ctypes.Structure):
+    _fields_ = [
+        ("param", c_void_p),
+        ("grad", c_void_p),
+        ("exp_avg", c_void_p),
+        ("exp_avg_sq", c_void_p),
+        ("numel", ctypes.c_int64),
+        ("lr", ctypes.c_float),
+    ]
+
+
 class MarchOptions(ctypes.Structure):
     _fields_ = [
         ("max_large_steps", ctypes.c_int32),
@@ -143,6 +154,8 @@ SIGNATURES = {
     "dgmr_profile_read": (ctypes.c_int, [_P(StageTime), ctypes.c_int32, _P(ctypes.c_int32)]),
     "dgmr_last_error": (ctypes.c_char_p, []),
     "dgmr_version": (ctypes.c_char_p, []),
+    "dgmr_adam_step": (ctypes.c_int, [_P(AdamTensor), ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_int64, c_void_p]),
     "dgmr_selftest_wave_sum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
 

@@ -195,6 +195,26 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
     void* workspace, size_t workspace_bytes, int64_t pair_capacity, int64_t n_pairs, const int32_t* view_group, int32_t n_groups,
     float* grad_means, float* grad_scales, float* grad_rotations, float* grad_opacity, float* grad_sh, dgmr_stream stream);
 
+/* ------------------------------------------------------------ optimiser step
+ * Adam over float32 parameter tensors, the step that follows the backward in
+ * a 3DGS training loop (means, log-scales, quaternions, opacity logits, SH).
+ * No counterpart in the reference, which returns gradients only
+ * (python_bindings/vol_marcher.py:48-65).  Per element, as torch.optim.Adam
+ * (L2 weight decay added to the gradient):
+ *   m = b1 m + (1-b1) g;  v = b2 v + (1-b2) g^2;
+ *   p -= lr / (1-b1^step) * m / (sqrt(v) / sqrt(1-b2^step) + eps). */
+typedef struct dgmr_adam_tensor {
+    float* param;
+    const float* grad;
+    float* exp_avg;
+    float* exp_avg_sq;
+    int64_t numel;
+    float lr;
+} dgmr_adam_tensor;
+
+int dgmr_adam_step(const dgmr_adam_tensor* tensors, int32_t n_tensors, float beta1, float beta2, float eps, float weight_decay, int64_t step,
+    dgmr_stream stream);
+
 /* ------------------------------------------------------------ stage timing
  * When enabled, every entry point brackets each kernel stage with hipEvents on
  * its stream (no extra synchronisation).  dgmr_profile_read waits for the

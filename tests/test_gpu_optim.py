@@ -1,0 +1,39 @@
+"""FusedAdam against torch.optim.Adam: three steps over tensors of the
+Gaussian parameter shapes (sizes with and without a multiple of 4), two
+parameter groups with their own learning rates, L2 weight decay on one.
+Bar: parameters and both moments within 1e-6 relative (float reassociation)."""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fused_adam_matches_torch_adam():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import torch
+    from differentiable_gaussian_mixture_renderer_amd.optim import FusedAdam
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev).manual_seed(0)
+    shapes = [(1001, 3), (1001, 4), (1001, 1), (1001, 16, 3), (7,)]
+    init = [torch.randn(s, generator=gen, device=dev) for s in shapes]
+    grads = [[torch.randn(s, generator=gen, device=dev) for s in shapes] for _ in range(3)]
+    a = [x.clone().requires_grad_() for x in init]
+    b = [x.clone().requires_grad_() for x in init]
+    groups = lambda ps: [{"params": ps[:3], "lr": 1.6e-4}, {"params": ps[3:], "lr": 2.5e-3, "weight_decay": 0.01}]
+    oa = FusedAdam(groups(a), betas=(0.9, 0.999), eps=1e-15)
+    ob = torch.optim.Adam(groups(b), betas=(0.9, 0.999), eps=1e-15, foreach=False)
+    print("[gpu optim] fused Adam, 5 tensors x 3 steps", flush=True)
+    for step in range(3):
+        for p, q, g in zip(a, b, grads[step]):
+            p.grad = g.clone()
+            q.grad = g.clone()
+        oa.step()
+        ob.step()
+    torch.cuda.synchronize()
+    for p, q in zip(a, b):
+        np.testing.assert_allclose(p.detach().cpu().numpy(), q.detach().cpu().numpy(), rtol=1e-6, atol=1e-7)
+        for k in ("exp_avg", "exp_avg_sq"):
+            np.testing.assert_allclose(oa.state[p][k].cpu().numpy(), ob.state[q][k].cpu().numpy(), rtol=1e-6, atol=1e-12)
