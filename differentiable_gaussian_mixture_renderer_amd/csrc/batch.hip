@@ -301,7 +301,12 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     }
     dgmr_binning bins { nullptr, L.point_list, L.ranges, pair_capacity };
     Carver cv(L.binning, L.binning_bytes);
-    const BatchBinning bb { L.grids, std::max<int64_t>(n, 1), n_views, L.total_tiles };
+    int64_t max_view_tiles = 1;
+    for (int v = 0; v < n_views; ++v) {
+        const DevCamera<float> c = make_dev_camera<float>(views[v].cam, *tiling);
+        max_view_tiles = std::max<int64_t>(max_view_tiles, int64_t(c.grid_x) * c.grid_y);
+    }
+    const BatchBinning bb { L.grids, std::max<int64_t>(n, 1), n_views, L.total_tiles, max_view_tiles };
     PendingCount pending;
     if (m > 0) {
         const int rc = bin_gaussians(m, L.rects, L.tiles, L.xy, false, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &pending, st, &bb);

@@ -109,11 +109,15 @@ int enqueue_count(const uint32_t* dev_count, hipStream_t st, PendingCount* pc);
 // multi-view binning: Gaussian ids span n_views blocks of n_per_view; view v's
 // tiles occupy [views[v].z, views[v].z + views[v].x * views[v].y) of the
 // global tile range (views: device array of (grid_x, grid_y, first tile, -))
+// Pairs are sorted on the view-local tile id only: emission is in (view,
+// depth) order and the sort is stable, so each (view, tile) run stays
+// contiguous and in depth order; the tile ranges add the view's first tile.
 struct BatchBinning {
     const int4* views;
     int64_t n_per_view;
     int n_views;
     int64_t total_tiles;
+    int64_t max_view_tiles; // largest grid_x * grid_y over the views
 };
 
 // binning shared by both renderers (implemented with the splatter): depth sort,

@@ -103,8 +103,9 @@ __device__ __forceinline__ void owner_rect(uint32_t id, const int32_t* __restric
     }
     int a, b, c, d;
     get_rect(xy[2 * id], xy[2 * id + 1], rects[2 * id], rects[2 * id + 1], gx, gy, tile_w, tile_h, &a, &b, &c, &d);
-    // a view's tiles occupy [tbase, tbase + gx * gy) of the global tile range
-    *start = uint32_t(tbase) + uint32_t(b) * uint32_t(gx) + uint32_t(a);
+    // sort keys are view-local tile ids (the view comes back through the id)
+    (void)tbase;
+    *start = uint32_t(b) * uint32_t(gx) + uint32_t(a);
     *shape = (uint32_t(gx) << 16) | uint32_t(c - a);
 }
 
@@ -216,17 +217,21 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const ui
     }
 }
 
-__global__ void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ n_pairs, uint32_t capacity,
-    uint32_t* __restrict__ ranges)
+__global__ void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, const uint32_t* __restrict__ n_pairs,
+    uint32_t capacity, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ ranges)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t R = min(*n_pairs, capacity);
     if (k >= R)
         return;
-    const uint32_t tile = keys[k];
-    if (k == 0 || keys[k - 1] != tile)
+    // global tile: the view's first tile plus the sorted (view-local) key
+    const auto global_tile = [&](uint32_t q) {
+        return views ? uint32_t(views[vals[q] / n_per_view].z) + keys[q] : keys[q];
+    };
+    const uint32_t tile = global_tile(k);
+    if (k == 0 || global_tile(k - 1) != tile)
         ranges[2 * tile] = k;
-    if (k + 1 == R || keys[k + 1] != tile)
+    if (k + 1 == R || global_tile(k + 1) != tile)
         ranges[2 * tile + 1] = k + 1;
 }
 
@@ -462,7 +467,7 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     if (cap <= 0)
         return DGMR_OK;
     // emit into the buffer pair that leaves the sorted ids in bins->point_list
-    const int tile_bits = int(higher_msb(uint32_t(n_tiles)));
+    const int tile_bits = int(higher_msb(uint32_t(batch ? batch->max_view_tiles : n_tiles)));
     const int passes = radix_passes(0, tile_bits);
     SortBuffers pb;
     if (passes % 2 == 0)
@@ -487,10 +492,12 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
         DGMR_HIP_CHECK(radix_sort_pairs(pb, cap, w.counter, 0, tile_bits, w.sort_scratch, st));
     }
     const uint32_t* sorted_keys = (passes % 2 == 0) ? pb.keys_a : pb.keys_b;
+    const uint32_t* sorted_vals = (passes % 2 == 0) ? pb.vals_a : pb.vals_b;
     {
         StageScope sc("bin.ranges", st);
-        pending->add_stage(sc.slot, 4.0, 8.0 * double(n_tiles));
-        k_tile_ranges<<<unsigned(ceil_div64(cap, 256)), 256, 0, st>>>(sorted_keys, w.counter, uint32_t(cap), bins->ranges);
+        pending->add_stage(sc.slot, views ? 8.0 : 4.0, 8.0 * double(n_tiles));
+        k_tile_ranges<<<unsigned(ceil_div64(cap, 256)), 256, 0, st>>>(sorted_keys, sorted_vals, w.counter, uint32_t(cap), views, n_per_view,
+            bins->ranges);
         DGMR_HIP_CHECK(hipGetLastError());
     }
     return DGMR_OK;
