@@ -154,8 +154,8 @@ SIGNATURES = {
     "dgmr_profile_read": (ctypes.c_int, [_P(StageTime), ctypes.c_int32, _P(ctypes.c_int32)]),
     "dgmr_last_error": (ctypes.c_char_p, []),
     "dgmr_version": (ctypes.c_char_p, []),
-    "dgmr_adam_step": (ctypes.c_int, [_P(AdamTensor), ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                                      ctypes.c_int64, c_void_p]),
+    "dgmr_adam_step": (ctypes.c_int, [_P(AdamTensor), ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int64, c_void_p]),
     "dgmr_selftest_wave_sum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
 

@@ -15,6 +15,7 @@ namespace {
 
 struct AdamScalars {
     float lr, beta1, beta2, eps, weight_decay;
+    float omb1, omb2;        // 1 - beta1, 1 - beta2, formed in double from the host values
     float bias1, bias2_sqrt; // 1 - beta1^t, sqrt(1 - beta2^t)
 };
 
@@ -22,8 +23,8 @@ __device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, 
 {
     if (a.weight_decay != 0.f)
         g = fmaf(a.weight_decay, p, g);
-    m = fmaf(a.beta1, m, (1.f - a.beta1) * g);
-    v = fmaf(a.beta2, v, (1.f - a.beta2) * g * g);
+    m = fmaf(a.beta1, m, a.omb1 * g);
+    v = fmaf(a.beta2, v, a.omb2 * g * g);
     const float denom = sqrtf(v) / a.bias2_sqrt + a.eps;
     p -= (a.lr / a.bias1) * (m / denom);
 }
@@ -67,15 +68,16 @@ using namespace dgmr;
 
 extern "C" {
 
-int dgmr_adam_step(const dgmr_adam_tensor* tensors, int32_t n_tensors, float beta1, float beta2, float eps, float weight_decay, int64_t step,
-    dgmr_stream stream)
+int dgmr_adam_step(const dgmr_adam_tensor* tensors, int32_t n_tensors, double beta1, double beta2, double eps, double weight_decay,
+    int64_t step, dgmr_stream stream)
 {
     DGMR_ARG_CHECK(tensors && n_tensors >= 0, "dgmr_adam_step: null tensor table");
     DGMR_ARG_CHECK(step >= 1, "dgmr_adam_step: step counts from 1");
-    DGMR_ARG_CHECK(beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f && eps > 0.f, "dgmr_adam_step: bad hyper-parameters");
+    DGMR_ARG_CHECK(beta1 >= 0 && beta1 < 1 && beta2 >= 0 && beta2 < 1 && eps > 0, "dgmr_adam_step: bad hyper-parameters");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    AdamScalars a { 0.f, beta1, beta2, eps, weight_decay, float(1.0 - std::pow(double(beta1), double(step))),
-        float(std::sqrt(1.0 - std::pow(double(beta2), double(step)))) };
+    // scalars formed in double, then rounded once (as torch forms 1 - beta from Python floats)
+    AdamScalars a { 0.f, float(beta1), float(beta2), float(eps), float(weight_decay), float(1.0 - beta1), float(1.0 - beta2),
+        float(1.0 - std::pow(beta1, double(step))), float(std::sqrt(1.0 - std::pow(beta2, double(step)))) };
     for (int t = 0; t < n_tensors; ++t) {
         const dgmr_adam_tensor& x = tensors[t];
         DGMR_ARG_CHECK(x.numel >= 0, "dgmr_adam_step: negative size");

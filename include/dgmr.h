@@ -212,8 +212,8 @@ typedef struct dgmr_adam_tensor {
     float lr;
 } dgmr_adam_tensor;
 
-int dgmr_adam_step(const dgmr_adam_tensor* tensors, int32_t n_tensors, float beta1, float beta2, float eps, float weight_decay, int64_t step,
-    dgmr_stream stream);
+int dgmr_adam_step(const dgmr_adam_tensor* tensors, int32_t n_tensors, double beta1, double beta2, double eps, double weight_decay,
+    int64_t step, dgmr_stream stream);
 
 /* ------------------------------------------------------------ stage timing
  * When enabled, every entry point brackets each kernel stage with hipEvents on
