@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--batched-cams", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run a device-to-device copy of MIB MiB three times (calibrates HBM byte counters) and exit")
+                    help="only run a device-to-device copy and an elementwise multiply of MIB MiB, three times each (calibrates HBM byte counters), and exit")
     return ap.parse_args()
 
 
@@ -57,9 +57,11 @@ def copy_probe(mib: int):
     src = torch.ones(n, device="cuda:0")
     dst = torch.empty_like(src)
     for _ in range(3):
-        dst.copy_(src)
+        dst.copy_(src)  # the runtime's copy kernel
+    for _ in range(3):
+        torch.mul(src, 2.0, out=dst)  # an elementwise kernel, 16 B per lane access
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}))
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches_per_kernel": 3}))
 
 
 def stage_times(lib):

@@ -99,10 +99,11 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
     const int count = int(min<int64_t>(blockDim.x, n - first));
     stage_sh(s_sh, p.sh, first, count, p.sh_coeffs);
     __syncthreads();
+    // the view loop and its group boundaries are uniform over the block, so
+    // every thread takes part in the block-wide staging at a boundary
     const int64_t i = first + threadIdx.x;
-    if (i >= n)
-        return;
-    const float* row = s_sh + threadIdx.x * kShStride;
+    const bool valid = i < n;
+    float* row = s_sh + threadIdx.x * kShStride;
     const int kc = 3 * p.sh_coeffs;
     float g_mu[3], g_s[3], g_q[4], g_w, g_shv[48];
     int cur = -1;
@@ -110,20 +111,34 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
         const int gv = v < n_views ? group[v] : -1;
         if (gv != cur) {
             if (cur >= 0) {
-                // views of group cur done: write its block
-                activation_backward(p, i, g_s, g_q, &g_w);
-                const int64_t o = int64_t(cur) * n + i;
-                for (int a = 0; a < 3; ++a) {
-                    g_means[3 * o + a] = g_mu[a];
-                    g_scales[3 * o + a] = g_s[a];
-                }
-                for (int a = 0; a < 4; ++a)
-                    g_rotations[4 * o + a] = g_q[a];
-                g_opacity[o] = g_w;
+                // views of group cur done: write its block; the SH gradient rows
+                // go through LDS so the block writes them as one contiguous span
+                __syncthreads(); // every thread has read its SH row
+                if (valid) {
+                    activation_backward(p, i, g_s, g_q, &g_w);
+                    const int64_t o = int64_t(cur) * n + i;
+                    for (int a = 0; a < 3; ++a) {
+                        g_means[3 * o + a] = g_mu[a];
+                        g_scales[3 * o + a] = g_s[a];
+                    }
+                    for (int a = 0; a < 4; ++a)
+                        g_rotations[4 * o + a] = g_q[a];
+                    g_opacity[o] = g_w;
 #pragma unroll
-                for (int k = 0; k < 48; ++k)
-                    if (k < kc)
-                        g_sh[o * kc + k] = g_shv[k];
+                    for (int k = 0; k < 48; ++k)
+                        if (k < kc)
+                            row[k] = g_shv[k];
+                }
+                __syncthreads();
+                unstage_sh(g_sh + int64_t(cur) * n * kc, s_sh, first, count, p.sh_coeffs);
+                bool more = false;
+                for (int u = v; u < n_views; ++u)
+                    more |= group[u] >= 0;
+                if (more) {
+                    __syncthreads();
+                    stage_sh(s_sh, p.sh, first, count, p.sh_coeffs);
+                    __syncthreads();
+                }
             }
             cur = gv;
             for (int a = 0; a < 3; ++a)
@@ -135,12 +150,12 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
             for (int k = 0; k < 48; ++k)
                 g_shv[k] = 0.f;
         }
-        if (gv < 0)
+        if (gv < 0 || !valid)
             continue;
         const int64_t o = int64_t(v) * n + i;
         if (tiles[o] != 0)
-            preprocess_backward_gaussian(i, p, const_cast<float*>(row), cams[v], conic_opacity[o], clamped + 3 * o, screen_grads + o * 9, g_mu,
-                g_s, g_q, &g_w, g_shv);
+            preprocess_backward_gaussian(i, p, row, cams[v], conic_opacity[o], clamped + 3 * o, screen_grads + o * 9, g_mu, g_s, g_q, &g_w,
+                g_shv);
     }
 }
 

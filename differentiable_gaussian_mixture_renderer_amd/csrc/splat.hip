@@ -271,32 +271,59 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_blend_backward(const uint3
 
 namespace {
 
+// SH gradient rows (3 * sh_coeffs floats per Gaussian, 192 B at degree 3)
+// are written by the whole wave: each lane stages its Gaussian's factors
+// (16 basis values, 3 colour gradients) in LDS, then the wave writes its 64
+// rows as one contiguous span, 256 B per store instruction.  Per-lane row
+// stores would leave lines partly written while 64 rows of every resident
+// wave compete for L2, and each partial eviction is a separate HBM write.
+constexpr int kShStage = 19; // odd row stride: the 64 lanes' rows hit distinct banks
+
 __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess_backward(int64_t n, SplatParams p, DevCamera<float> cam,
     const uint32_t* __restrict__ tiles, const float4* __restrict__ conic_opacity, const uint8_t* __restrict__ clamped,
     const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations,
     float* __restrict__ g_opacity, float* __restrict__ g_sh)
 {
-    // SH rows are read from and their gradients written to global memory
-    // directly: no LDS, so occupancy is bounded by registers only
+    __shared__ float s_stage[kPreThreads * kShStage];
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
+    const int lane = int(threadIdx.x & 63);
     const int kc = 3 * p.sh_coeffs;
-    float g_mu[3] = { 0.f, 0.f, 0.f }, g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f }, g_w = 0.f;
-    if (tiles[i] != 0)
-        preprocess_backward_gaussian<true>(i, p, p.sh + i * kc, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9, g_mu, g_s, g_q,
-            &g_w, g_sh + i * kc);
-    else
-        for (int k = 0; k < kc; ++k)
-            g_sh[i * kc + k] = 0.f;
-    activation_backward(p, i, g_s, g_q, &g_w);
-    for (int a = 0; a < 3; ++a) {
-        g_means[3 * i + a] = g_mu[a];
-        g_scales[3 * i + a] = g_s[a];
+    float* row = s_stage + threadIdx.x * kShStage;
+    if (i < n) {
+        float g_mu[3] = { 0.f, 0.f, 0.f }, g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f }, g_w = 0.f;
+        if (tiles[i] != 0) {
+            preprocess_backward_gaussian<ShGrad::Export>(i, p, p.sh + i * kc, cam, conic_opacity[i], clamped + 3 * i, screen_grads + i * 9,
+                g_mu, g_s, g_q, &g_w, row);
+        } else {
+            for (int k = 0; k < kShStage; ++k)
+                row[k] = 0.f;
+        }
+        activation_backward(p, i, g_s, g_q, &g_w);
+        for (int a = 0; a < 3; ++a) {
+            g_means[3 * i + a] = g_mu[a];
+            g_scales[3 * i + a] = g_s[a];
+        }
+        for (int a = 0; a < 4; ++a)
+            g_rotations[4 * i + a] = g_q[a];
+        g_opacity[i] = g_w;
     }
-    for (int a = 0; a < 4; ++a)
-        g_rotations[4 * i + a] = g_q[a];
-    g_opacity[i] = g_w;
+    wave_lds_sync();
+    const int64_t first = i - lane; // the wave's first Gaussian
+    if (first >= n)
+        return;
+    const int count = int(min<int64_t>(64, n - first)) * kc;
+    const float* rows = s_stage + (threadIdx.x - lane) * kShStage;
+    float* dst = g_sh + first * kc;
+    // f = g * kc + 3 * k + ch; the quotients come from float products, exact
+    // here since f < 64 * 48 keeps every quotient 1/(2 kc) away from an integer
+    const float inv_kc = 1.f / float(kc);
+    for (int f = lane; f < count; f += 64) {
+        const int g = int((float(f) + 0.5f) * inv_kc);
+        const int e = f - g * kc;
+        const int k = int((float(e) + 0.5f) * (1.f / 3.f));
+        const float* r = rows + g * kShStage;
+        dst[f] = r[k] * r[16 + (e - 3 * k)];
+    }
 }
 
 SplatParams make_params(const dgmr_gaussians* g)

@@ -535,10 +535,13 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
 // grad::sh_to_colour (grad/math.h:203-282) and normalize.  Adds into the
 // accumulators (g_sh has 48 slots; only k < sh_coeffs are meaningful).
 // c: this Gaussian's SH row.
-// SH gradient: ASSIGN writes Y_k * dL/drgb to g_sh (sh_coeffs x 3 values,
-// zero beyond the active degree; g_sh may alias c, each value is read
-// before it is written); otherwise it is added to g_sh (48 accumulators).
-template <bool ASSIGN = false>
+// SH gradient (the outer product Y_k * dL/drgb_ch), by mode:
+//   ShGrad::Add     adds it to g_sh (48 accumulators);
+//   ShGrad::Export  writes its factors, g_sh[k] = Y_k (k < 16, zero beyond the
+//                   active degree) and g_sh[16 + ch] = dL/drgb_ch, for a
+//                   wave-cooperative coalesced write of the products.
+enum class ShGrad { Add, Export };
+template <ShGrad MODE = ShGrad::Add>
 __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const SplatParams& p, const float* c, const DevCamera<float>& cam,
     const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc, float* g_q_acc, float* g_w_acc, float* g_sh)
 {
@@ -568,10 +571,8 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
     for (int k = 0; k < 16; ++k) {
         if (k < kmax) {
             const float c0 = c[3 * k], c1 = c[3 * k + 1], c2 = c[3 * k + 2];
-            if constexpr (ASSIGN) {
-                g_sh[3 * k + 0] = Y[k] * gr[0];
-                g_sh[3 * k + 1] = Y[k] * gr[1];
-                g_sh[3 * k + 2] = Y[k] * gr[2];
+            if constexpr (MODE == ShGrad::Export) {
+                g_sh[k] = Y[k];
             } else {
                 g_sh[3 * k + 0] += Y[k] * gr[0];
                 g_sh[3 * k + 1] += Y[k] * gr[1];
@@ -583,9 +584,14 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
             g_dir[2] = fmaf(dY[k][2], dot_c, g_dir[2]);
         }
     }
-    if constexpr (ASSIGN) {
-        for (int k = kmax; k < p.sh_coeffs; ++k)
-            g_sh[3 * k] = g_sh[3 * k + 1] = g_sh[3 * k + 2] = 0.f; // beyond the active degree
+    if constexpr (MODE == ShGrad::Export) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k >= kmax)
+                g_sh[k] = 0.f; // beyond the active degree
+        g_sh[16] = gr[0];
+        g_sh[17] = gr[1];
+        g_sh[18] = gr[2];
     }
     const float dd = dot3(g_dir, dir);
 #pragma unroll
