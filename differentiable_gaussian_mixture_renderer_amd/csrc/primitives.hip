@@ -124,23 +124,28 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t 
 namespace {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;                           // per thread
-constexpr int kSortTile = kSortThreads * kSortItems;     // 4096 items per block
-constexpr int kRadix = 256;
+#ifndef DGMR_SORT_ITEMS
+#define DGMR_SORT_ITEMS 16
+#endif
+constexpr int kSortItems = DGMR_SORT_ITEMS;              // per thread
+constexpr int kSortTile = kSortThreads * kSortItems;     // items per block
+constexpr int kRadixMax = 256;
 
 __device__ __forceinline__ int64_t sort_count(int64_t n_max, const uint32_t* n_dev)
 {
     return n_dev ? min(int64_t(*n_dev), n_max) : n_max;
 }
 
-// per-block digit histogram, stored digit-major: hist[digit * n_blocks + block]
+// per-block histogram of the digit (key >> shift) & (2^bits - 1), stored
+// digit-major: hist[digit * n_blocks + block]
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n_max, const uint32_t* n_dev,
-    int shift, int n_blocks, uint32_t* __restrict__ hist)
+    int shift, int bits, int n_blocks, uint32_t* __restrict__ hist)
 {
-    __shared__ uint32_t s_hist[4][kRadix];
+    __shared__ uint32_t s_hist[4][kRadixMax];
     const int64_t n = sort_count(n_max, n_dev);
     const int64_t base = int64_t(blockIdx.x) * kSortTile;
-    for (int i = threadIdx.x; i < 4 * kRadix; i += kSortThreads)
+    const uint32_t mask = (1u << bits) - 1u;
+    for (int i = threadIdx.x; i < 4 * kRadixMax; i += kSortThreads)
         (&s_hist[0][0])[i] = 0;
     __syncthreads();
     if (base < n) {
@@ -149,33 +154,35 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
         for (int j = 0; j < kSortItems; ++j) {
             const int64_t i = base + int64_t(j) * kSortThreads + threadIdx.x;
             if (i < n)
-                atomicAdd(&s_hist[wave][(keys[i] >> shift) & 0xffu], 1u);
+                atomicAdd(&s_hist[wave][(keys[i] >> shift) & mask], 1u);
         }
     }
     __syncthreads();
-    const uint32_t c = s_hist[0][threadIdx.x] + s_hist[1][threadIdx.x] + s_hist[2][threadIdx.x] + s_hist[3][threadIdx.x];
-    hist[int64_t(threadIdx.x) * n_blocks + blockIdx.x] = c;
+    if (threadIdx.x <= mask) {
+        const uint32_t c = s_hist[0][threadIdx.x] + s_hist[1][threadIdx.x] + s_hist[2][threadIdx.x] + s_hist[3][threadIdx.x];
+        hist[int64_t(threadIdx.x) * n_blocks + blockIdx.x] = c;
+    }
 }
 
-// Stable scatter.  Wave w of the block owns items [1024w, 1024w + 1024) of
-// the block's 4096 and walks them in 16 rounds of 64: lanes holding the same
-// digit find each other with 8 ballots, the lowest of them advances the
-// wave's running count of that digit, and every item keeps its rank inside
-// (wave, digit).  One block scan over (digit, wave) then places every item in
-// LDS in digit order (stable: digit, wave, round, lane = input order), and the
-// LDS copy leaves the block one contiguous run per digit.  Four barriers per
-// block.
+// Stable scatter.  Wave w of the block owns a contiguous quarter of the
+// block's items and walks it in rounds of 64: lanes holding the same digit
+// find each other with one ballot per digit bit, the lowest of them advances
+// the wave's running count of that digit, and every item keeps its rank
+// inside (wave, digit).  One block scan over (digit, wave) then places every
+// item in LDS in digit order (stable: digit, wave, round, lane = input
+// order), and the LDS copy leaves the block one contiguous run per digit.
+// Four barriers per block.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int n_blocks,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int bits, int n_blocks,
     const uint32_t* __restrict__ offsets)
 {
     constexpr int kWaves = kSortThreads / 64;
     constexpr int kRounds = kSortTile / kSortThreads; // rounds of 64 items per wave
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
-    __shared__ uint32_t s_count[kWaves][kRadix]; // per-wave digit counts, then per-wave starts in LDS order
-    __shared__ uint32_t s_local[kRadix];         // block-local start of each digit
-    __shared__ uint32_t s_global[kRadix];        // global start of each digit for this block
+    __shared__ uint32_t s_count[kWaves][kRadixMax]; // per-wave digit counts, then per-wave starts in LDS order
+    __shared__ uint32_t s_local[kRadixMax];         // block-local start of each digit
+    __shared__ uint32_t s_global[kRadixMax];        // global start of each digit for this block
     __shared__ uint32_t s_scan[kWaves];
     const int64_t n = sort_count(n_max, n_dev);
     const int64_t base = int64_t(blockIdx.x) * kSortTile;
@@ -185,6 +192,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / 64;
     const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+    const uint32_t mask = (1u << bits) - 1u;
     for (int w = 0; w < kWaves; ++w)
         s_count[w][threadIdx.x] = 0;
     __syncthreads();
@@ -200,13 +208,15 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     for (int r = 0; r < kRounds; ++r) {
         const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
         const bool valid = i < count;
-        const uint32_t digit = (keys[r] >> shift) & 0xffu;
+        const uint32_t digit = (keys[r] >> shift) & mask;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            const bool bit = (digit >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
+            if (b < bits) {
+                const bool bit = (digit >> b) & 1u;
+                const uint64_t m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
         }
         // the wave's LDS operations run in order: every lane reads the running
         // count before the lowest lane of its digit group writes it back
@@ -240,7 +250,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
         for (unsigned w = 0; w < wave; ++w)
             start += s_scan[w];
         s_local[d] = start;
-        s_global[d] = offsets[int64_t(d) * n_blocks + blockIdx.x];
+        if (d <= mask)
+            s_global[d] = offsets[int64_t(d) * n_blocks + blockIdx.x];
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
             s_count[w][d] = start;
@@ -252,7 +263,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     for (int r = 0; r < kRounds; ++r) {
         const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
         if (i < count) {
-            const uint32_t pos = s_count[wave][(keys[r] >> shift) & 0xffu] + rank[r];
+            const uint32_t pos = s_count[wave][(keys[r] >> shift) & mask] + rank[r];
             s_keys[pos] = keys[r];
             s_vals[pos] = vals[r];
         }
@@ -260,7 +271,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     __syncthreads();
     for (int q = threadIdx.x; q < count; q += kSortThreads) {
         const uint32_t key = s_keys[q];
-        const uint32_t d = (key >> shift) & 0xffu;
+        const uint32_t d = (key >> shift) & mask;
         const uint32_t dst = s_global[d] + (uint32_t(q) - s_local[d]);
         keys_out[dst] = key;
         vals_out[dst] = s_vals[q];
@@ -274,11 +285,19 @@ int radix_passes(int begin_bit, int end_bit)
     return end_bit <= begin_bit ? 0 : (end_bit - begin_bit + 7) / 8;
 }
 
+// the passes split the bits evenly (13 bits: 7 + 6, not 8 + 5): fewer
+// digits per pass mean longer runs per digit in each block's output
+static int digit_bits(int begin_bit, int end_bit)
+{
+    const int passes = radix_passes(begin_bit, end_bit);
+    return passes == 0 ? 0 : (end_bit - begin_bit + passes - 1) / passes;
+}
+
 size_t radix_scratch_bytes(int64_t n_max)
 {
     const int64_t blocks = std::max<int64_t>(1, ceil_div(n_max, kSortTile));
-    const size_t hist = size_t(blocks) * kRadix * sizeof(uint32_t);
-    return ((hist + 255) / 256) * 256 + scan_scratch_bytes(blocks * kRadix);
+    const size_t hist = size_t(blocks) * kRadixMax * sizeof(uint32_t);
+    return ((hist + 255) / 256) * 256 + scan_scratch_bytes(blocks * kRadixMax);
 }
 
 hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit, void* scratch,
@@ -287,20 +306,22 @@ hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_
     const int passes = radix_passes(begin_bit, end_bit);
     if (n_max <= 0 || passes == 0)
         return hipSuccess;
+    const int width = digit_bits(begin_bit, end_bit);
     const int blocks = int(std::max<int64_t>(1, ceil_div(n_max, kSortTile)));
     uint32_t* hist = static_cast<uint32_t*>(scratch);
-    void* scan_scratch = static_cast<char*>(scratch) + ((size_t(blocks) * kRadix * sizeof(uint32_t) + 255) / 256) * 256;
+    void* scan_scratch = static_cast<char*>(scratch) + ((size_t(blocks) * kRadixMax * sizeof(uint32_t) + 255) / 256) * 256;
     for (int p = 0; p < passes; ++p) {
-        const int shift = begin_bit + 8 * p;
+        const int shift = begin_bit + width * p;
+        const int bits = std::min(width, end_bit - shift);
         const uint32_t* kin = (p % 2 == 0) ? buf.keys_a : buf.keys_b;
         const uint32_t* vin = (p % 2 == 0) ? buf.vals_a : buf.vals_b;
         uint32_t* kout = (p % 2 == 0) ? buf.keys_b : buf.keys_a;
         uint32_t* vout = (p % 2 == 0) ? buf.vals_b : buf.vals_a;
-        k_radix_hist<<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, blocks, hist);
-        hipError_t e = scan_u32(hist, hist, nullptr, int64_t(blocks) * kRadix, false, scan_scratch, stream);
+        k_radix_hist<<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist);
+        hipError_t e = scan_u32(hist, hist, nullptr, int64_t(blocks) << bits, false, scan_scratch, stream);
         if (e != hipSuccess)
             return e;
-        k_radix_scatter<<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, blocks, hist);
+        k_radix_scatter<<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist);
         e = hipGetLastError();
         if (e != hipSuccess)
             return e;

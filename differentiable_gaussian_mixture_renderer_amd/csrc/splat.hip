@@ -649,6 +649,32 @@ int dgmr_splat_backward(const dgmr_gaussians* g, const dgmr_camera* cam, const d
     return DGMR_OK;
 }
 
+int dgmr_sort_workspace_size(int64_t n, size_t* bytes)
+{
+    DGMR_ARG_CHECK(bytes && n >= 0 && n <= int64_t(UINT32_MAX), "dgmr_sort_workspace_size: n out of range");
+    *bytes = radix_scratch_bytes(std::max<int64_t>(n, 1));
+    return DGMR_OK;
+}
+
+int dgmr_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, int64_t n, int32_t begin_bit, int32_t end_bit,
+    void* ws, size_t ws_bytes, dgmr_stream stream)
+{
+    DGMR_ARG_CHECK(n >= 0 && n <= int64_t(UINT32_MAX), "dgmr_sort_pairs: n out of range");
+    DGMR_ARG_CHECK(begin_bit >= 0 && end_bit <= 32 && begin_bit <= end_bit, "dgmr_sort_pairs: bits must satisfy 0 <= begin <= end <= 32");
+    if (n == 0 || begin_bit == end_bit)
+        return DGMR_OK;
+    DGMR_ARG_CHECK(keys && vals && keys_tmp && vals_tmp && ws, "dgmr_sort_pairs: null buffer");
+    DGMR_ARG_CHECK(ws_bytes >= radix_scratch_bytes(n), "dgmr_sort_pairs: workspace too small");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const SortBuffers buf { keys, vals, keys_tmp, vals_tmp };
+    DGMR_HIP_CHECK(radix_sort_pairs(buf, n, nullptr, begin_bit, end_bit, ws, st));
+    if (radix_passes(begin_bit, end_bit) % 2 == 1) {
+        DGMR_HIP_CHECK(hipMemcpyAsync(keys, keys_tmp, size_t(n) * 4, hipMemcpyDeviceToDevice, st));
+        DGMR_HIP_CHECK(hipMemcpyAsync(vals, vals_tmp, size_t(n) * 4, hipMemcpyDeviceToDevice, st));
+    }
+    return DGMR_OK;
+}
+
 int dgmr_selftest_wave_sum(const float* in, float* out, dgmr_stream stream)
 {
     DGMR_ARG_CHECK(in && out, "dgmr_selftest_wave_sum: null argument");

@@ -231,6 +231,18 @@ typedef struct dgmr_stage_time {
 int dgmr_profile_enable(int on);
 int dgmr_profile_read(dgmr_stage_time* out, int32_t max_stages, int32_t* n_stages);
 
+/* ------------------------------------------------------------ sort
+ * Stable LSD radix sort of n (uint32 key, uint32 value) pairs on key bits
+ * [begin_bit, end_bit), the primitive both binnings use (replaces
+ * cub::DeviceRadixSort::SortPairs, splat_forward.cu:193-211 and
+ * vol_marcher_forward.cu:247-263).  keys/vals (device, n each) are sorted in
+ * place; keys_tmp/vals_tmp (device, n each) are the other half of the
+ * ping-pong; ws (device) holds at least dgmr_sort_workspace_size(n) bytes.
+ * Bits outside [begin_bit, end_bit) do not take part in the order. */
+int dgmr_sort_workspace_size(int64_t n, size_t* bytes);
+int dgmr_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, int64_t n, int32_t begin_bit, int32_t end_bit,
+    void* ws, size_t ws_bytes, dgmr_stream stream);
+
 /* ------------------------------------------------------------ self-test
  * Runs the blend backward's wave reductions on one wave: in holds 9 x 64
  * floats (device), out receives 128 (device): out[8m] = sum over lanes of

@@ -39,3 +39,35 @@ def test_wave_sum8_lane_layout(seed):
         q = ORDER[m]
         assert abs(o[8 * m] - want[q]) <= 1e-5 * scale[q], (m, q, o[8 * m], want[q])
     assert abs(o[64 + 63] - want[8]) <= 1e-5 * scale[8]
+
+
+@pytest.mark.parametrize("n,begin,end", [(0, 0, 13), (1, 0, 13), (4095, 0, 13), (70001, 0, 13), (70001, 0, 32), (12345, 3, 20),
+                                         (50000, 0, 6), (100003, 0, 17)])
+def test_sort_pairs_matches_stable_argsort(n, begin, end):
+    """Bit-exact against numpy's stable sort on the key bits [begin, end); keys
+    carry random bits outside that range, which must not take part."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import torch
+    from differentiable_gaussian_mixture_renderer_amd import _native as N
+    lib = N.load()
+    rng = np.random.default_rng(n + 7 * end)
+    # few distinct digits so stability is exercised
+    keys = rng.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+    if n > 4:
+        keys[: n // 2] &= np.uint32(0xFFFFFFFF ^ (((1 << (end - begin)) - 1) << begin) | (7 << begin))
+    vals = np.arange(n, dtype=np.uint32)
+    field = (keys.astype(np.uint64) >> np.uint64(begin)) & np.uint64((1 << (end - begin)) - 1)
+    order = np.argsort(field, kind="stable")
+    dev = torch.device("cuda:0")
+    as_dev = lambda a: torch.tensor(a.view(np.int32), device=dev)
+    k, v = as_dev(keys), as_dev(vals)
+    kt, vt = torch.empty_like(k), torch.empty_like(v)
+    nbytes = ctypes.c_size_t(0)
+    assert lib.dgmr_sort_workspace_size(n, ctypes.byref(nbytes)) == N.DGMR_OK
+    ws = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=dev)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    assert lib.dgmr_sort_pairs(p(k), p(v), p(kt), p(vt), n, begin, end, p(ws), nbytes.value, None) == N.DGMR_OK
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(v.cpu().numpy().view(np.uint32), order.astype(np.uint32))
+    np.testing.assert_array_equal(k.cpu().numpy().view(np.uint32), keys[order])
