@@ -12,7 +12,7 @@
 //      Steps 2-5 yield exactly the order of the reference's single sort on
 //      (tile << 32 | depth bits): within a tile, pairs stay in (depth bits,
 //      Gaussian id) order, which is the reference's stable order.
-//   6. k_tile_ranges        [first, last) per tile
+//   6. k_tile_ranges        [first, last) per tile (binary search per tile)
 //   7. k_blend_forward      per tile, LDS-staged batches of Gaussians
 // Backward: k_blend_backward (front-to-back replay, transmittance recomputed,
 // wave-reduced atomics) then k_splat_preprocess_backward (per Gaussian).
@@ -52,8 +52,8 @@ __global__ void k_gather_counts(int64_t n, const uint32_t* __restrict__ ids, con
 
 // Emission of (tile, Gaussian) pairs balanced over pairs: block b writes pairs
 // [b * kEmitPairs, (b + 1) * kEmitPairs) of the depth-ordered pair list.  Its
-// first and last owners come from a 64-ary search of the exclusive offsets;
-// the owners in between (culled Gaussians carry the largest depth key, so the
+// first owner and an upper bound of its last come from k_emit_bounds; the
+// owners in between (culled Gaussians carry the largest depth key, so the
 // owners of a block's pairs are contiguous and each owns at least one pair)
 // are staged in LDS with their tile rectangle, and every thread writes 8
 // consecutive pairs.  A window wider than the LDS stage (only possible with
@@ -64,24 +64,27 @@ constexpr int kEmitThreads = 256;
 constexpr int kEmitPer = 8;
 constexpr int kEmitPairs = kEmitThreads * kEmitPer;
 
-// last j in [lo, hi] with off[j] <= t, given off[lo] <= t (one wave, 64-ary)
-__device__ uint32_t wave_owner_search(const uint32_t* __restrict__ off, uint32_t lo, uint32_t hi, uint32_t t)
+// First owners of the emission blocks: bounds[b] is the Gaussian (position
+// in depth order) that owns pair b * kEmitPairs, for every block that emits,
+// and bounds[n_blocks] owns the last emitted pair.  Position k owns pairs
+// [offsets[k], offsets[k + 1]) and writes the entries whose pair falls in its
+// range, so each entry is written exactly once and no block searches.
+__global__ void k_emit_bounds(int64_t n, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ n_pairs, uint32_t capacity,
+    uint32_t* __restrict__ bounds)
 {
-    const unsigned lane = lane_id();
-    while (hi > lo) {
-        const uint32_t len = hi - lo + 1;
-        const uint32_t stride = (len + 63) / 64;
-        const uint32_t idx = lo + lane * stride;
-        const bool pred = idx <= hi && off[idx] <= t;
-        const uint64_t m = __ballot(pred);
-        const unsigned last = 63 - __clzll(m); // lane 0 always holds
-        const uint32_t nlo = lo + last * stride;
-        hi = min(hi, nlo + stride - 1);
-        lo = nlo;
-        if (stride == 1)
-            break;
-    }
-    return lo;
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    const uint64_t total = *n_pairs;
+    const uint64_t R = min(total, uint64_t(capacity)); // pairs past the capacity are not emitted
+    const uint64_t lo = offsets[k];
+    const uint64_t hi = min(k + 1 < n ? uint64_t(offsets[k + 1]) : total, R);
+    if (lo >= hi)
+        return;
+    for (uint64_t b = (lo + kEmitPairs - 1) / kEmitPairs; b * kEmitPairs < hi; ++b)
+        bounds[b] = uint32_t(k);
+    if (R - 1 < hi)
+        bounds[(R + kEmitPairs - 1) / kEmitPairs] = uint32_t(k);
 }
 
 // tile rectangle of Gaussian id: first tile and (grid_x << 16 | width); 0 shape when it has no tiles
@@ -96,7 +99,7 @@ __device__ __forceinline__ void owner_rect(uint32_t id, const int32_t* __restric
         return;
     int gx = grid_x, gy = grid_y, tbase = 0;
     if (views) {
-        const int4 v = views[id / n_per_view];
+        const int4 v = views[id / uint32_t(n_per_view)]; // ids < 2^32: a 32-bit division
         gx = v.x;
         gy = v.y;
         tbase = v.z;
@@ -111,26 +114,20 @@ __device__ __forceinline__ void owner_rect(uint32_t id, const int32_t* __restric
 
 template <class S>
 __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const uint32_t* __restrict__ n_pairs, uint32_t capacity,
-    const uint32_t* __restrict__ sorted_ids,
+    const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ bounds,
     const uint32_t* __restrict__ offsets, const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const S* __restrict__ xy,
     int grid_x, int grid_y, int tile_w, int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out)
 {
     __shared__ uint32_t s_off[kEmitPairs], s_id[kEmitPairs], s_start[kEmitPairs], s_shape[kEmitPairs];
-    __shared__ uint32_t s_bounds[2];
     const uint32_t R = min(*n_pairs, capacity); // pairs past the capacity are not written
     const uint32_t first = blockIdx.x * uint32_t(kEmitPairs);
     if (first >= R)
         return;
     const uint32_t end = min(R, first + uint32_t(kEmitPairs));
-    const unsigned wave = threadIdx.x / 64;
-    if (wave < 2) {
-        const uint32_t g = wave_owner_search(offsets, 0, uint32_t(n - 1), wave == 0 ? first : end - 1);
-        if (lane_id() == 0)
-            s_bounds[wave] = g;
-    }
-    __syncthreads();
-    const uint32_t g_lo = s_bounds[0], g_hi = s_bounds[1];
+    // owners of the block's first pair and of the next block's first pair (or
+    // of the last pair): the owners of this block's pairs lie in between
+    const uint32_t g_lo = bounds[blockIdx.x], g_hi = bounds[blockIdx.x + 1];
     const uint32_t window = g_hi - g_lo + 1;
     if (window <= uint32_t(kEmitPairs)) {
         for (uint32_t k = threadIdx.x; k < window; k += kEmitThreads) {
@@ -217,22 +214,61 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const ui
     }
 }
 
-__global__ void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, const uint32_t* __restrict__ n_pairs,
-    uint32_t capacity, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ ranges)
+// [first, last) of every tile by binary search over the sorted pairs: one
+// thread per tile, every tile written (an empty tile gets (0, 0)).
+// The sorted order is (view-local tile, view, depth); a pair's view is
+// val / n_per_view, and val >= v * n_per_view exactly when its view is >= v.
+// So both ends of tile t of view v are partition points of a predicate that
+// is monotone along the array: "key > t, or key == t and val >= bound".  With
+// one view the bound is 0 and the two ends are the lower bounds of t and t + 1.
+// The two searches of a thread run interleaved.
+__global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+    const uint32_t* __restrict__ n_pairs, uint32_t capacity, const int4* __restrict__ views, int n_views, int64_t n_per_view, int64_t n_tiles,
+    uint32_t* __restrict__ ranges)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t R = min(*n_pairs, capacity);
-    if (k >= R)
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n_tiles)
         return;
-    // global tile: the view's first tile plus the sorted (view-local) key
-    const auto global_tile = [&](uint32_t q) {
-        return views ? uint32_t(views[vals[q] / n_per_view].z) + keys[q] : keys[q];
-    };
-    const uint32_t tile = global_tile(k);
-    if (k == 0 || global_tile(k - 1) != tile)
-        ranges[2 * tile] = k;
-    if (k + 1 == R || global_tile(k + 1) != tile)
-        ranges[2 * tile + 1] = k + 1;
+    const uint32_t R = min(*n_pairs, capacity);
+    uint32_t t0 = uint32_t(g), t1 = uint32_t(g) + 1;
+    uint64_t b0 = 0, b1 = 0;
+    if (views) {
+        int lo = 0, hi = n_views - 1; // last view whose first tile is <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (int64_t(views[mid].z) <= g)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        t0 = t1 = uint32_t(g - views[lo].z);
+        b0 = uint64_t(lo) * uint64_t(n_per_view);
+        b1 = b0 + uint64_t(n_per_view);
+    }
+    uint32_t lo0 = 0, hi0 = R, lo1 = 0, hi1 = R;
+    while (lo0 < hi0 || lo1 < hi1) {
+        const uint32_t m0 = lo0 + (hi0 - lo0) / 2, m1 = lo1 + (hi1 - lo1) / 2;
+        const bool a0 = lo0 < hi0, a1 = lo1 < hi1;
+        const uint32_t k0 = a0 ? keys[m0] : 0u, k1 = a1 ? keys[m1] : 0u;
+        const uint32_t v0 = a0 && views ? vals[m0] : 0u, v1 = a1 && views ? vals[m1] : 0u;
+        const bool after0 = k0 > t0 || (k0 == t0 && uint64_t(v0) >= b0);
+        const bool after1 = k1 > t1 || (k1 == t1 && uint64_t(v1) >= b1);
+        if (a0) {
+            if (after0)
+                hi0 = m0;
+            else
+                lo0 = m0 + 1;
+        }
+        if (a1) {
+            if (after1)
+                hi1 = m1;
+            else
+                lo1 = m1 + 1;
+        }
+    }
+    // an empty tile holds (0, 0), as the reference's zero-filled ranges
+    ranges[2 * g] = lo0 < lo1 ? lo0 : 0u;
+    ranges[2 * g + 1] = lo0 < lo1 ? lo1 : 0u;
 }
 
 // kBlendWaves independent waves per workgroup, one tile each, LDS private per wave
@@ -348,7 +384,7 @@ __global__ void k_view_keys(int64_t n, const uint32_t* __restrict__ ids, int64_t
 {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j < n)
-        keys[j] = uint32_t(ids[j] / n_per_view);
+        keys[j] = ids[j] / uint32_t(n_per_view); // ids < 2^32: a 32-bit division
 }
 } // namespace
 
@@ -359,6 +395,7 @@ struct SplatWorkspace {
     uint32_t* dkeys_b;
     uint32_t* dvals_b;
     uint32_t* counts;
+    uint32_t* emit_bounds;
     uint32_t* pair_offsets;
     uint32_t* pkeys_a;
     uint32_t* pvals_a;
@@ -382,6 +419,7 @@ static SplatWorkspace carve_splat(Carver& cv, int64_t n, int64_t cap)
     w.pvals_a = cv.take<uint32_t>(cc);
     w.pkeys_b = cv.take<uint32_t>(cc);
     w.counter = cv.take<uint32_t>(64);
+    w.emit_bounds = cv.take<uint32_t>(cc / kEmitPairs + 2);
     w.sort_scratch = cv.take<char>(radix_scratch_bytes(std::max(nn, cc)));
     w.scan_scratch = cv.take<char>(scan_scratch_bytes(nn));
     return w;
@@ -489,10 +527,11 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     if (rc_count != DGMR_OK)
         return rc_count;
     const int64_t n_tiles = batch ? batch->total_tiles : int64_t(grid_x) * grid_y;
-    DGMR_HIP_CHECK(hipMemsetAsync(bins->ranges, 0, size_t(n_tiles) * 2 * sizeof(uint32_t), st));
     const int64_t cap = std::min<int64_t>(bins->pair_capacity, int64_t(UINT32_MAX));
-    if (cap <= 0)
+    if (cap <= 0) {
+        DGMR_HIP_CHECK(hipMemsetAsync(bins->ranges, 0, size_t(n_tiles) * 2 * sizeof(uint32_t), st));
         return DGMR_OK;
+    }
     // emit into the buffer pair that leaves the sorted ids in bins->point_list
     const int tile_bits = int(higher_msb(uint32_t(batch ? batch->max_view_tiles : n_tiles)));
     const int passes = radix_passes(0, tile_bits);
@@ -505,11 +544,13 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     {
         StageScope sc("bin.emit", st);
         pending->add_stage(sc.slot, 8.0, 28.0 * double(n));
+        k_emit_bounds<<<blocks_n, 256, 0, st>>>(n, w.pair_offsets, w.counter, uint32_t(cap), w.emit_bounds);
+        DGMR_HIP_CHECK(hipGetLastError());
         if (xy_double)
-            k_emit_pairs<double><<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.pair_offsets, rects, tiles,
+            k_emit_pairs<double><<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.emit_bounds, w.pair_offsets, rects, tiles,
                 static_cast<const double*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
         else
-            k_emit_pairs<float><<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.pair_offsets, rects, tiles,
+            k_emit_pairs<float><<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.emit_bounds, w.pair_offsets, rects, tiles,
                 static_cast<const float*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
         DGMR_HIP_CHECK(hipGetLastError());
     }
@@ -522,9 +563,9 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     const uint32_t* sorted_vals = (passes % 2 == 0) ? pb.vals_a : pb.vals_b;
     {
         StageScope sc("bin.ranges", st);
-        pending->add_stage(sc.slot, views ? 8.0 : 4.0, 8.0 * double(n_tiles));
-        k_tile_ranges<<<unsigned(ceil_div64(cap, 256)), 256, 0, st>>>(sorted_keys, sorted_vals, w.counter, uint32_t(cap), views, n_per_view,
-            bins->ranges);
+        pending->add_stage(sc.slot, 0.0, 8.0 * double(n_tiles));
+        k_tile_ranges<<<unsigned(ceil_div64(n_tiles, 256)), 256, 0, st>>>(sorted_keys, sorted_vals, w.counter, uint32_t(cap), views,
+            batch ? int(batch->n_views) : 0, n_per_view, n_tiles, bins->ranges);
         DGMR_HIP_CHECK(hipGetLastError());
     }
     return DGMR_OK;
