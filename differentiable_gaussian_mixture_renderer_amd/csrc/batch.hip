@@ -305,13 +305,14 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
         stats->n_pairs = 0;
     const int64_t m = n * n_views;
     uint32_t *dkeys, *dids;
-    binning_key_buffers(L.binning, m, &dkeys, &dids);
+    uint2* brects;
+    binning_key_buffers(L.binning, m, &dkeys, &dids, &brects);
     if (n > 0) {
         StageScope sc("batch.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 67.0);
         const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
             static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), g->activation };
         k_batch_preprocess<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams,
-            SplatOut { L.rects, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, L.depths, dkeys, dids });
+            SplatOut { L.rects, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, L.depths, dkeys, dids, brects });
         DGMR_HIP_CHECK(hipGetLastError());
     }
     dgmr_binning bins { nullptr, L.point_list, L.ranges, pair_capacity };
@@ -324,7 +325,7 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     const BatchBinning bb { L.grids, std::max<int64_t>(n, 1), n_views, L.total_tiles, max_view_tiles };
     PendingCount pending;
     if (m > 0) {
-        const int rc = bin_gaussians(m, L.rects, L.tiles, L.xy, false, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &pending, st, &bb);
+        const int rc = bin_gaussians(m, L.rects, L.tiles, L.xy, false, true, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &pending, st, &bb);
         if (rc != DGMR_OK)
             return rc;
     } else {

@@ -6,8 +6,9 @@
 //   1. k_splat_preprocess   per Gaussian: EWA splat (Ots, 0.3 px^2 filter), rect,
 //                           tiles touched, depth key, SH colour, conic.
 //   2. radix sort of the Gaussians by depth bits (stable, ties by index)
-//   3. gather tiles touched in depth order + exclusive scan -> pair offsets
-//   4. k_emit_pairs         wave-balanced emission of (tile id, Gaussian id)
+//   3. k_gather_owners      tiles touched and tile rectangle in depth order,
+//                           exclusive scan -> pair offsets
+//   4. k_emit_bounds, k_emit_pairs   pair-balanced emission of (tile id, Gaussian id)
 //   5. radix sort of the pairs by tile id only (stable)
 //      Steps 2-5 yield exactly the order of the reference's single sort on
 //      (tile << 32 | depth bits): within a tile, pairs stay in (depth bits,
@@ -39,13 +40,6 @@ __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess(int64_t n, Spl
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n)
         preprocess_gaussian(i, i, p, p.sh + i * p.sh_coeffs * 3, cam, out);
-}
-
-__global__ void k_gather_counts(int64_t n, const uint32_t* __restrict__ ids, const uint32_t* __restrict__ tiles, uint32_t* __restrict__ counts)
-{
-    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j < n)
-        counts[j] = tiles[ids[j]];
 }
 
 } // namespace
@@ -112,12 +106,39 @@ __device__ __forceinline__ void owner_rect(uint32_t id, const int32_t* __restric
     *shape = (uint32_t(gx) << 16) | uint32_t(c - a);
 }
 
-template <class S>
+// Per position j of the depth order: the pair count of Gaussian ids[j] (for
+// the scan) and its tile rectangle (for the emission).  The random reads of
+// per-Gaussian data happen here, in a streaming kernel, so the emission reads
+// its owners' rectangles as contiguous runs; with the preprocess's packed bin
+// rectangle that is one 8-byte read per Gaussian.
+template <class S, bool RECTS>
+__global__ __launch_bounds__(256) void k_gather_owners(int64_t n, const uint32_t* __restrict__ ids, const uint2* __restrict__ bin_rects,
+    const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const S* __restrict__ xy, int grid_x, int grid_y, int tile_w,
+    int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ counts, uint2* __restrict__ owner_rects)
+{
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n)
+        return;
+    const uint32_t id = ids[j];
+    uint32_t start, shape, count;
+    if constexpr (RECTS) {
+        const uint2 r = bin_rects[id];
+        const uint32_t cols = r.y & 0xffffu, rows = r.y >> 16;
+        const uint32_t gx = views ? uint32_t(views[id / uint32_t(n_per_view)].x) : uint32_t(grid_x);
+        start = r.x;
+        shape = cols * rows == 0 ? 0u : (gx << 16) | cols;
+        count = cols * rows;
+    } else {
+        owner_rect(id, rects, tiles, xy, grid_x, grid_y, tile_w, tile_h, views, n_per_view, &start, &shape);
+        count = tiles[id];
+    }
+    counts[j] = count;
+    owner_rects[j] = make_uint2(start, shape);
+}
+
 __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const uint32_t* __restrict__ n_pairs, uint32_t capacity,
-    const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ bounds,
-    const uint32_t* __restrict__ offsets, const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const S* __restrict__ xy,
-    int grid_x, int grid_y, int tile_w, int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ keys_out,
-    uint32_t* __restrict__ vals_out)
+    const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ bounds, const uint32_t* __restrict__ offsets,
+    const uint2* __restrict__ owner_rects, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
 {
     __shared__ uint32_t s_off[kEmitPairs], s_id[kEmitPairs], s_start[kEmitPairs], s_shape[kEmitPairs];
     const uint32_t R = min(*n_pairs, capacity); // pairs past the capacity are not written
@@ -131,10 +152,11 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const ui
     const uint32_t window = g_hi - g_lo + 1;
     if (window <= uint32_t(kEmitPairs)) {
         for (uint32_t k = threadIdx.x; k < window; k += kEmitThreads) {
-            const uint32_t id = sorted_ids[g_lo + k];
+            const uint2 r = owner_rects[g_lo + k];
             s_off[k] = offsets[g_lo + k];
-            s_id[k] = id;
-            owner_rect(id, rects, tiles, xy, grid_x, grid_y, tile_w, tile_h, views, n_per_view, &s_start[k], &s_shape[k]);
+            s_id[k] = sorted_ids[g_lo + k];
+            s_start[k] = r.x;
+            s_shape[k] = r.y;
         }
         __syncthreads();
         const uint32_t p0 = first + threadIdx.x * kEmitPer;
@@ -203,8 +225,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const ui
                     hi = mid - 1;
             }
             const uint32_t id = sorted_ids[lo];
-            uint32_t start, shape;
-            owner_rect(id, rects, tiles, xy, grid_x, grid_y, tile_w, tile_h, views, n_per_view, &start, &shape);
+            const uint2 r = owner_rects[lo];
+            const uint32_t start = r.x, shape = r.y;
             const uint32_t local = p - offsets[lo];
             const uint32_t w = shape & 0xffffu;
             const uint32_t ty = local / w, tx = local - ty * w;
@@ -396,6 +418,8 @@ struct SplatWorkspace {
     uint32_t* dvals_b;
     uint32_t* counts;
     uint32_t* emit_bounds;
+    uint2* owner_rects;
+    uint2* bin_rects;
     uint32_t* pair_offsets;
     uint32_t* pkeys_a;
     uint32_t* pvals_a;
@@ -411,6 +435,7 @@ static SplatWorkspace carve_splat(Carver& cv, int64_t n, int64_t cap)
     const int64_t nn = std::max<int64_t>(n, 1), cc = std::max<int64_t>(cap, 1);
     w.dkeys_a = cv.take<uint32_t>(nn);
     w.dvals_a = cv.take<uint32_t>(nn);
+    w.bin_rects = cv.take<uint2>(nn);
     w.dkeys_b = cv.take<uint32_t>(nn);
     w.dvals_b = cv.take<uint32_t>(nn);
     w.counts = cv.take<uint32_t>(nn);
@@ -420,6 +445,7 @@ static SplatWorkspace carve_splat(Carver& cv, int64_t n, int64_t cap)
     w.pkeys_b = cv.take<uint32_t>(cc);
     w.counter = cv.take<uint32_t>(64);
     w.emit_bounds = cv.take<uint32_t>(cc / kEmitPairs + 2);
+    w.owner_rects = cv.take<uint2>(nn);
     w.sort_scratch = cv.take<char>(radix_scratch_bytes(std::max(nn, cc)));
     w.scan_scratch = cv.take<char>(scan_scratch_bytes(nn));
     return w;
@@ -489,8 +515,8 @@ void splat_blend_backward(const dgmr_tiling& t, int n_tiles, hipStream_t st, con
 // binning shared by the splatter and the marcher: depth sort, scan, emission,
 // pair sort, ranges.  depth_keys/ids are the workspace's A buffers, filled by
 // the preprocess.  Returns the pair count through *n_pairs (host).
-int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, int grid_x, int grid_y, int tile_w,
-    int tile_h, dgmr_binning* bins, Carver& cv, PendingCount* pending, hipStream_t st, const BatchBinning* batch)
+int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, bool bin_rects_written, int grid_x,
+    int grid_y, int tile_w, int tile_h, dgmr_binning* bins, Carver& cv, PendingCount* pending, hipStream_t st, const BatchBinning* batch)
 {
     const int4* views = batch ? batch->views : nullptr;
     const int64_t n_per_view = batch ? batch->n_per_view : 0;
@@ -514,8 +540,16 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     }
     const unsigned blocks_n = unsigned(ceil_div64(n, 256));
     {
-        StageScope sc("bin.scan", st, 24.0 * double(n));
-        k_gather_counts<<<blocks_n, 256, 0, st>>>(n, sorted_ids, tiles, w.counts);
+        StageScope sc("bin.scan", st, 44.0 * double(n));
+        if (bin_rects_written)
+            k_gather_owners<float, true><<<blocks_n, 256, 0, st>>>(n, sorted_ids, w.bin_rects, rects, tiles, static_cast<const float*>(xy), grid_x,
+                grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
+        else if (xy_double)
+            k_gather_owners<double, false><<<blocks_n, 256, 0, st>>>(n, sorted_ids, w.bin_rects, rects, tiles, static_cast<const double*>(xy),
+                grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
+        else
+            k_gather_owners<float, false><<<blocks_n, 256, 0, st>>>(n, sorted_ids, w.bin_rects, rects, tiles, static_cast<const float*>(xy),
+                grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
         DGMR_HIP_CHECK(hipGetLastError());
         DGMR_HIP_CHECK(scan_u32(w.counts, w.pair_offsets, w.counter, n, false, w.scan_scratch, st));
         // the reference's cache keeps the inclusive scan in Gaussian order
@@ -546,12 +580,8 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
         pending->add_stage(sc.slot, 8.0, 28.0 * double(n));
         k_emit_bounds<<<blocks_n, 256, 0, st>>>(n, w.pair_offsets, w.counter, uint32_t(cap), w.emit_bounds);
         DGMR_HIP_CHECK(hipGetLastError());
-        if (xy_double)
-            k_emit_pairs<double><<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.emit_bounds, w.pair_offsets, rects, tiles,
-                static_cast<const double*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
-        else
-            k_emit_pairs<float><<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.emit_bounds, w.pair_offsets, rects, tiles,
-                static_cast<const float*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, pb.keys_a, pb.vals_a);
+        k_emit_pairs<<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.emit_bounds, w.pair_offsets,
+            w.owner_rects, pb.keys_a, pb.vals_a);
         DGMR_HIP_CHECK(hipGetLastError());
     }
     {
@@ -574,12 +604,16 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
 size_t binning_workspace_bytes(int64_t n, int64_t cap) { return splat_workspace_bytes(n, cap); }
 
 // depth keys / ids buffers the preprocess writes (the first two slices of the workspace)
-void binning_key_buffers(void* workspace, int64_t n, uint32_t** keys, uint32_t** ids)
+void binning_key_buffers(void* workspace, int64_t n, uint32_t** keys, uint32_t** ids, uint2** bin_rects)
 {
+    // the first three slices of carve_splat
     Carver cv(workspace, ~size_t(0));
     const int64_t nn = std::max<int64_t>(n, 1);
     *keys = cv.take<uint32_t>(nn);
     *ids = cv.take<uint32_t>(nn);
+    uint2* r = cv.take<uint2>(nn);
+    if (bin_rects)
+        *bin_rects = r;
 }
 
 } // namespace dgmr
@@ -624,17 +658,18 @@ int dgmr_splat_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dg
         return DGMR_OK;
     }
     uint32_t *dkeys, *dids;
-    binning_key_buffers(workspace, n, &dkeys, &dids);
+    uint2* brects;
+    binning_key_buffers(workspace, n, &dkeys, &dids, &brects);
     {
-    StageScope sc("splat.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs + 67.0));
+    StageScope sc("splat.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs + 75.0));
     k_splat_preprocess<<<unsigned(ceil_div64(n, kPreThreads)), kPreThreads, 0, st>>>(n, make_params(g), dc,
         SplatOut { geom->rects, geom->tiles_touched, geom->xy, reinterpret_cast<float4*>(geom->conic_opacity), geom->rgb, geom->rgb_clamped,
-            geom->depths, dkeys, dids });
+            geom->depths, dkeys, dids, brects });
     DGMR_HIP_CHECK(hipGetLastError());
     }
     Carver cv(workspace, workspace_bytes);
     PendingCount pending;
-    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, false, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv,
+    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, false, true, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv,
         &pending, st, nullptr);
     if (rc != DGMR_OK)
         return rc;

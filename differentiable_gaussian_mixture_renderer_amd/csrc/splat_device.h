@@ -82,6 +82,7 @@ struct SplatOut {
     float* depths;
     uint32_t* depth_keys;
     uint32_t* ids;
+    uint2* bin_rects; // (first tile, rows << 16 | columns) for the binning
 };
 
 // SH coefficients of a block of Gaussians staged in LDS: the block's
@@ -121,6 +122,7 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     out.xy[2 * o + 1] = 0.f;
     out.depth_keys[o] = 0xffffffffu;
     out.ids[o] = uint32_t(o);
+    out.bin_rects[o] = make_uint2(0u, 0u);
     const float mu[3] = { p.means[3 * i], p.means[3 * i + 1], p.means[3 * i + 2] };
     float t[3];
     transform_point(cam.view, mu, t, 3);
@@ -143,6 +145,7 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     if (touched == 0)
         return;
     out.tiles[o] = touched;
+    out.bin_rects[o] = make_uint2(uint32_t(y0) * uint32_t(cam.grid_x) + uint32_t(x0), (uint32_t(y1 - y0) << 16) | uint32_t(x1 - x0));
     out.xy[2 * o] = xy[0];
     out.xy[2 * o + 1] = xy[1];
     const float dv[3] = { cam.pos[0] - mu[0], cam.pos[1] - mu[1], cam.pos[2] - mu[2] };
