@@ -75,6 +75,15 @@ def lib() -> ctypes.CDLL:
         L.orc_sample_gaussian_grad.argtypes = [f64] + [vp] * 11
         L.orc_integrate_bins.argtypes = [vp, vp, vp]
         L.orc_integrate_bins_grad.argtypes = [vp, vp, f64, vp, vp, f64, vp]
+        f32 = ctypes.c_float
+        L.orc_raster_render.argtypes = [i64, vp, vp, vp, vp, vp, i32, f64, vp, vp, vp, f64, f64, i32, i32, vp, i32, i32, f32, i32, i32, i32,
+                                        i32, vp, vp]
+        L.orc_binsizer_run.restype = i32
+        L.orc_binsizer_run.argtypes = [i32, f32, f32, i32, vp, vp, vp, vp, vp]
+        L.orc_pl_create.argtypes = [vp, vp, vp, vp, f64, vp]
+        L.orc_pl_sample.argtypes = [vp, f64, vp]
+        L.orc_pl_sample_many.argtypes = [vp, i64, vp, vp]
+        L.orc_pl_volume_integrate.argtypes = [i32, vp, i32, vp]
         _lib = L
     return _lib
 
@@ -232,3 +241,69 @@ def scene_arrays(scene):
     sh16 = np.zeros((n, 16, 3), scene.sh.dtype)
     sh16[:, :k] = scene.sh
     return scene.means, scene.scales, scene.rotations, scene.opacity, sh16, scene.sh_degree
+
+
+RASTER_FULL, RASTER_BINS, RASTER_DEPTH = 0, 1, 2
+
+
+def raster_render(scene_arrays, cam, bg, mode=RASTER_FULL, debug_bin=-1, max_depth=20.0, crop=None, scale_multiplier=1.0):
+    """Float restatement of vol_raster_forward (vol_raster_forward.cu:36-520):
+    planar [3, H, W] image (pixels outside crop = (x0, y0, x1, y1) stay 0) and
+    the pair count of the binning."""
+    means, scales, rots, opac, sh16, deg = scene_arrays
+    f = np.float32
+    keep = [_c(means, f), _c(scales, f), _c(rots, f), _c(np.reshape(opac, -1), f), _c(sh16, f), _c(cam.world_view_transform, f),
+            _c(cam.full_proj_transform, f), _c(cam.camera_center, f), _c(bg, f)]
+    W, H = cam.width, cam.height
+    x0, y0, x1, y1 = crop if crop is not None else (0, 0, W, H)
+    fb = np.zeros((3, H, W), np.float32)
+    n_pairs = ctypes.c_int64(0)
+    lib().orc_raster_render(means.shape[0], P(keep[0]), P(keep[1]), P(keep[2]), P(keep[3]), P(keep[4]), deg, scale_multiplier,
+                            P(keep[5]), P(keep[6]), P(keep[7]), float(np.float32(cam.tan_fovx)), float(np.float32(cam.tan_fovy)), W, H,
+                            P(keep[8]), mode, debug_bin, max_depth, x0, y0, x1, y1, P(fb), ctypes.byref(n_pairs))
+    return fb, int(n_pairs.value)
+
+
+def binsizer_run(n_bins, transmission_threshold, relevance_sigma, adds):
+    """math::RasterBinSizer in float: is_full() after each add, then finalise();
+    returns (full_after, begin, end, full_final)."""
+    a = np.ascontiguousarray(np.asarray(adds, np.float32).reshape(-1, 3))
+    full_after = np.zeros(len(a), np.uint8)
+    begin = np.zeros(n_bins, np.float32)
+    end = np.zeros(n_bins, np.float32)
+    full_final = np.zeros(1, np.uint8)
+    rc = lib().orc_binsizer_run(n_bins, transmission_threshold, relevance_sigma, len(a), P(a), P(full_after), P(begin), P(end),
+                                P(full_final))
+    if rc != 0:
+        raise ValueError(f"unsupported bin count {n_bins}")
+    return full_after.astype(bool), begin, end, bool(full_final[0])
+
+
+def pl_create(percent_left, total, eval_left, eval_right, t_right):
+    """piecewise_linear::create_approximation (double): 21 values d0, k0, t1, t2, k2 (x4), t_right"""
+    args = [np.ascontiguousarray(v, np.float64) for v in (percent_left, total, eval_left, eval_right)]
+    out = np.zeros(21, np.float64)
+    lib().orc_pl_create(P(args[0]), P(args[1]), P(args[2]), P(args[3]), float(t_right), P(out))
+    return out
+
+
+def pl_sample(params, t):
+    p = np.ascontiguousarray(params, np.float64)
+    out = np.zeros(4, np.float64)
+    lib().orc_pl_sample(P(p), float(t), P(out))
+    return out
+
+
+def pl_sample_many(params, ts):
+    p = np.ascontiguousarray(params, np.float64)
+    t = np.ascontiguousarray(ts, np.float64)
+    out = np.zeros((len(t), 4), np.float64)
+    lib().orc_pl_sample_many(P(p), len(t), P(t), P(out))
+    return out
+
+
+def pl_volume_integrate(groups, n_steps):
+    p = np.ascontiguousarray(np.stack(groups), np.float64)
+    out = np.zeros(3, np.float64)
+    lib().orc_pl_volume_integrate(len(groups), P(p), n_steps, P(out))
+    return out

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "oracle_march.hpp"
+#include "oracle_raster.hpp"
 #include "oracle_splat.hpp"
 
 using namespace orc;
@@ -685,6 +686,125 @@ double orc_time_splat_fwd_bwd(void* p, const float* bg, const float* grad_fb, fl
     omp_set_num_threads(saved);
 #endif
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// ----------------------------------------------------------------- rasteriser
+// Float restatement of vol_raster_forward (16x16 tiles) for the pixels
+// [x0, x1) x [y0, y1) of the planar [3, H, W] framebuffer fb (float inputs).
+// n_pairs (may be null) receives the binning's pair count.
+void orc_raster_render(int64_t n, const float* means, const float* scales, const float* rotations, const float* opacity, const float* sh16,
+    int sh_degree, double scale_multiplier, const float* view, const float* proj, const float* cam_pos, double tan_fovx, double tan_fovy,
+    int width, int height, const float* bg, int mode, int debug_bin, float max_depth, int x0, int y0, int x1, int y1, float* fb,
+    int64_t* n_pairs)
+{
+    SceneStore<float> sf;
+    sf.load(n, means, scales, rotations, opacity, sh16, sh_degree, scale_multiplier, view, proj, cam_pos, tan_fovx, tan_fovy, width, height,
+        false);
+    SplatGeometry<float> g;
+    splat_preprocess(sf.scene, sf.cam, 16, 16, &g);
+    Binning b;
+    bin(g.rect, g.tiles, g.xy, g.key_depth, width, height, 16, 16, &b);
+    if (n_pairs)
+        *n_pairs = int64_t(b.point_list.size());
+    RasterExtra<float> x;
+    raster_preprocess_extra(sf.scene, sf.cam, g, &x);
+    double pd[16], ipd[16];
+    for (int i = 0; i < 16; ++i)
+        pd[i] = sf.cam.proj[i];
+    invert4(pd, ipd);
+    float ip[16];
+    for (int i = 0; i < 16; ++i)
+        ip[i] = float(ipd[i]);
+    crop_guard<float>(width, height, &x0, &y0, &x1, &y1);
+    const size_t plane = size_t(width) * height;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int py = y0; py < y1; ++py)
+        for (int px = x0; px < x1; ++px) {
+            float out[3];
+            raster_pixel(sf.scene, g, x, b, sf.cam, ip, px, py, bg, mode, debug_bin, max_depth, out);
+            for (int c = 0; c < 3; ++c)
+                fb[c * plane + size_t(py) * width + px] = out[c];
+        }
+}
+
+// RasterBinSizer with n_bins = nb (4 or 32) in float: the adds (mass, centre,
+// sd) in order, full_after[k] = is_full() after add k, then finalise();
+// begin/end of every bin and is_full() afterwards.  Returns 0, or -1 for an
+// unsupported nb.
+int orc_binsizer_run(int nb, float transmission_threshold, float relevance_sigma, int n_adds, const float* adds, uint8_t* full_after,
+    float* begin, float* end, uint8_t* full_final)
+{
+    const auto go = [&](auto& z) {
+        for (int k = 0; k < n_adds; ++k) {
+            z.add_gaussian(adds[3 * k], adds[3 * k + 1], adds[3 * k + 2]);
+            full_after[k] = z.is_full();
+        }
+        z.finalise();
+        for (int i = 0; i < nb; ++i) {
+            begin[i] = z.begin_of(i);
+            end[i] = z.end_of(i);
+        }
+        *full_final = z.is_full();
+    };
+    if (nb == 4) {
+        BinSizer<float, 4> z(transmission_threshold, relevance_sigma);
+        go(z);
+    } else if (nb == 32) {
+        BinSizer<float, 32> z(transmission_threshold, relevance_sigma);
+        go(z);
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+// piecewise_linear in double: create_approximation -> 21 values (d0, k0, t1,
+// t2, k2 per channel, then t_right); sample; volume integration over a chain
+// of groups with n_steps per group.
+void orc_pl_create(const double* percent_left, const double* total, const double* eval_left, const double* eval_right, double t_right,
+    double* out)
+{
+    const PiecewiseLinear<double> f = create_approximation(percent_left, total, eval_left, eval_right, t_right);
+    for (int i = 0; i < 4; ++i) {
+        out[i] = f.d0[i];
+        out[4 + i] = f.k0[i];
+        out[8 + i] = f.t1[i];
+        out[12 + i] = f.t2[i];
+        out[16 + i] = f.k2[i];
+    }
+    out[20] = f.t_right;
+}
+
+static PiecewiseLinear<double> pl_from(const double* in)
+{
+    PiecewiseLinear<double> f;
+    for (int i = 0; i < 4; ++i) {
+        f.d0[i] = in[i];
+        f.k0[i] = in[4 + i];
+        f.t1[i] = in[8 + i];
+        f.t2[i] = in[12 + i];
+        f.k2[i] = in[16 + i];
+    }
+    f.t_right = in[20];
+    return f;
+}
+
+void orc_pl_sample(const double* params, double t, double* out) { pl_from(params).sample(t, out); }
+
+void orc_pl_sample_many(const double* params, int64_t n, const double* ts, double* out)
+{
+    const PiecewiseLinear<double> f = pl_from(params);
+    for (int64_t i = 0; i < n; ++i)
+        f.sample(ts[i], out + 4 * i);
+}
+
+void orc_pl_volume_integrate(int n_groups, const double* params, int n_steps, double* out_rgb)
+{
+    double result[3] = { 0, 0, 0 }, transparency = 1;
+    for (int k = 0; k < n_groups; ++k)
+        volume_integrate_step(pl_from(params + 21 * k), n_steps, result, &transparency);
+    for (int c = 0; c < 3; ++c)
+        out_rgb[c] = result[c];
 }
 
 } // extern "C"
