@@ -129,6 +129,10 @@ class StageTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char_p), ("total_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("bytes", ctypes.c_double)]
 
 
+class RasterOptions(ctypes.Structure):
+    _fields_ = [("render_mode", ctypes.c_int32), ("debug_bin", ctypes.c_int32), ("max_depth", ctypes.c_float)]
+
+
 _P = ctypes.POINTER
 # exported symbol -> (restype, argtypes); this list mirrors include/dgmr.h
 SIGNATURES = {
@@ -156,6 +160,8 @@ SIGNATURES = {
     "dgmr_version": (ctypes.c_char_p, []),
     "dgmr_adam_step": (ctypes.c_int, [_P(AdamTensor), ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int64, c_void_p]),
+    "dgmr_raster_forward": (ctypes.c_int, [_P(Gaussians), _P(Camera), _P(RasterOptions), _P(SplatGeometry), _P(Binning), c_void_p,
+                                           c_void_p, c_void_p, c_void_p, ctypes.c_size_t, _P(Stats), c_void_p]),
     "dgmr_sort_workspace_size": (ctypes.c_int, [ctypes.c_int64, _P(ctypes.c_size_t)]),
     "dgmr_sort_pairs": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, c_void_p,
                                        ctypes.c_size_t, c_void_p]),

@@ -393,6 +393,14 @@ SplatParams make_params(const dgmr_gaussians* g)
 
 } // namespace
 
+SplatParams make_splat_params(const dgmr_gaussians* g) { return make_params(g); }
+
+hipError_t launch_splat_preprocess(const dgmr_gaussians* g, const DevCamera<float>& cam, const SplatOut& out, hipStream_t stream)
+{
+    k_splat_preprocess<<<unsigned(ceil_div64(g->n, kPreThreads)), kPreThreads, 0, stream>>>(g->n, make_params(g), cam, out);
+    return hipGetLastError();
+}
+
 namespace {
 // one wave: out[l] = wave_sum8 of in[q * 64 + l] (q < 8), out[64 + l] = wave_sum_to_last of in[512 + l]
 __global__ __launch_bounds__(64) void k_selftest_wave_sum(const float* __restrict__ in, float* __restrict__ out)

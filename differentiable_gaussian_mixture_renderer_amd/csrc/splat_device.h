@@ -85,6 +85,10 @@ struct SplatOut {
     uint2* bin_rects; // (first tile, rows << 16 | columns) for the binning
 };
 
+// host side of the splatter preprocess, shared with the rasteriser
+SplatParams make_splat_params(const dgmr_gaussians* g);
+hipError_t launch_splat_preprocess(const dgmr_gaussians* g, const DevCamera<float>& cam, const SplatOut& out, hipStream_t stream);
+
 // SH coefficients of a block of Gaussians staged in LDS: the block's
 // contiguous [n_block, K, 3] slice is read with coalesced loads and each
 // Gaussian's row sits at a stride of 3K + 1 floats (odd, so the 64 lanes of a

@@ -231,6 +231,26 @@ typedef struct dgmr_stage_time {
 int dgmr_profile_enable(int on);
 int dgmr_profile_read(dgmr_stage_time* out, int32_t max_stages, int32_t* n_stages);
 
+/* ------------------------------------------------------------ rasteriser
+ * dgmr::vol_raster_forward(VolRasterForwardData&) (vol_raster_forward.h:68,
+ * vol_raster_forward.cu:36-520): forward only, as the reference.  16x16
+ * tiles; the binning (geom, bins, workspace) is the splatter's, sized with
+ * dgmr_workspace_size and a 16x16 tiling.  inv_cov [N,6] and mass [N]
+ * (device) receive the inverse filtered covariance and the filtered mass of
+ * each visible Gaussian.  render_mode: 0 full, 1 bins (background only, as
+ * the reference's unfinished mode), 2 depth (end of bin debug_bin / max_depth).
+ * With no Gaussian binned the framebuffer is left untouched, as the reference
+ * returns before rendering. */
+typedef struct {
+    int32_t render_mode;
+    int32_t debug_bin;  /* -1 selects the last bin */
+    float max_depth;    /* depth mode scale (the reference's default 20) */
+} dgmr_raster_options;
+
+int dgmr_raster_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dgmr_raster_options* opt, dgmr_splat_geometry* geom,
+    dgmr_binning* bins, float* inv_cov, float* mass, float* framebuffer, void* workspace, size_t workspace_bytes, dgmr_stats* stats,
+    dgmr_stream stream);
+
 /* ------------------------------------------------------------ sort
  * Stable LSD radix sort of n (uint32 key, uint32 value) pairs on key bits
  * [begin_bit, end_bit), the primitive both binnings use (replaces

@@ -248,8 +248,9 @@ RASTER_FULL, RASTER_BINS, RASTER_DEPTH = 0, 1, 2
 
 def raster_render(scene_arrays, cam, bg, mode=RASTER_FULL, debug_bin=-1, max_depth=20.0, crop=None, scale_multiplier=1.0):
     """Float restatement of vol_raster_forward (vol_raster_forward.cu:36-520):
-    planar [3, H, W] image (pixels outside crop = (x0, y0, x1, y1) stay 0) and
-    the pair count of the binning."""
+    planar [3, H, W] image (pixels outside crop = (x0, y0, x1, y1) stay 0, and
+    all stay 0 when nothing is binned, as the reference returns before
+    rendering) and the pair count of the binning."""
     means, scales, rots, opac, sh16, deg = scene_arrays
     f = np.float32
     keep = [_c(means, f), _c(scales, f), _c(rots, f), _c(np.reshape(opac, -1), f), _c(sh16, f), _c(cam.world_view_transform, f),

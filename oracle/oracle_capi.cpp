@@ -706,6 +706,9 @@ void orc_raster_render(int64_t n, const float* means, const float* scales, const
     bin(g.rect, g.tiles, g.xy, g.key_depth, width, height, 16, 16, &b);
     if (n_pairs)
         *n_pairs = int64_t(b.point_list.size());
+    // nothing binned: the reference returns before rendering (vol_raster_forward.cu:153-155)
+    if (b.point_list.empty())
+        return;
     RasterExtra<float> x;
     raster_preprocess_extra(sf.scene, sf.cam, g, &x);
     double pd[16], ipd[16];
