@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-camera batched workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-marcher", action="store_true", help="skip the volumetric marcher workload")
+    ap.add_argument("--no-raster", action="store_true", help="skip the volumetric rasteriser workload")
     ap.add_argument("--batched-cams", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -176,6 +177,7 @@ def main():
     batched = None if args.no_batched else batched_workload(args.n, args.batched_cams)
     served = None if args.no_serving else serving_workload(args.serving_requests)
     marcher = None if args.no_marcher else marcher_workload(lib, with_cpu=not args.no_cpu)
+    raster = None if args.no_raster else raster_workload(lib, with_cpu=not args.no_cpu)
 
     line = {
         "metric": METRIC,
@@ -203,6 +205,7 @@ def main():
         "batched": batched,
         "serving": served,
         "marcher": marcher,
+        "raster": raster,
     }
     print(json.dumps(line))
 
@@ -357,6 +360,52 @@ def marcher_workload(lib, with_cpu: bool, steps: int = 3, max_large_steps: int =
         out["cpu_baseline"] = {"value": round(W * rows / secs / 1e6, 5), "unit": "Mrays/s", "kind": "port",
                                "sample": f"float restatement: preprocess+binning of all Gaussians, forward of {W}x{rows} rays on "
                                          f"{threads} OpenMP threads, backward of the same rays on 1 thread, {secs:.1f} s"}
+    return out
+
+
+def raster_workload(lib, with_cpu: bool, steps: int = 3):
+    """Volumetric rasteriser (forward only, as the reference): 200k Gaussians,
+    1024x1024 pixels, 32 bins per pixel.  The host baseline is the float
+    restatement on a band of rows."""
+    import numpy as np
+    import torch
+
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    from differentiable_gaussian_mixture_renderer_amd.python_bindings.vol_raster import vol_raster_forward
+
+    scene, cam = scenes.marcher_frame()
+    W, H = cam.width, cam.height
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float32, device=dev)
+    args = (t(scene.sh), t(np.reshape(scene.opacity, (-1, 1))), t(scene.means), t(scene.scales), t(scene.rotations),
+            t(cam.world_view_transform), t(cam.full_proj_transform), t(cam.camera_center), t(np.zeros(3)), W, H, cam.tan_fovx,
+            cam.tan_fovy, scene.sh_degree)
+    vol_raster_forward(*args)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        vol_raster_forward(*args)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    lib.dgmr_profile_enable(1)
+    vol_raster_forward(*args)
+    torch.cuda.synchronize()
+    lib.dgmr_profile_enable(0)
+    stages = {k: round(v[0] / max(v[1], 1), 3) for k, v in stage_times(lib).items() if k.startswith(("raster", "bin."))}
+    out = {"workload": f"volumetric rasteriser forward, {scene.n} Gaussians SH3, {W}x{H}, 32 bins per pixel",
+           "ms_per_frame": round(ms, 3), "mpixel_per_s": round(W * H / ms / 1e3, 2), "stages_ms": stages}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        rows = 32
+        y0 = (H - rows) // 2
+        t0 = time.perf_counter()
+        O.raster_render(O.scene_arrays(scene), cam, np.zeros(3, np.float32), crop=(0, y0, W, y0 + rows))
+        secs = time.perf_counter() - t0
+        threads = os.environ.get("OMP_NUM_THREADS", str(os.cpu_count()))
+        out["cpu_baseline"] = {"value": round(W * rows / secs / 1e6, 5), "unit": "Mpixel/s", "kind": "port",
+                               "sample": f"float restatement: preprocess+binning of all Gaussians and {W}x{rows} pixels on "
+                                         f"{threads} OpenMP threads, {secs:.1f} s"}
     return out
 
 
