@@ -316,15 +316,20 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_blend_backward(const uint3
     int n_tiles, float bg0, float bg1, float bg2, const float* __restrict__ framebuffer, const float* __restrict__ final_T,
     const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
 {
+    // kBwdSplit waves per tile, each on TH / kBwdSplit rows: the tile's list
+    // is walked once per part, and the longest tile's work is spread over
+    // kBwdSplit waves with less per-lane state
+    constexpr int SPLIT = (TW * TH / kBwdSplit >= 64) ? kBwdSplit : 1;
     __shared__ BlendSmemBwd<B> sm_all[kBlendWaves];
     const int wave = int(uniform(threadIdx.x / 64));
-    const int tile = xcd_tile(blockIdx.x, gridDim.x) * kBlendWaves + wave;
+    const int unit = xcd_tile(blockIdx.x, gridDim.x) * kBlendWaves + wave;
+    const int tile = unit / SPLIT, part = unit % SPLIT;
     if (tile >= n_tiles)
         return;
     BlendSmemBwd<B>& sm = sm_all[wave];
     const uint32_t first = uniform(ranges[2 * tile]), last = uniform(ranges[2 * tile + 1]);
-    blend_tile_backward<TW, TH, B>(sm, first, last, (tile % grid_x) * TW, (tile / grid_x) * TH, width, height, bg0, bg1,
-        bg2, point_list, xy, conic_opacity, rgb, framebuffer, final_T, grad_fb, screen_grads);
+    blend_tile_backward<TW, TH / SPLIT, B>(sm, first, last, (tile % grid_x) * TW, (tile / grid_x) * TH + part * (TH / SPLIT), width, height,
+        bg0, bg1, bg2, point_list, xy, conic_opacity, rgb, framebuffer, final_T, grad_fb, screen_grads);
 }
 
 namespace {
@@ -484,7 +489,8 @@ template <int TW, int TH>
 static void launch_blend_bwd(int batch, int n_tiles, hipStream_t st, const uint32_t* ranges, const uint32_t* pl, const float2* xy,
     const float4* co, const float* rgb, int W, int H, int gx, const double* bg, const float* fb, const float* T, const float* gfb, float* sg)
 {
-    const dim3 grid(unsigned(ceil_div64(n_tiles, kBlendWaves))), block(64 * kBlendWaves);
+    constexpr int SPLIT = (TW * TH / kBwdSplit >= 64) ? kBwdSplit : 1;
+    const dim3 grid(unsigned(ceil_div64(int64_t(n_tiles) * SPLIT, kBlendWaves))), block(64 * kBlendWaves);
     const float b0 = float(bg[0]), b1 = float(bg[1]), b2 = float(bg[2]);
     if (batch == 64)
         k_blend_backward<TW, TH, 64><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);

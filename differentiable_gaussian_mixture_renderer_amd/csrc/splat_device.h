@@ -362,12 +362,19 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
     }
 }
 
+// waves per tile in the plain blend backward
+#ifndef DGMR_BWD_SPLIT
+#define DGMR_BWD_SPLIT 1
+#endif
+constexpr int kBwdSplit = DGMR_BWD_SPLIT;
+
 template <int B>
 struct BlendSmemBwd {
     float4 geo[B];   // x, y, A, B
     float4 col[B];   // C, opacity, r, g
     float blue[B];
     float acc[9][B]; // per-batch gradient sums of the tile: xy 2, conic 3, opacity 1, rgb 3
+    uint32_t gid[B]; // the batch's Gaussian ids, for the flush
 };
 
 // Blend adjoint of one tile, front to back.  The remaining colour is carried
@@ -427,21 +434,19 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
     const int slot8 = (lane & 7) == 0 ? kSum8Order[lane >> 3] : (lane == 63 ? 8 : -1);
     // a batch's sums leave for global memory at the top of the next batch,
     // after its data has arrived, so no wait on the loads covers the atomics
-    uint32_t ids[NS];
     int cnt = 0;
+    // The batch's sums leave as float atomics in Gaussian-major order: lane l
+    // of flush step t adds value q = f % 9 of Gaussian f / 9, f = 64 t + l, so
+    // one instruction touches about seven Gaussians' 36-byte rows and lanes
+    // on the same cache line travel together (value-major order would send
+    // every lane of an instruction to a different line).
     const auto flush = [&]() {
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int i = lane + 64 * s;
-            if (i < cnt) {
-                float* dst = screen_grads + int64_t(ids[s]) * 9;
-#pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    const float v = sm.acc[q][i];
-                    if (v != 0.f)
-                        unsafeAtomicAdd(dst + q, v);
-                }
-            }
+        const int total = cnt * 9;
+        for (int f = lane; f < total; f += 64) {
+            const int i = f / 9, q = f - 9 * (f / 9);
+            const float v = sm.acc[q][i];
+            if (v != 0.f)
+                unsafeAtomicAdd(screen_grads + int64_t(sm.gid[i]) * 9 + q, v);
         }
     };
     for (uint32_t base = first; base < last; base += B) {
@@ -453,10 +458,11 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             break;
         wave_lds_sync(); // the previous batch has been read
         flush();
+        wave_lds_sync(); // the flush has read the previous batch's ids and sums
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int i = lane + 64 * s;
-            ids[s] = f.id[s];
+            sm.gid[i] = f.id[s];
             f.stage(sm.geo, sm.col, sm.blue, s);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
