@@ -42,10 +42,11 @@ def test_wave_sum8_lane_layout(seed):
 
 
 @pytest.mark.parametrize("n,begin,end", [(0, 0, 13), (1, 0, 13), (4095, 0, 13), (70001, 0, 13), (70001, 0, 32), (12345, 3, 20),
-                                         (50000, 0, 6), (100003, 0, 17)])
+                                         (50000, 0, 6), (100003, 0, 17), ((48 << 20) + 5, 0, 13)])
 def test_sort_pairs_matches_stable_argsort(n, begin, end):
     """Bit-exact against numpy's stable sort on the key bits [begin, end); keys
-    carry random bits outside that range, which must not take part."""
+    carry random bits outside that range, which must not take part.  The last
+    case takes the large-key path (8 192 keys per block)."""
     if not gpu_available():
         pytest.skip("no GPU")
     import torch
