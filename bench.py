@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame in the host sample (0: the whole frame)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-camera batched workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--torch-loss", action="store_true", help="L1 loss as torch's composite (sub, abs, mean) instead of the fused kernel")
     ap.add_argument("--no-marcher", action="store_true", help="skip the volumetric marcher workload")
     ap.add_argument("--no-raster", action="store_true", help="skip the volumetric rasteriser workload")
     ap.add_argument("--batched-cams", type=int, default=64)
@@ -85,7 +86,7 @@ def main():
     import torch
 
     from differentiable_gaussian_mixture_renderer_amd import _native as N
-    from differentiable_gaussian_mixture_renderer_amd import scenes
+    from differentiable_gaussian_mixture_renderer_amd import losses, scenes
     from differentiable_gaussian_mixture_renderer_amd.python_bindings import splatter
 
     tile = tuple(int(v) for v in args.tile.split("x"))
@@ -118,7 +119,8 @@ def main():
 
     def step():
         img = render()
-        loss = (img - target).abs().mean()
+        # mean |img - target| and its gradient, one fused pass each way (torch's composite with --torch-loss)
+        loss = (img - target).abs().mean() if args.torch_loss else losses.l1_loss(img, target)
         return torch.autograd.grad(loss, params)
 
     if tile != (16, 16) or args.batch != 64:
@@ -194,7 +196,8 @@ def main():
         "data": "synthetic (seeded SplitMix64 scene: means U[-5,5]^3, log-scales N(-3,0.5), SH degree 3; uniform L1 target)",
         "config": {"workload": f"tile splat fwd+bwd, {args.n} Gaussians SH3, {W}x{H}, L1 loss", "tile": f"{tile[0]}x{tile[1]}",
                    "lds_batch": args.batch, "pairs": R, "visible": n_visible,
-                   "activations": "torch" if args.torch_activations else "fused"},
+                   "activations": "torch" if args.torch_activations else "fused",
+                   "l1_loss": "torch" if args.torch_loss else "fused"},
         "roofline": {"bound": "hbm", "kernel": d_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(d_avg_ms, 4), "algorithmic_bytes_per_launch": d_bytes / max(d_launches, 1)},
@@ -246,7 +249,7 @@ def batched_workload(n: int, n_cams: int, reps: int = 3):
     import numpy as np
     import torch
 
-    from differentiable_gaussian_mixture_renderer_amd import autotune, scenes
+    from differentiable_gaussian_mixture_renderer_amd import autotune, losses, scenes
     from differentiable_gaussian_mixture_renderer_amd.batching import BatchSplatter, render_plain
 
     scene, cams = scenes.multiview_batch(n=n, n_cams=n_cams)
@@ -255,7 +258,7 @@ def batched_workload(n: int, n_cams: int, reps: int = 3):
     params = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations)]
     gen = torch.Generator(device=dev).manual_seed(4)
     targets = [torch.rand((3, c.height, c.width), generator=gen, device=dev) for c in cams]
-    l1 = lambda img, v: torch.sign(img - targets[v]) / img.numel()
+    l1 = lambda img, v: losses.l1_grad(img, targets[v])  # sign(img - target) / n in one pass
     px = sum(c.width * c.height for c in cams)
 
     (tile, lds), table = autotune.autotune(params, cams, reps=1, use_cache=False)
@@ -309,7 +312,7 @@ def marcher_workload(lib, with_cpu: bool, steps: int = 3, max_large_steps: int =
     import numpy as np
     import torch
 
-    from differentiable_gaussian_mixture_renderer_amd import scenes
+    from differentiable_gaussian_mixture_renderer_amd import losses, scenes
     from differentiable_gaussian_mixture_renderer_amd.python_bindings import vol_marcher
 
     scene, cam = scenes.marcher_frame()
@@ -326,7 +329,7 @@ def marcher_workload(lib, with_cpu: bool, steps: int = 3, max_large_steps: int =
     try:
         def step():
             img = vol_marcher.VolMarcher.apply(*params, *cam_args)
-            return torch.autograd.grad((img - target).abs().mean(), params)
+            return torch.autograd.grad(losses.l1_loss(img, target), params)
 
         step()
         torch.cuda.synchronize()

@@ -162,6 +162,9 @@ SIGNATURES = {
                                       ctypes.c_double, ctypes.c_int64, c_void_p]),
     "dgmr_raster_forward": (ctypes.c_int, [_P(Gaussians), _P(Camera), _P(RasterOptions), _P(SplatGeometry), _P(Binning), c_void_p,
                                            c_void_p, c_void_p, c_void_p, ctypes.c_size_t, _P(Stats), c_void_p]),
+    "dgmr_l1_partials_size": (ctypes.c_int, [ctypes.c_int64, _P(ctypes.c_int32)]),
+    "dgmr_l1_loss_forward": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p, c_void_p]),
+    "dgmr_l1_loss_backward": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p, c_void_p, c_void_p]),
     "dgmr_sort_workspace_size": (ctypes.c_int, [ctypes.c_int64, _P(ctypes.c_size_t)]),
     "dgmr_sort_pairs": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, c_void_p,
                                        ctypes.c_size_t, c_void_p]),

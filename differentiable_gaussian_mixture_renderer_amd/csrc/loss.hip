@@ -1,0 +1,108 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Fused L1 photometric loss for the training step: the reference leaves the
+// loss to its caller (python_bindings/vol_marcher.py returns the image and
+// takes dL/dimage); a 3DGS step computes L = mean |image - target| and its
+// gradient sign(image - target) / n.  The forward reads both images once and
+// leaves one partial sum per workgroup (summed in fixed order by the caller,
+// so the loss is deterministic); the backward reads them once more and
+// writes the gradient scaled by dL/dloss, which it reads from device memory.
+#include "host_common.h"
+
+namespace dgmr {
+namespace {
+
+constexpr int kLossThreads = 256;
+constexpr int kLossBlocks = 1024;
+
+__global__ __launch_bounds__(kLossThreads) void k_l1_forward(const float* __restrict__ a, const float* __restrict__ b, int64_t n, float scale,
+    float* __restrict__ partials)
+{
+    __shared__ float s_wave[kLossThreads / 64];
+    float acc = 0.f;
+    const int64_t stride = int64_t(gridDim.x) * kLossThreads;
+    const int64_t n4 = n / 4;
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    for (int64_t i = int64_t(blockIdx.x) * kLossThreads + threadIdx.x; i < n4; i += stride) {
+        const float4 x = a4[i], y = b4[i];
+        acc += (fabsf(x.x - y.x) + fabsf(x.y - y.y)) + (fabsf(x.z - y.z) + fabsf(x.w - y.w));
+    }
+    for (int64_t i = 4 * n4 + int64_t(blockIdx.x) * kLossThreads + threadIdx.x; i < n; i += stride)
+        acc += fabsf(a[i] - b[i]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        acc += __shfl_xor(acc, d, 64);
+    if ((threadIdx.x & 63) == 0)
+        s_wave[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.f;
+#pragma unroll
+        for (int w = 0; w < kLossThreads / 64; ++w)
+            t += s_wave[w];
+        partials[blockIdx.x] = t * scale;
+    }
+}
+
+// torch.sign semantics: 0 for equal values, NaN propagates
+__device__ __forceinline__ float sign_of(float d) { return d > 0.f ? 1.f : (d < 0.f ? -1.f : d); }
+
+__global__ __launch_bounds__(kLossThreads) void k_l1_backward(const float* __restrict__ a, const float* __restrict__ b, int64_t n, float scale,
+    const float* __restrict__ grad_loss, float* __restrict__ grad)
+{
+    const float s = scale * grad_loss[0];
+    const int64_t stride = int64_t(gridDim.x) * kLossThreads;
+    const int64_t n4 = n / 4;
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    float4* g4 = reinterpret_cast<float4*>(grad);
+    for (int64_t i = int64_t(blockIdx.x) * kLossThreads + threadIdx.x; i < n4; i += stride) {
+        const float4 x = a4[i], y = b4[i];
+        g4[i] = make_float4(sign_of(x.x - y.x) * s, sign_of(x.y - y.y) * s, sign_of(x.z - y.z) * s, sign_of(x.w - y.w) * s);
+    }
+    for (int64_t i = 4 * n4 + int64_t(blockIdx.x) * kLossThreads + threadIdx.x; i < n; i += stride)
+        grad[i] = sign_of(a[i] - b[i]) * s;
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+unsigned loss_blocks(int64_t n) { return unsigned(std::min<int64_t>(kLossBlocks, std::max<int64_t>(1, ceil_div64(n, 4 * kLossThreads)))); }
+
+} // namespace
+} // namespace dgmr
+
+using namespace dgmr;
+
+extern "C" {
+
+int dgmr_l1_partials_size(int64_t n, int32_t* count)
+{
+    DGMR_ARG_CHECK(count && n >= 0, "dgmr_l1_partials_size: bad argument");
+    *count = int32_t(loss_blocks(n));
+    return DGMR_OK;
+}
+
+int dgmr_l1_loss_forward(const float* image, const float* target, int64_t n, float scale, float* partials, dgmr_stream stream)
+{
+    DGMR_ARG_CHECK(image && target && partials && n >= 0, "dgmr_l1_loss_forward: bad argument");
+    DGMR_ARG_CHECK(aligned16(image) && aligned16(target), "dgmr_l1_loss_forward: images must be 16-byte aligned");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    StageScope sc("loss.l1_forward", st, 8.0 * double(n));
+    k_l1_forward<<<loss_blocks(n), kLossThreads, 0, st>>>(image, target, n, scale, partials);
+    DGMR_HIP_CHECK(hipGetLastError());
+    return DGMR_OK;
+}
+
+int dgmr_l1_loss_backward(const float* image, const float* target, int64_t n, float scale, const float* grad_loss, float* grad,
+    dgmr_stream stream)
+{
+    DGMR_ARG_CHECK(image && target && grad_loss && grad && n >= 0, "dgmr_l1_loss_backward: bad argument");
+    DGMR_ARG_CHECK(aligned16(image) && aligned16(target) && aligned16(grad), "dgmr_l1_loss_backward: images must be 16-byte aligned");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    StageScope sc("loss.l1_backward", st, 12.0 * double(n));
+    k_l1_backward<<<loss_blocks(n), kLossThreads, 0, st>>>(image, target, n, scale, grad_loss, grad);
+    DGMR_HIP_CHECK(hipGetLastError());
+    return DGMR_OK;
+}
+
+} // extern "C"

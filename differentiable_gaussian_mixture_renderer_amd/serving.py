@@ -25,7 +25,7 @@ from typing import Dict, List, Optional
 import numpy as np
 import torch
 
-from . import scenes
+from . import losses, scenes
 from .batching import BatchSplatter
 
 # bytes of device memory per (view, Gaussian) slot of the batched path and per
@@ -105,7 +105,7 @@ def l1_grad(img: torch.Tensor, seed: int) -> torch.Tensor:
     g = torch.Generator(device=img.device)
     g.manual_seed(seed)
     target = torch.rand(img.shape, generator=g, device=img.device, dtype=img.dtype)
-    return torch.sign(img - target) / img.numel()
+    return losses.l1_grad(img, target)
 
 
 def estimate_bytes(scene: ResidentScene, reqs: List[Request]) -> int:

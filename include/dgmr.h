@@ -251,6 +251,19 @@ int dgmr_raster_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const d
     dgmr_binning* bins, float* inv_cov, float* mass, float* framebuffer, void* workspace, size_t workspace_bytes, dgmr_stats* stats,
     dgmr_stream stream);
 
+/* ------------------------------------------------------------ L1 loss
+ * No counterpart in the reference, which leaves the loss to its caller: the
+ * photometric L1 loss of a training step, L = scale * sum |image - target|
+ * (scale = 1/n for the mean), fused into one pass each way.  The forward
+ * writes dgmr_l1_partials_size(n) scaled partial sums to partials (device),
+ * whose sum is L; the backward writes grad = sign(image - target) *
+ * scale * grad_loss[0] (grad_loss on the device).  All arrays float32,
+ * 16-byte aligned. */
+int dgmr_l1_partials_size(int64_t n, int32_t* count);
+int dgmr_l1_loss_forward(const float* image, const float* target, int64_t n, float scale, float* partials, dgmr_stream stream);
+int dgmr_l1_loss_backward(const float* image, const float* target, int64_t n, float scale, const float* grad_loss, float* grad,
+    dgmr_stream stream);
+
 /* ------------------------------------------------------------ sort
  * Stable LSD radix sort of n (uint32 key, uint32 value) pairs on key bits
  * [begin_bit, end_bit), the primitive both binnings use (replaces
