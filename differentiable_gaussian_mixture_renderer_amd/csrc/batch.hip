@@ -106,9 +106,42 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
     float* row = s_sh + threadIdx.x * kShStride;
     const int kc = 3 * p.sh_coeffs;
     float g_mu[3], g_s[3], g_q[4], g_w, g_shv[48];
+    // the Gaussian's own values are the same for every view: read once
+    float mu[3] = { 0.f, 0.f, 0.f }, s[3] = { 0.f, 0.f, 0.f }, q[4] = { 1.f, 0.f, 0.f, 0.f }, w = 0.f;
+    if (valid) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            mu[a] = p.means[3 * i + a];
+        gaussian_params(p, i, s, q, &w);
+    }
+    // per-(view, Gaussian) inputs of view v + 1 are loaded while view v is
+    // evaluated; the loads stay in bounds for every view and are unconditional
+    struct ViewIn {
+        uint32_t tiles;
+        float4 A;
+        float sg[9];
+        uint8_t cl[3];
+    };
+    const auto load_view = [&](int v, ViewIn& x) {
+        const int64_t o = int64_t(v) * n + i;
+        x.tiles = tiles[o];
+        x.A = conic_opacity[o];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            x.sg[k] = screen_grads[o * 9 + k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            x.cl[k] = clamped[3 * o + k];
+    };
+    ViewIn nx {};
+    if (valid && n_views > 0)
+        load_view(0, nx);
     int cur = -1;
     for (int v = 0; v <= n_views; ++v) {
         const int gv = v < n_views ? group[v] : -1;
+        ViewIn in = nx;
+        if (valid && v + 1 < n_views)
+            load_view(v + 1, nx);
         if (gv != cur) {
             if (cur >= 0) {
                 // views of group cur done: write its block; the SH gradient rows
@@ -152,10 +185,8 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
         }
         if (gv < 0 || !valid)
             continue;
-        const int64_t o = int64_t(v) * n + i;
-        if (tiles[o] != 0)
-            preprocess_backward_gaussian(i, p, row, cams[v], conic_opacity[o], clamped + 3 * o, screen_grads + o * 9, g_mu, g_s, g_q, &g_w,
-                g_shv);
+        if (in.tiles != 0)
+            preprocess_backward_core(mu, s, q, w, p, row, cams[v], in.A, in.cl, in.sg, g_mu, g_s, g_q, &g_w, g_shv);
     }
 }
 

@@ -554,14 +554,29 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
 //                   active degree) and g_sh[16 + ch] = dL/drgb_ch, for a
 //                   wave-cooperative coalesced write of the products.
 enum class ShGrad { Add, Export };
+// the per-Gaussian values (mean, scales times the multiplier, rotation,
+// opacity) passed in, for callers that evaluate one Gaussian for many views
+template <ShGrad MODE = ShGrad::Add>
+__device__ __forceinline__ void preprocess_backward_core(const float* mu, const float* s, const float* q, float w, const SplatParams& p,
+    const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc,
+    float* g_q_acc, float* g_w_acc, float* g_sh);
+
 template <ShGrad MODE = ShGrad::Add>
 __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const SplatParams& p, const float* c, const DevCamera<float>& cam,
     const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc, float* g_q_acc, float* g_w_acc, float* g_sh)
 {
     const float mu[3] = { p.means[3 * i], p.means[3 * i + 1], p.means[3 * i + 2] };
-    const float m = p.scale_multiplier;
     float s[3], q[4], w;
     gaussian_params(p, i, s, q, &w);
+    preprocess_backward_core<MODE>(mu, s, q, w, p, c, cam, A, clamped3, sg, g_mu_acc, g_s_acc, g_q_acc, g_w_acc, g_sh);
+}
+
+template <ShGrad MODE>
+__device__ __forceinline__ void preprocess_backward_core(const float* mu, const float* s, const float* q, float w, const SplatParams& p,
+    const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc,
+    float* g_q_acc, float* g_w_acc, float* g_sh)
+{
+    const float m = p.scale_multiplier;
     // conic = cov^-1: dL/dcov = -A G A
     const float G00 = sg[2], G01 = 0.5f * sg[3], G11 = sg[4];
     const float t00 = A.x * G00 + A.y * G01, t01 = A.x * G01 + A.y * G11;
