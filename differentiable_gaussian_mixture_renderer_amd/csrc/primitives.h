@@ -35,4 +35,12 @@ size_t radix_scratch_bytes(int64_t n_max);
 hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit,
     void* scratch, hipStream_t stream);
 
+// --- stable compaction of (key, value) pairs whose key is not skip_key ----
+// Keeps input order.  *count (device) receives the number kept; block_counts
+// and block_offsets hold compact_blocks(n) entries; scratch holds
+// scan_scratch_bytes(compact_blocks(n)) bytes.
+int64_t compact_blocks(int64_t n);
+hipError_t compact_pairs(const uint32_t* keys_in, const uint32_t* vals_in, int64_t n, uint32_t skip_key, uint32_t* keys_out,
+    uint32_t* vals_out, uint32_t* count, uint32_t* block_counts, uint32_t* block_offsets, void* scratch, hipStream_t stream);
+
 } // namespace dgmr

@@ -284,7 +284,95 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     }
 }
 
+// ------------------------------------------------------------- compaction
+constexpr int kCompactThreads = 256;
+constexpr int kCompactItems = 16;
+constexpr int kCompactTile = kCompactThreads * kCompactItems;
+
+__global__ __launch_bounds__(kCompactThreads) void k_compact_count(const uint32_t* __restrict__ keys, int64_t n, uint32_t skip_key,
+    uint32_t* __restrict__ block_counts)
+{
+    __shared__ uint32_t s_wave[kCompactThreads / 64];
+    const int64_t base = int64_t(blockIdx.x) * kCompactTile;
+    uint32_t c = 0;
+#pragma unroll 4
+    for (int j = 0; j < kCompactItems; ++j) {
+        const int64_t i = base + int64_t(j) * kCompactThreads + threadIdx.x;
+        c += (i < n && keys[i] != skip_key) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        c += __shfl_xor(c, d, 64);
+    if (lane_id() == 0)
+        s_wave[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kCompactThreads / 64; ++w)
+            t += s_wave[w];
+        block_counts[blockIdx.x] = t;
+    }
+}
+
+// Each wave owns a contiguous quarter of the block's items and walks it in
+// rounds of 64; kept items take ranks by ballot prefix, so the output keeps
+// the input order.
+__global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+    int64_t n, uint32_t skip_key, const uint32_t* __restrict__ block_offsets, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
+{
+    constexpr int kWaves = kCompactThreads / 64;
+    constexpr int kRounds = kCompactTile / kCompactThreads;
+    __shared__ uint32_t s_wave[kWaves];
+    const int64_t base = int64_t(blockIdx.x) * kCompactTile;
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64;
+    const uint64_t lt = (uint64_t(1) << lane) - 1;
+    uint32_t k[kRounds], v[kRounds];
+    bool keep[kRounds];
+    uint32_t wave_count = 0;
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int64_t i = base + int64_t(wave) * (kRounds * 64) + r * 64 + lane;
+        k[r] = i < n ? keys[i] : skip_key;
+        v[r] = i < n ? vals[i] : 0u;
+        keep[r] = i < n && k[r] != skip_key;
+        wave_count += uint32_t(__popcll(__ballot(keep[r])));
+    }
+    if (lane == 0)
+        s_wave[wave] = wave_count;
+    __syncthreads();
+    uint32_t pos = block_offsets[blockIdx.x];
+    for (unsigned w = 0; w < wave; ++w)
+        pos += s_wave[w];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const uint64_t m = __ballot(keep[r]);
+        if (keep[r]) {
+            const uint32_t o = pos + uint32_t(__popcll(m & lt));
+            keys_out[o] = k[r];
+            vals_out[o] = v[r];
+        }
+        pos += uint32_t(__popcll(m));
+    }
+}
+
 } // namespace
+
+int64_t compact_blocks(int64_t n) { return std::max<int64_t>(1, ceil_div(n, kCompactTile)); }
+
+hipError_t compact_pairs(const uint32_t* keys_in, const uint32_t* vals_in, int64_t n, uint32_t skip_key, uint32_t* keys_out,
+    uint32_t* vals_out, uint32_t* count, uint32_t* block_counts, uint32_t* block_offsets, void* scratch, hipStream_t stream)
+{
+    const int64_t blocks = compact_blocks(n);
+    k_compact_count<<<unsigned(blocks), kCompactThreads, 0, stream>>>(keys_in, n, skip_key, block_counts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    e = scan_u32(block_counts, block_offsets, count, blocks, false, scratch, stream);
+    if (e != hipSuccess)
+        return e;
+    k_compact_scatter<<<unsigned(blocks), kCompactThreads, 0, stream>>>(keys_in, vals_in, n, skip_key, block_offsets, keys_out, vals_out);
+    return hipGetLastError();
+}
 
 int radix_passes(int begin_bit, int end_bit)
 {

@@ -31,6 +31,8 @@ namespace dgmr {
 namespace {
 
 constexpr int kPreThreads = 256;
+// depth sorts of at least this many keys compact the visible entries first
+constexpr int64_t kCompactMin = int64_t(4) << 20;
 
 __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess(int64_t n, SplatParams p, DevCamera<float> cam, SplatOut out)
 {
@@ -112,13 +114,20 @@ __device__ __forceinline__ void owner_rect(uint32_t id, const int32_t* __restric
 // its owners' rectangles as contiguous runs; with the preprocess's packed bin
 // rectangle that is one 8-byte read per Gaussian.
 template <class S, bool RECTS>
-__global__ __launch_bounds__(256) void k_gather_owners(int64_t n, const uint32_t* __restrict__ ids, const uint2* __restrict__ bin_rects,
+__global__ __launch_bounds__(256) void k_gather_owners(int64_t n, const uint32_t* __restrict__ n_sorted, const uint32_t* __restrict__ ids,
+    const uint2* __restrict__ bin_rects,
     const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const S* __restrict__ xy, int grid_x, int grid_y, int tile_w,
     int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ counts, uint2* __restrict__ owner_rects)
 {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n)
         return;
+    if (n_sorted && j >= int64_t(*n_sorted)) {
+        // past the compacted (visible) entries: no pairs
+        counts[j] = 0;
+        owner_rects[j] = make_uint2(0u, 0u);
+        return;
+    }
     const uint32_t id = ids[j];
     uint32_t start, shape, count;
     if constexpr (RECTS) {
@@ -415,10 +424,11 @@ __global__ __launch_bounds__(64) void k_selftest_wave_sum(const float* __restric
     out[64 + l] = wave_sum_to_last(in[512 + l]);
 }
 
-__global__ void k_view_keys(int64_t n, const uint32_t* __restrict__ ids, int64_t n_per_view, uint32_t* __restrict__ keys)
+__global__ void k_view_keys(int64_t n, const uint32_t* __restrict__ n_sorted, const uint32_t* __restrict__ ids, int64_t n_per_view,
+    uint32_t* __restrict__ keys)
 {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j < n)
+    if (j < n && (!n_sorted || j < int64_t(*n_sorted)))
         keys[j] = ids[j] / uint32_t(n_per_view); // ids < 2^32: a 32-bit division
 }
 } // namespace
@@ -535,35 +545,53 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     const int4* views = batch ? batch->views : nullptr;
     const int64_t n_per_view = batch ? batch->n_per_view : 0;
     SplatWorkspace w = carve_splat(cv, n, bins->pair_capacity);
-    // stable sort of Gaussian ids by depth bits
-    SortBuffers db { w.dkeys_a, w.dvals_a, w.dkeys_b, w.dvals_b };
+    // stable sort of Gaussian ids by depth bits.  Large sorts first keep only
+    // the visible entries (culled ones carry the key 0xffffffff and would sort
+    // last anyway), in order, so the sort and everything after it sees fewer
+    // keys; n_sorted (device) is then their count.
+    uint32_t *K = w.dkeys_a, *V = w.dvals_a, *K2 = w.dkeys_b, *V2 = w.dvals_b;
+    const uint32_t* n_sorted = nullptr;
+    if (n >= kCompactMin) {
+        StageScope sc("bin.compact", st, 16.0 * double(n));
+        uint32_t* count = w.counter + 1;
+        DGMR_HIP_CHECK(compact_pairs(K, V, n, 0xffffffffu, K2, V2, count, w.counts, w.pair_offsets, w.scan_scratch, st));
+        std::swap(K, K2);
+        std::swap(V, V2);
+        n_sorted = count;
+    }
     {
         StageScope sc("bin.depth_sort", st, 16.0 * double(n) * radix_passes(0, 32));
-        DGMR_HIP_CHECK(radix_sort_pairs(db, n, nullptr, 0, 32, w.sort_scratch, st));
+        DGMR_HIP_CHECK(radix_sort_pairs(SortBuffers { K, V, K2, V2 }, n, n_sorted, 0, 32, w.sort_scratch, st));
+        if (radix_passes(0, 32) % 2 == 1) {
+            std::swap(K, K2);
+            std::swap(V, V2);
+        }
     }
-    const uint32_t* sorted_ids = (radix_passes(0, 32) % 2 == 0) ? w.dvals_a : w.dvals_b;
     if (batch && batch->n_views > 1) {
         // stable pass on the view index: order becomes (view, depth bits, id)
         StageScope sc("bin.view_sort", st, 16.0 * double(n));
-        k_view_keys<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, sorted_ids, n_per_view, w.dkeys_a);
+        k_view_keys<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_sorted, V, n_per_view, K);
         DGMR_HIP_CHECK(hipGetLastError());
-        SortBuffers vb { w.dkeys_a, w.dvals_a, w.dkeys_b, w.dvals_b };
         const int vbits = int(higher_msb(uint32_t(batch->n_views - 1)));
-        DGMR_HIP_CHECK(radix_sort_pairs(vb, n, nullptr, 0, vbits, w.sort_scratch, st));
-        sorted_ids = (radix_passes(0, vbits) % 2 == 0) ? w.dvals_a : w.dvals_b;
+        DGMR_HIP_CHECK(radix_sort_pairs(SortBuffers { K, V, K2, V2 }, n, n_sorted, 0, vbits, w.sort_scratch, st));
+        if (radix_passes(0, vbits) % 2 == 1) {
+            std::swap(K, K2);
+            std::swap(V, V2);
+        }
     }
+    const uint32_t* sorted_ids = V;
     const unsigned blocks_n = unsigned(ceil_div64(n, 256));
     {
         StageScope sc("bin.scan", st, 44.0 * double(n));
         if (bin_rects_written)
-            k_gather_owners<float, true><<<blocks_n, 256, 0, st>>>(n, sorted_ids, w.bin_rects, rects, tiles, static_cast<const float*>(xy), grid_x,
-                grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
+            k_gather_owners<float, true><<<blocks_n, 256, 0, st>>>(n, n_sorted, sorted_ids, w.bin_rects, rects, tiles, static_cast<const float*>(xy),
+                grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
         else if (xy_double)
-            k_gather_owners<double, false><<<blocks_n, 256, 0, st>>>(n, sorted_ids, w.bin_rects, rects, tiles, static_cast<const double*>(xy),
-                grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
+            k_gather_owners<double, false><<<blocks_n, 256, 0, st>>>(n, n_sorted, sorted_ids, w.bin_rects, rects, tiles,
+                static_cast<const double*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
         else
-            k_gather_owners<float, false><<<blocks_n, 256, 0, st>>>(n, sorted_ids, w.bin_rects, rects, tiles, static_cast<const float*>(xy),
-                grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
+            k_gather_owners<float, false><<<blocks_n, 256, 0, st>>>(n, n_sorted, sorted_ids, w.bin_rects, rects, tiles,
+                static_cast<const float*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
         DGMR_HIP_CHECK(hipGetLastError());
         DGMR_HIP_CHECK(scan_u32(w.counts, w.pair_offsets, w.counter, n, false, w.scan_scratch, st));
         // the reference's cache keeps the inclusive scan in Gaussian order
