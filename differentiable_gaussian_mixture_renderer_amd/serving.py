@@ -29,8 +29,10 @@ from . import losses, scenes
 from .batching import BatchSplatter
 
 # bytes of device memory per (view, Gaussian) slot of the batched path and per
-# (tile, Gaussian) pair, used for admission (geometry + binning + screen grads)
-BYTES_PER_VIEW_GAUSSIAN = 112
+# (tile, Gaussian) pair, used for admission: geometry 91 (rects, tiles, xy,
+# conic, rgb, clamp mask, depth, screen gradients) + binning 40 (depth keys
+# and ids twice, bin and owner rectangles, counts, offsets) + sort scratch
+BYTES_PER_VIEW_GAUSSIAN = 136
 BYTES_PER_PAIR = 24
 PAIRS_PER_PIXEL = 8
 
@@ -130,7 +132,7 @@ class ServerStats:
 
 
 class ContinuousBatchingServer:
-    def __init__(self, resident: List[ResidentScene], hbm_budget_bytes: int, workers: int = 3, max_batch_views: int = 32,
+    def __init__(self, resident: List[ResidentScene], hbm_budget_bytes: int, workers: int = 3, max_batch_views: int = 64,
                  tile=(16, 16), lds_batch: int = 64, device="cuda:0"):
         self.resident = resident
         self.budget = int(hbm_budget_bytes)
