@@ -86,6 +86,16 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t
     }
 }
 
+__global__ __launch_bounds__(256) void k_zero_visible(int64_t m, const uint32_t* __restrict__ tiles, float* __restrict__ screen_grads)
+{
+    const int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (o < m && tiles[o] != 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            screen_grads[9 * o + k] = 0.f;
+    }
+}
+
 // thread per Gaussian, summing its adjoint over the views of each gradient
 // group: group[v] is -1 (no gradient) or a group id whose views are
 // consecutive; group g's sums go to block g of the outputs ([n_groups][n][..]).
@@ -431,7 +441,10 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
     DGMR_HIP_CHECK(hipMemcpyAsync(L.views, vb.data(), sizeof(ViewBlend) * n_views, hipMemcpyHostToDevice, st));
     DGMR_HIP_CHECK(hipMemcpyAsync(L.groups, grp.data(), sizeof(int32_t) * n_views, hipMemcpyHostToDevice, st));
     const int64_t m = n * n_views;
-    DGMR_HIP_CHECK(hipMemsetAsync(L.screen_grads, 0, size_t(m) * 9 * sizeof(float), st));
+    // the blend backward adds into the screen gradients of visible entries only,
+    // and only those are read afterwards: clear just them
+    k_zero_visible<<<unsigned(ceil_div64(m, 256)), 256, 0, st>>>(m, L.tiles, L.screen_grads);
+    DGMR_HIP_CHECK(hipGetLastError());
     if (n_pairs > 0) {
         StageScope sc("batch.blend_backward", st, 40.0 * double(n_pairs));
         DGMR_HIP_CHECK(hipMemsetAsync(L.queue, 0, sizeof(uint32_t), st));
