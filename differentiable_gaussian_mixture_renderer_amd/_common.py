@@ -75,10 +75,11 @@ def camera_struct(world_view_transform, full_proj_transform, camera_center, bg_c
                            image_width, image_height)
 
 
-def tiling_struct(tile=(16, 16), batch=64) -> N.Tiling:
+def tiling_struct(tile=(16, 16), batch=64, cull: bool = False) -> N.Tiling:
     t = N.Tiling()
     t.tile_w, t.tile_h = int(tile[0]), int(tile[1])
     t.batch = int(batch)
+    t.cull = int(bool(cull))
     return t
 
 

@@ -55,7 +55,7 @@ class Camera(ctypes.Structure):
 
 
 class Tiling(ctypes.Structure):
-    _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32), ("batch", ctypes.c_int32)]
+    _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32), ("batch", ctypes.c_int32), ("cull", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -133,6 +133,11 @@ class RasterOptions(ctypes.Structure):
     _fields_ = [("render_mode", ctypes.c_int32), ("debug_bin", ctypes.c_int32), ("max_depth", ctypes.c_float)]
 
 
+class L1Item(ctypes.Structure):
+    _fields_ = [("image", c_void_p), ("target", c_void_p), ("grad", c_void_p), ("grad_loss", c_void_p), ("n", ctypes.c_int64),
+                ("scale", ctypes.c_float), ("reserved", ctypes.c_int32)]
+
+
 _P = ctypes.POINTER
 # exported symbol -> (restype, argtypes); this list mirrors include/dgmr.h
 SIGNATURES = {
@@ -165,6 +170,7 @@ SIGNATURES = {
     "dgmr_l1_partials_size": (ctypes.c_int, [ctypes.c_int64, _P(ctypes.c_int32)]),
     "dgmr_l1_loss_forward": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p, c_void_p]),
     "dgmr_l1_loss_backward": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p, c_void_p, c_void_p]),
+    "dgmr_l1_loss_backward_many": (ctypes.c_int, [_P(L1Item), ctypes.c_int32, c_void_p]),
     "dgmr_sort_workspace_size": (ctypes.c_int, [ctypes.c_int64, _P(ctypes.c_size_t)]),
     "dgmr_sort_pairs": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, c_void_p,
                                        ctypes.c_size_t, c_void_p]),

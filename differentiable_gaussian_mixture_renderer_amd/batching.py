@@ -19,6 +19,7 @@ import torch
 
 from . import _common as C
 from . import _native as N
+from . import losses
 
 
 def _cam_struct(cam, bg):
@@ -29,8 +30,10 @@ def _cam_struct(cam, bg):
 class BatchSplatter:
     """Forward and backward of one scene over a list of cameras (scenes.Camera)."""
 
-    def __init__(self, tile=(16, 16), batch: int = 64, device="cuda:0"):
-        self.tiling = C.tiling_struct(tile, batch)
+    def __init__(self, tile=(16, 16), batch: int = 64, device="cuda:0", cull: bool = False):
+        # cull: bin into the tiles where alpha can reach 1/255 only (same images and gradients, about a
+        # fifth fewer pairs; over many small views the per-row reach arithmetic costs more than it saves)
+        self.tiling = C.tiling_struct(tile, batch, cull)
         self.device = torch.device(device)
         self.capacity = 0
         self._ws = None
@@ -38,27 +41,31 @@ class BatchSplatter:
         self._n_pairs = 0
         self._keep = []
 
-    def _view_array(self, cams, bg, fbs, Ts, grads=None):
+    def _view_array(self, cams, bg, fbs, Ts):
         arr = (N.BatchView * len(cams))()
         for v, cam in enumerate(cams):
             arr[v].cam = _cam_struct(cam, bg)
             arr[v].framebuffer = fbs[v].data_ptr()
             arr[v].final_transmittance = Ts[v].data_ptr()
-            arr[v].grad_framebuffer = grads[v].data_ptr() if grads is not None else None
+            arr[v].grad_framebuffer = None
         return arr
 
-    def forward(self, shs, opacity, means, scales, rotations, cameras: Sequence, bg=(0.0, 0.0, 0.0), sh_degree: int = 3):
+    def _prepare(self, shs, opacity, means, scales, rotations, cameras, bg, sh_degree):
         shs, opacity, means, scales, rotations = C.prepare_inputs(shs, opacity, means, scales, rotations, True)
         dev = means.device
-        n = means.shape[0]
-        lib = N.load()
         self._g = C.gaussians_struct(shs, opacity, means, scales, rotations, sh_degree)
         self._keep = [shs, opacity, means, scales, rotations]
         self.cameras = list(cameras)
         self.bg = tuple(float(b) for b in bg)
         fbs = [torch.empty((3, c.height, c.width), dtype=torch.float32, device=dev) for c in self.cameras]
         Ts = [torch.empty((c.height, c.width), dtype=torch.float32, device=dev) for c in self.cameras]
-        views = self._view_array(self.cameras, self.bg, fbs, Ts)
+        self._views = self._view_array(self.cameras, self.bg, fbs, Ts)
+        self._fbs, self._Ts = fbs, Ts
+        return dev
+
+    def _launch_forward(self, dev):
+        n = self._keep[2].shape[0]
+        lib = N.load()
         # pair capacity from the largest pairs-per-pixel density seen so far
         px = sum(c.width * c.height for c in self.cameras)
         cap = max(int(C.pairs_per_pixel(dev) * px * 1.25) + 65536, 1 << 16)
@@ -67,12 +74,12 @@ class BatchSplatter:
         with torch.cuda.device(dev):
             for _attempt in range(3):
                 size = ctypes.c_size_t(0)
-                N.check(lib.dgmr_batch_workspace_size(n, views, len(self.cameras), ctypes.byref(self.tiling), cap, ctypes.byref(size)),
+                N.check(lib.dgmr_batch_workspace_size(n, self._views, len(self.cameras), ctypes.byref(self.tiling), cap, ctypes.byref(size)),
                         "dgmr_batch_workspace_size")
                 if self._ws is None or self._ws.numel() < size.value:
                     self._ws = None
                     self._ws = torch.empty(int(size.value * 1.05) + 256, dtype=torch.uint8, device=dev)
-                rc = lib.dgmr_splat_batch_forward(ctypes.byref(self._g), views, len(self.cameras), ctypes.byref(self.tiling),
+                rc = lib.dgmr_splat_batch_forward(ctypes.byref(self._g), self._views, len(self.cameras), ctypes.byref(self.tiling),
                                                   self._ws.data_ptr(), self._ws.numel(), cap, ctypes.byref(stats), stream)
                 if rc == N.DGMR_ERR_CAPACITY:
                     cap = int(stats.n_pairs * 1.25) + 4096
@@ -84,8 +91,43 @@ class BatchSplatter:
         self.capacity = cap
         self._n_pairs = int(stats.n_pairs)
         C.remember_pairs_per_pixel(dev, self._n_pairs / max(px, 1))
-        self._fbs, self._Ts = fbs, Ts
-        return fbs, Ts
+
+    def forward(self, shs, opacity, means, scales, rotations, cameras: Sequence, bg=(0.0, 0.0, 0.0), sh_degree: int = 3):
+        dev = self._prepare(shs, opacity, means, scales, rotations, cameras, bg, sh_degree)
+        self._launch_forward(dev)
+        return self._fbs, self._Ts
+
+    def step_l1(self, shs, opacity, means, scales, rotations, cameras: Sequence, targets: Sequence[torch.Tensor], bg=(0.0, 0.0, 0.0),
+                sh_degree: int = 3):
+        """One multi-view training step with the L1 loss: the forward of every
+        view, dL/dimage = sign(image - target) / numel for each view (one
+        launch), and the gradients summed over the views.  Equals
+        forward + losses.l1_grad_many + backward; every buffer and descriptor
+        is prepared before the forward waits for its pair count, so the loss
+        and backward launches follow that wait at once.  Returns (images,
+        (g_sh, g_opacity, g_means, g_scales, g_rotations))."""
+        dev = self._prepare(shs, opacity, means, scales, rotations, cameras, bg, sh_degree)
+        if len(targets) != len(self.cameras):
+            raise ValueError("step_l1: one target per camera")
+        targets = [t.to(device=dev, dtype=torch.float32).contiguous() for t in targets]
+        for f, t in zip(self._fbs, targets):
+            if f.shape != t.shape:
+                raise ValueError("step_l1: each target has its view's image shape")
+        sizes = [f.numel() for f in self._fbs]
+        flat = torch.empty(sum(sizes) + 4 * len(sizes), dtype=torch.float32, device=dev)
+        grads, off = [], 0
+        for f, k in zip(self._fbs, sizes):
+            grads.append(flat[off:off + k].view(f.shape))
+            off += (k + 3) // 4 * 4  # 16-byte aligned views
+        items = losses.l1_items(self._fbs, targets, grads)
+        for v, g in enumerate(grads):
+            self._views[v].grad_framebuffer = g.data_ptr()
+        out = self._grad_buffers(1)
+        self._launch_forward(dev)
+        N.check(N.load().dgmr_l1_loss_backward_many(items, len(items), C.stream_handle(dev)), "dgmr_l1_loss_backward_many")
+        self._launch_backward(None, 1, out)
+        self._step_keep = (targets, flat)  # alive until the next step's launches are queued behind these
+        return self._fbs, tuple(t[0] for t in out)
 
     @property
     def n_pairs(self) -> int:
@@ -97,9 +139,6 @@ class BatchSplatter:
         Otherwise groups[v] in {-1, 0..G-1} (a group's views consecutive, -1 =
         no gradient, its grad image may be None) and a list of G such tuples is
         returned, one per group."""
-        shs, opacity, means, scales, rotations = self._keep
-        dev = means.device
-        n = means.shape[0]
         G = 1 if groups is None else max(1, max(groups) + 1)
         zero = None
         grads = []
@@ -109,26 +148,34 @@ class BatchSplatter:
                     zero = torch.zeros_like(self._fbs[v])
                 g = zero
             grads.append(g.to(torch.float32).contiguous())
-        views = self._view_array(self.cameras, self.bg, self._fbs, self._Ts, grads)
-        f32 = dict(dtype=torch.float32, device=dev)
-        g_means = torch.empty((G, n, 3), **f32)
-        g_scales = torch.empty((G, n, 3), **f32)
-        g_rot = torch.empty((G, n, 4), **f32)
-        g_op = torch.empty((G, *opacity.shape), **f32)
-        g_sh = torch.empty((G, *shs.shape), **f32)
+        for v, g in enumerate(grads):  # the forward's descriptors; only the gradient images are new
+            self._views[v].grad_framebuffer = g.data_ptr()
+        out = self._grad_buffers(G)
+        self._launch_backward(groups, G, out)
+        out = [(g_sh, g_op, g_means, g_scales, g_rot) for g_sh, g_op, g_means, g_scales, g_rot in zip(*[list(t) for t in out])]
+        return out[0] if groups is None else out
+
+    def _grad_buffers(self, G):
+        shs, opacity, means, _, _ = self._keep
+        n = means.shape[0]
+        f32 = dict(dtype=torch.float32, device=means.device)
+        return (torch.empty((G, *shs.shape), **f32), torch.empty((G, *opacity.shape), **f32), torch.empty((G, n, 3), **f32),
+                torch.empty((G, n, 3), **f32), torch.empty((G, n, 4), **f32))
+
+    def _launch_backward(self, groups, G, out):
+        g_sh, g_op, g_means, g_scales, g_rot = out
+        dev = g_sh.device
         grp = None if groups is None else (ctypes.c_int32 * len(groups))(*[int(x) for x in groups])
         with torch.cuda.device(dev):
-            rc = N.load().dgmr_splat_batch_backward(ctypes.byref(self._g), views, len(self.cameras), ctypes.byref(self.tiling),
+            rc = N.load().dgmr_splat_batch_backward(ctypes.byref(self._g), self._views, len(self.cameras), ctypes.byref(self.tiling),
                                                     self._ws.data_ptr(), self._ws.numel(), self.capacity, self._n_pairs, grp, G,
                                                     g_means.data_ptr(), g_scales.data_ptr(), g_rot.data_ptr(), g_op.data_ptr(),
                                                     g_sh.data_ptr(), C.stream_handle(dev))
             N.check(rc, "dgmr_splat_batch_backward")
-        out = [(g_sh[k], g_op[k], g_means[k], g_scales[k], g_rot[k]) for k in range(G)]
-        return out[0] if groups is None else out
 
 
 def render_plain(shs, opacity, means, scales, rotations, cameras: Sequence, bg=(0.0, 0.0, 0.0), sh_degree: int = 3,
-                 grad_fn=None, tile=(16, 16), batch: int = 64):
+                 grad_fn=None, tile=(16, 16), batch: int = 64, cull: bool = False):
     """One Splatter forward (and backward when grad_fn maps image -> dL/dimage)
     per camera; gradients summed over views."""
     from .python_bindings import splatter
@@ -138,7 +185,7 @@ def render_plain(shs, opacity, means, scales, rotations, cameras: Sequence, bg=(
     for cam in cameras:
         args = (shs, opacity, means, scales, rotations, torch.tensor(cam.world_view_transform), torch.tensor(cam.full_proj_transform),
                 torch.tensor(cam.camera_center), bg_t, cam.width, cam.height, cam.tan_fovx, cam.tan_fovy, sh_degree)
-        cache = splatter.cpp_binding.forward(*args, tile=tile, batch=batch)
+        cache = splatter.cpp_binding.forward(*args, tile=tile, batch=batch, cull=cull)
         images.append(cache[0])
         if grad_fn is not None:
             g = splatter.cpp_binding.backward(*args, grad_fn(cache[0]), cache, tile=tile, batch=batch)

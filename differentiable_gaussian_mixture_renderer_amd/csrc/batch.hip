@@ -86,14 +86,32 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t
     }
 }
 
+// clears the 9 screen-gradient floats of every visible entry: thread q
+// clears floats [4q, 4q + 4) of the flat array with one 16-byte store when
+// either entry they belong to is visible (a culled neighbour's slot may be
+// cleared too; it is never read)
 __global__ __launch_bounds__(256) void k_zero_visible(int64_t m, const uint32_t* __restrict__ tiles, float* __restrict__ screen_grads)
 {
-    const int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (o < m && tiles[o] != 0) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-            screen_grads[9 * o + k] = 0.f;
+    const int64_t f = 4 * (int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
+    const int64_t total = 9 * m;
+    if (f >= total)
+        return;
+    const int64_t last = min(f + 3, total - 1);
+    int64_t o0, o1;
+    if (total <= int64_t(UINT32_MAX)) { // 32-bit divisions
+        o0 = uint32_t(f) / 9u;
+        o1 = uint32_t(last) / 9u;
+    } else {
+        o0 = f / 9;
+        o1 = last / 9;
     }
+    if (tiles[o0] == 0 && tiles[o1] == 0)
+        return;
+    if (f + 4 <= total)
+        *reinterpret_cast<float4*>(screen_grads + f) = make_float4(0.f, 0.f, 0.f, 0.f);
+    else
+        for (int64_t k = f; k < total; ++k)
+            screen_grads[k] = 0.f;
 }
 
 // thread per Gaussian, summing its adjoint over the views of each gradient
@@ -247,7 +265,7 @@ BatchLayout carve_batch(void* ws, int64_t n, const dgmr_batch_view* views, int n
     L.point_list = cv.take<uint32_t>(std::max<int64_t>(cap, 1));
     L.ranges = cv.take<uint32_t>(2 * std::max<int64_t>(total_tiles, 1));
     L.screen_grads = cv.take<float>(9 * m);
-    L.binning_bytes = splat_workspace_bytes(m, cap);
+    L.binning_bytes = splat_workspace_bytes(m, cap, t.cull != 0);
     L.binning = cv.take<char>(L.binning_bytes);
     *used = cv.used;
     return L;
@@ -345,15 +363,17 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     if (stats)
         stats->n_pairs = 0;
     const int64_t m = n * n_views;
+    const bool cull = tiling->cull != 0;
     uint32_t *dkeys, *dids;
-    uint2* brects;
-    binning_key_buffers(L.binning, m, &dkeys, &dids, &brects);
+    void* recs;
+    binning_key_buffers(L.binning, m, cull, &dkeys, &dids, &recs);
     if (n > 0) {
         StageScope sc("batch.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 67.0);
         const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
             static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), g->activation };
         k_batch_preprocess<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams,
-            SplatOut { L.rects, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, L.depths, dkeys, dids, brects });
+            SplatOut { L.rects, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, L.depths, dkeys, dids,
+                cull ? nullptr : static_cast<uint2*>(recs), cull ? static_cast<ReachRec*>(recs) : nullptr });
         DGMR_HIP_CHECK(hipGetLastError());
     }
     dgmr_binning bins { nullptr, L.point_list, L.ranges, pair_capacity };
@@ -366,7 +386,7 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     const BatchBinning bb { L.grids, std::max<int64_t>(n, 1), n_views, L.total_tiles, max_view_tiles };
     PendingCount pending;
     if (m > 0) {
-        const int rc = bin_gaussians(m, L.rects, L.tiles, L.xy, false, true, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &pending, st, &bb);
+        const int rc = bin_gaussians(m, L.rects, L.tiles, L.xy, false, true, cull, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &pending, st, &bb);
         if (rc != DGMR_OK)
             return rc;
     } else {
@@ -443,7 +463,7 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
     const int64_t m = n * n_views;
     // the blend backward adds into the screen gradients of visible entries only,
     // and only those are read afterwards: clear just them
-    k_zero_visible<<<unsigned(ceil_div64(m, 256)), 256, 0, st>>>(m, L.tiles, L.screen_grads);
+    k_zero_visible<<<unsigned(ceil_div64(ceil_div64(9 * m, 4), 256)), 256, 0, st>>>(m, L.tiles, L.screen_grads);
     DGMR_HIP_CHECK(hipGetLastError());
     if (n_pairs > 0) {
         StageScope sc("batch.blend_backward", st, 40.0 * double(n_pairs));

@@ -122,16 +122,19 @@ struct BatchBinning {
 
 // binning shared by both renderers (implemented with the splatter): depth sort,
 // scan, emission, pair sort, tile ranges.  The preprocess must have filled the
-// depth-key and id buffers returned by binning_key_buffers, and the bin
-// rectangles too when bin_rects_written (otherwise they are derived from
-// rects and xy).
+// depth-key and id buffers returned by binning_key_buffers, and its records
+// too when bin_rects_written (otherwise they are derived from rects and xy):
+// the bin rectangles, or with cull (tight culling) the reach records.
 // R comes back through *pending (wait() after the dependent launches); work
 // sized by R is bounded by bins->pair_capacity, and R > capacity leaves a
 // truncated, in-bounds binning that the caller reports as DGMR_ERR_CAPACITY.
-int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, bool bin_rects_written, int grid_x,
-    int grid_y, int tile_w, int tile_h, dgmr_binning* bins, Carver& cv, PendingCount* pending, hipStream_t st, const BatchBinning* batch);
-// bin_rects[o] = (first tile y0 * grid_x + x0, (y1 - y0) << 16 | (x1 - x0)), zero when culled
-void binning_key_buffers(void* workspace, int64_t n, uint32_t** keys, uint32_t** ids, uint2** bin_rects = nullptr);
-size_t splat_workspace_bytes(int64_t n, int64_t cap);
+int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, bool bin_rects_written, bool cull,
+    int grid_x, int grid_y, int tile_w, int tile_h, dgmr_binning* bins, Carver& cv, PendingCount* pending, hipStream_t st,
+    const BatchBinning* batch);
+// the preprocess's outputs in the workspace: depth keys, ids and the
+// per-Gaussian records (cull: ReachRec; otherwise uint2 bin rectangles
+// (first tile y0 * grid_x + x0, (y1 - y0) << 16 | (x1 - x0)), zero when culled)
+void binning_key_buffers(void* workspace, int64_t n, bool cull, uint32_t** keys, uint32_t** ids, void** records = nullptr);
+size_t splat_workspace_bytes(int64_t n, int64_t cap, bool cull = false);
 
 } // namespace dgmr

@@ -18,21 +18,28 @@ size_t scan_scratch_bytes(int64_t n);
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n, bool inclusive, void* scratch,
     hipStream_t stream);
 
-// --- stable LSD radix sort of (uint32 key, uint32 value) pairs -----------
+// --- stable LSD radix sort of (uint32 or uint16 key, uint32 value) pairs --
 // Sorts bits [begin_bit, end_bit) in 8-bit digits.  Pass p reads buffer
 // (p % 2 == 0 ? A : B) and writes the other one; the sorted result ends in A
 // when the number of passes is even, in B when it is odd.  n may be given on
 // the device (n_dev != nullptr, capacity n_max; min(*n_dev, n_max) items are
 // sorted) so no host sync is needed.
-struct SortBuffers {
-    uint32_t* keys_a;
+template <class KeyT>
+struct SortBuffersT {
+    KeyT* keys_a;
     uint32_t* vals_a;
-    uint32_t* keys_b;
+    KeyT* keys_b;
     uint32_t* vals_b;
 };
+using SortBuffers = SortBuffersT<uint32_t>;
+// 16-bit keys (end_bit <= 16): a quarter less traffic per pass for keys that
+// fit, such as tile ids
+using SortBuffers16 = SortBuffersT<uint16_t>;
 int radix_passes(int begin_bit, int end_bit);
 size_t radix_scratch_bytes(int64_t n_max);
 hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit,
+    void* scratch, hipStream_t stream);
+hipError_t radix_sort_pairs(const SortBuffers16& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit,
     void* scratch, hipStream_t stream);
 
 // --- stable compaction of (key, value) pairs whose key is not skip_key ----

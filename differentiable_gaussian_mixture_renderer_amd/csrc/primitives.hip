@@ -140,8 +140,8 @@ __device__ __forceinline__ int64_t sort_count(int64_t n_max, const uint32_t* n_d
 
 // per-block histogram of the digit (key >> shift) & (2^bits - 1), stored
 // digit-major: hist[digit * n_blocks + block]
-template <int ITEMS>
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n_max, const uint32_t* n_dev,
+template <int ITEMS, class KeyT>
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const KeyT* __restrict__ keys, int64_t n_max, const uint32_t* n_dev,
     int shift, int bits, int n_blocks, uint32_t* __restrict__ hist)
 {
     constexpr int kSortItems = ITEMS, kSortTile = kSortThreads * ITEMS;
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
         for (int j = 0; j < kSortItems; ++j) {
             const int64_t i = base + int64_t(j) * kSortThreads + threadIdx.x;
             if (i < n)
-                atomicAdd(&s_hist[wave][(keys[i] >> shift) & mask], 1u);
+                atomicAdd(&s_hist[wave][(uint32_t(keys[i]) >> shift) & mask], 1u);
         }
     }
     __syncthreads();
@@ -176,15 +176,15 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
 // item in LDS in digit order (stable: digit, wave, round, lane = input
 // order), and the LDS copy leaves the block one contiguous run per digit.
 // Four barriers per block.
-template <int ITEMS>
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int bits, int n_blocks,
+template <int ITEMS, class KeyT>
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    KeyT* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int bits, int n_blocks,
     const uint32_t* __restrict__ offsets)
 {
     constexpr int kSortTile = kSortThreads * ITEMS;
     constexpr int kWaves = kSortThreads / 64;
     constexpr int kRounds = kSortTile / kSortThreads; // rounds of 64 items per wave
-    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ KeyT s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t s_count[kWaves][kRadixMax]; // per-wave digit counts, then per-wave starts in LDS order
     __shared__ uint32_t s_local[kRadixMax];         // block-local start of each digit
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     for (int r = 0; r < kRounds; ++r) {
         const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
         const bool valid = i < count;
-        keys[r] = valid ? keys_in[base + i] : 0u;
+        keys[r] = valid ? uint32_t(keys_in[base + i]) : 0u;
         vals[r] = valid ? vals_in[base + i] : 0u;
     }
 #pragma unroll
@@ -270,14 +270,14 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
         const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
         if (i < count) {
             const uint32_t pos = s_count[wave][(keys[r] >> shift) & mask] + rank[r];
-            s_keys[pos] = keys[r];
+            s_keys[pos] = KeyT(keys[r]);
             s_vals[pos] = vals[r];
         }
     }
     __syncthreads();
     for (int q = threadIdx.x; q < count; q += kSortThreads) {
-        const uint32_t key = s_keys[q];
-        const uint32_t d = (key >> shift) & mask;
+        const KeyT key = s_keys[q];
+        const uint32_t d = (uint32_t(key) >> shift) & mask;
         const uint32_t dst = s_global[d] + (uint32_t(q) - s_local[d]);
         keys_out[dst] = key;
         vals_out[dst] = s_vals[q];
@@ -397,7 +397,8 @@ size_t radix_scratch_bytes(int64_t n_max)
     return ((hist + 255) / 256) * 256 + scan_scratch_bytes(blocks * kRadixMax);
 }
 
-hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit, void* scratch,
+template <class KeyT>
+static hipError_t radix_sort_impl(const SortBuffersT<KeyT>& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit, void* scratch,
     hipStream_t stream)
 {
     const int passes = radix_passes(begin_bit, end_bit);
@@ -411,26 +412,38 @@ hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_
     for (int p = 0; p < passes; ++p) {
         const int shift = begin_bit + width * p;
         const int bits = std::min(width, end_bit - shift);
-        const uint32_t* kin = (p % 2 == 0) ? buf.keys_a : buf.keys_b;
+        const KeyT* kin = (p % 2 == 0) ? buf.keys_a : buf.keys_b;
         const uint32_t* vin = (p % 2 == 0) ? buf.vals_a : buf.vals_b;
-        uint32_t* kout = (p % 2 == 0) ? buf.keys_b : buf.keys_a;
+        KeyT* kout = (p % 2 == 0) ? buf.keys_b : buf.keys_a;
         uint32_t* vout = (p % 2 == 0) ? buf.vals_b : buf.vals_a;
         if (large)
-            k_radix_hist<kLargeItems><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist);
+            k_radix_hist<kLargeItems, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist);
         else
-            k_radix_hist<kSmallItems><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist);
+            k_radix_hist<kSmallItems, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist);
         hipError_t e = scan_u32(hist, hist, nullptr, int64_t(blocks) << bits, false, scan_scratch, stream);
         if (e != hipSuccess)
             return e;
         if (large)
-            k_radix_scatter<kLargeItems><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist);
+            k_radix_scatter<kLargeItems, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist);
         else
-            k_radix_scatter<kSmallItems><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist);
+            k_radix_scatter<kSmallItems, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist);
         e = hipGetLastError();
         if (e != hipSuccess)
             return e;
     }
     return hipSuccess;
+}
+
+hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit, void* scratch,
+    hipStream_t stream)
+{
+    return radix_sort_impl(buf, n_max, n_dev, begin_bit, end_bit, scratch, stream);
+}
+
+hipError_t radix_sort_pairs(const SortBuffers16& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit, void* scratch,
+    hipStream_t stream)
+{
+    return end_bit > 16 ? hipErrorInvalidValue : radix_sort_impl(buf, n_max, n_dev, begin_bit, end_bit, scratch, stream);
 }
 
 } // namespace dgmr

@@ -574,19 +574,19 @@ int dgmr_raster_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const d
     }
     uint32_t *dkeys, *dids;
     uint2* brects;
-    binning_key_buffers(workspace, n, &dkeys, &dids, &brects);
+    binning_key_buffers(workspace, n, false, &dkeys, &dids, reinterpret_cast<void**>(&brects));
     {
         StageScope sc("raster.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs + 75.0 + 28.0));
         DGMR_HIP_CHECK(launch_splat_preprocess(g, dc,
             SplatOut { geom->rects, geom->tiles_touched, geom->xy, reinterpret_cast<float4*>(geom->conic_opacity), geom->rgb, geom->rgb_clamped,
-                geom->depths, dkeys, dids, brects },
+                geom->depths, dkeys, dids, brects, nullptr },
             st));
         k_raster_extra<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, make_splat_params(g), dc, geom->tiles_touched, inv_cov, mass);
         DGMR_HIP_CHECK(hipGetLastError());
     }
     Carver cv(workspace, workspace_bytes);
     PendingCount pending;
-    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, false, true, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv,
+    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, false, true, false, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv,
         &pending, st, nullptr);
     if (rc != DGMR_OK)
         return rc;

@@ -145,9 +145,182 @@ __global__ __launch_bounds__(256) void k_gather_owners(int64_t n, const uint32_t
     owner_rects[j] = make_uint2(start, shape);
 }
 
+// eight consecutive keys at a 16-byte aligned address: 32-bit keys in two
+// stores, 16-bit keys packed into one
+__device__ __forceinline__ void store_keys8(uint32_t* out, const uint32_t (&kv)[kEmitPer])
+{
+    uint4* d = reinterpret_cast<uint4*>(out);
+    d[0] = make_uint4(kv[0], kv[1], kv[2], kv[3]);
+    d[1] = make_uint4(kv[4], kv[5], kv[6], kv[7]);
+}
+__device__ __forceinline__ void store_keys8(uint16_t* out, const uint32_t (&kv)[kEmitPer])
+{
+    *reinterpret_cast<uint4*>(out) = make_uint4(kv[0] | (kv[1] << 16), kv[2] | (kv[3] << 16), kv[4] | (kv[5] << 16), kv[6] | (kv[7] << 16));
+}
+
+// Tight culling: per position j of the depth order, the pair count of
+// Gaussian ids[j] (tiles its reach ellipse meets, row by row) and its record,
+// copied so the emission reads its owners' records as contiguous runs.
+// Entries the preprocess culled have no record: they are those past n_sorted
+// (compacted sorts) or whose final sort key is culled_key.
+__global__ __launch_bounds__(256) void k_gather_reach(int64_t n, const uint32_t* __restrict__ n_sorted, const uint32_t* __restrict__ keys,
+    uint32_t culled_key, const uint32_t* __restrict__ ids, const ReachRec* __restrict__ reach, int tile_w, int tile_h,
+    uint32_t* __restrict__ counts, ReachRec* __restrict__ owners)
+{
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n)
+        return;
+    uint4* dst = reinterpret_cast<uint4*>(owners + j);
+    if ((n_sorted && j >= int64_t(*n_sorted)) || keys[j] == culled_key) {
+        counts[j] = 0;
+        dst[0] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const uint4* src = reinterpret_cast<const uint4*>(reach + ids[j]);
+    const uint4 w0 = src[0], w1 = src[1];
+    uint32_t count = 0;
+    if (w0.y != 0u) {
+        const ReachRec r { w0.x, w0.y, w0.z, __uint_as_float(w0.w), __uint_as_float(w1.x), __uint_as_float(w1.y), __uint_as_float(w1.z),
+            __uint_as_float(w1.w) };
+        const ReachEllipse e = reach_ellipse(r);
+        const int x0 = int(r.origin & 0xffffu), y0 = int(r.origin >> 16), cols = int(r.dims & 0xffffu), rows = int(r.dims >> 16);
+        for (int q = 0; q < rows; ++q) {
+            const int2 c = reach_columns(e, y0 + q, x0, cols, tile_w, tile_h);
+            if (c.x <= c.y)
+                count += uint32_t(c.y - c.x + 1);
+        }
+    }
+    counts[j] = count;
+    dst[0] = w0;
+    dst[1] = w1;
+}
+
+// the emission cursor of one owner: its tile row r (from the rectangle's
+// first), that row's reached columns c, and the column offset within them
+struct ReachCursor {
+    ReachEllipse e;
+    int x0, y0, cols, rows, gx;
+    int r;
+    int2 c;
+};
+
+__device__ __forceinline__ void cursor_load(ReachCursor& cu, const ReachRec* __restrict__ owners, uint32_t k, int tile_w, int tile_h)
+{
+    const uint4* src = reinterpret_cast<const uint4*>(owners + k);
+    const uint4 w0 = src[0], w1 = src[1];
+    const ReachRec r { w0.x, w0.y, w0.z, __uint_as_float(w0.w), __uint_as_float(w1.x), __uint_as_float(w1.y), __uint_as_float(w1.z),
+        __uint_as_float(w1.w) };
+    cu.e = reach_ellipse(r);
+    cu.x0 = int(r.origin & 0xffffu);
+    cu.y0 = int(r.origin >> 16);
+    cu.cols = int(r.dims & 0xffffu);
+    cu.rows = int(r.dims >> 16);
+    cu.gx = int(r.grid_x);
+    cu.r = 0;
+    cu.c = reach_columns(cu.e, cu.y0, cu.x0, cu.cols, tile_w, tile_h);
+}
+
+// moves the cursor to the row holding column offset *local (from the current
+// row) and returns the tile; rows past the rectangle are never entered
+__device__ __forceinline__ uint32_t cursor_tile(ReachCursor& cu, uint32_t* local, int tile_w, int tile_h)
+{
+    while (cu.r < cu.rows) {
+        const uint32_t w = cu.c.x <= cu.c.y ? uint32_t(cu.c.y - cu.c.x + 1) : 0u;
+        if (*local < w)
+            break;
+        *local -= w;
+        ++cu.r;
+        if (cu.r < cu.rows)
+            cu.c = reach_columns(cu.e, cu.y0 + cu.r, cu.x0, cu.cols, tile_w, tile_h);
+    }
+    return uint32_t(cu.y0 + cu.r) * uint32_t(cu.gx) + uint32_t(cu.c.x) + *local;
+}
+
+// Tight-culling emission: the same block layout as k_emit_pairs; each thread
+// writes kEmitPer consecutive pairs, walking its owners' reached rows.
+template <class KeyT>
+__global__ __launch_bounds__(kEmitThreads) void k_emit_reach(const uint32_t* __restrict__ n_pairs, uint32_t capacity,
+    const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ bounds, const uint32_t* __restrict__ offsets,
+    const ReachRec* __restrict__ owners, int tile_w, int tile_h, KeyT* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
+{
+    __shared__ uint32_t s_off[kEmitPairs];
+    const uint32_t R = min(*n_pairs, capacity);
+    const uint32_t first = blockIdx.x * uint32_t(kEmitPairs);
+    if (first >= R)
+        return;
+    const uint32_t end = min(R, first + uint32_t(kEmitPairs));
+    const uint32_t g_lo = bounds[blockIdx.x], g_hi = bounds[blockIdx.x + 1];
+    const uint32_t window = g_hi - g_lo + 1;
+    const bool staged = window <= uint32_t(kEmitPairs);
+    if (staged) {
+        for (uint32_t k = threadIdx.x; k < window; k += kEmitThreads)
+            s_off[k] = offsets[g_lo + k];
+        __syncthreads();
+    }
+    // start of owner g_lo + k's pairs; past the window: no end within this block
+    auto off = [&](uint32_t k) -> uint32_t {
+        if (k >= window)
+            return 0xffffffffu;
+        return staged ? s_off[k] : offsets[g_lo + k];
+    };
+    const uint32_t p0 = first + threadIdx.x * kEmitPer;
+    if (p0 >= end)
+        return;
+    uint32_t lo = 0, hi = window - 1; // owner of p0: last k with off(k) <= p0
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (off(mid) <= p0)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    uint32_t k = lo;
+    uint32_t next = off(k + 1);
+    uint32_t local = p0 - off(k);
+    ReachCursor cu;
+    cursor_load(cu, owners, g_lo + k, tile_w, tile_h);
+    uint32_t id = sorted_ids[g_lo + k];
+    uint32_t kv[kEmitPer], vv[kEmitPer];
+#pragma unroll
+    for (int q = 0; q < kEmitPer; ++q) {
+        const uint32_t p = p0 + q;
+        if (p >= end) {
+            kv[q] = vv[q] = 0u;
+            continue;
+        }
+        if (p >= next) {
+            do {
+                ++k;
+                next = off(k + 1);
+            } while (p >= next);
+            cursor_load(cu, owners, g_lo + k, tile_w, tile_h);
+            id = sorted_ids[g_lo + k];
+            local = 0;
+        }
+        kv[q] = cursor_tile(cu, &local, tile_w, tile_h);
+        vv[q] = id;
+        ++local;
+    }
+    const bool aligned = ((reinterpret_cast<uintptr_t>(keys_out) | reinterpret_cast<uintptr_t>(vals_out)) & 15u) == 0;
+    if (aligned && p0 + kEmitPer <= end) {
+        store_keys8(keys_out + p0, kv);
+        uint4* vd = reinterpret_cast<uint4*>(vals_out + p0);
+        vd[0] = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+        vd[1] = make_uint4(vv[4], vv[5], vv[6], vv[7]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kEmitPer; ++q)
+            if (p0 + q < end) {
+                keys_out[p0 + q] = KeyT(kv[q]);
+                vals_out[p0 + q] = vv[q];
+            }
+    }
+}
+
+template <class KeyT>
 __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const uint32_t* __restrict__ n_pairs, uint32_t capacity,
     const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ bounds, const uint32_t* __restrict__ offsets,
-    const uint2* __restrict__ owner_rects, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
+    const uint2* __restrict__ owner_rects, KeyT* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
 {
     __shared__ uint32_t s_off[kEmitPairs], s_id[kEmitPairs], s_start[kEmitPairs], s_shape[kEmitPairs];
     const uint32_t R = min(*n_pairs, capacity); // pairs past the capacity are not written
@@ -209,17 +382,15 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const ui
         const bool aligned = ((reinterpret_cast<uintptr_t>(keys_out) | reinterpret_cast<uintptr_t>(vals_out)) & 15u) == 0;
         if (aligned && p0 + kEmitPer <= end) {
             // 16-byte aligned: p0 is a multiple of 8
-            uint4* kd = reinterpret_cast<uint4*>(keys_out + p0);
+            store_keys8(keys_out + p0, kv);
             uint4* vd = reinterpret_cast<uint4*>(vals_out + p0);
-            kd[0] = make_uint4(kv[0], kv[1], kv[2], kv[3]);
-            kd[1] = make_uint4(kv[4], kv[5], kv[6], kv[7]);
             vd[0] = make_uint4(vv[0], vv[1], vv[2], vv[3]);
             vd[1] = make_uint4(vv[4], vv[5], vv[6], vv[7]);
         } else {
 #pragma unroll
             for (int q = 0; q < kEmitPer; ++q)
                 if (p0 + q < end) {
-                    keys_out[p0 + q] = kv[q];
+                    keys_out[p0 + q] = KeyT(kv[q]);
                     vals_out[p0 + q] = vv[q];
                 }
         }
@@ -239,7 +410,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const ui
             const uint32_t local = p - offsets[lo];
             const uint32_t w = shape & 0xffffu;
             const uint32_t ty = local / w, tx = local - ty * w;
-            keys_out[p] = start + ty * (shape >> 16) + tx;
+            keys_out[p] = KeyT(start + ty * (shape >> 16) + tx);
             vals_out[p] = id;
         }
     }
@@ -253,7 +424,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const ui
 // is monotone along the array: "key > t, or key == t and val >= bound".  With
 // one view the bound is 0 and the two ends are the lower bounds of t and t + 1.
 // The two searches of a thread run interleaved.
-__global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+template <class KeyT>
+__global__ __launch_bounds__(256) void k_tile_ranges(const KeyT* __restrict__ keys, const uint32_t* __restrict__ vals,
     const uint32_t* __restrict__ n_pairs, uint32_t capacity, const int4* __restrict__ views, int n_views, int64_t n_per_view, int64_t n_tiles,
     uint32_t* __restrict__ ranges)
 {
@@ -280,7 +452,7 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict_
     while (lo0 < hi0 || lo1 < hi1) {
         const uint32_t m0 = lo0 + (hi0 - lo0) / 2, m1 = lo1 + (hi1 - lo1) / 2;
         const bool a0 = lo0 < hi0, a1 = lo1 < hi1;
-        const uint32_t k0 = a0 ? keys[m0] : 0u, k1 = a1 ? keys[m1] : 0u;
+        const uint32_t k0 = a0 ? uint32_t(keys[m0]) : 0u, k1 = a1 ? uint32_t(keys[m1]) : 0u;
         const uint32_t v0 = a0 && views ? vals[m0] : 0u, v1 = a1 && views ? vals[m1] : 0u;
         const bool after0 = k0 > t0 || (k0 == t0 && uint64_t(v0) >= b0);
         const bool after1 = k1 > t1 || (k1 == t1 && uint64_t(v1) >= b1);
@@ -424,12 +596,17 @@ __global__ __launch_bounds__(64) void k_selftest_wave_sum(const float* __restric
     out[64 + l] = wave_sum_to_last(in[512 + l]);
 }
 
+// view index of each sorted entry, in place of its depth key; with
+// culled_view != ~0 an entry still holding the culled depth key 0xffffffff
+// gets culled_view instead (past every view: it sorts last)
 __global__ void k_view_keys(int64_t n, const uint32_t* __restrict__ n_sorted, const uint32_t* __restrict__ ids, int64_t n_per_view,
-    uint32_t* __restrict__ keys)
+    uint32_t culled_view, uint32_t* __restrict__ keys)
 {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j < n && (!n_sorted || j < int64_t(*n_sorted)))
-        keys[j] = ids[j] / uint32_t(n_per_view); // ids < 2^32: a 32-bit division
+    if (j < n && (!n_sorted || j < int64_t(*n_sorted))) {
+        const uint32_t v = ids[j] / uint32_t(n_per_view); // ids < 2^32: a 32-bit division
+        keys[j] = (culled_view != 0xffffffffu && keys[j] == 0xffffffffu) ? culled_view : v;
+    }
 }
 } // namespace
 
@@ -441,8 +618,10 @@ struct SplatWorkspace {
     uint32_t* dvals_b;
     uint32_t* counts;
     uint32_t* emit_bounds;
-    uint2* owner_rects;
+    uint2* owner_rects;   // without tight culling
     uint2* bin_rects;
+    ReachRec* owner_reach; // with tight culling (the same memory as owner_rects / bin_rects)
+    ReachRec* reach;
     uint32_t* pair_offsets;
     uint32_t* pkeys_a;
     uint32_t* pvals_a;
@@ -452,13 +631,15 @@ struct SplatWorkspace {
     void* scan_scratch;
 };
 
-static SplatWorkspace carve_splat(Carver& cv, int64_t n, int64_t cap)
+static SplatWorkspace carve_splat(Carver& cv, int64_t n, int64_t cap, bool cull)
 {
     SplatWorkspace w;
     const int64_t nn = std::max<int64_t>(n, 1), cc = std::max<int64_t>(cap, 1);
+    const int64_t rec = cull ? int64_t(sizeof(ReachRec)) : int64_t(sizeof(uint2)); // bytes per Gaussian record
     w.dkeys_a = cv.take<uint32_t>(nn);
     w.dvals_a = cv.take<uint32_t>(nn);
-    w.bin_rects = cv.take<uint2>(nn);
+    w.reach = reinterpret_cast<ReachRec*>(cv.take<char>(nn * rec));
+    w.bin_rects = reinterpret_cast<uint2*>(w.reach);
     w.dkeys_b = cv.take<uint32_t>(nn);
     w.dvals_b = cv.take<uint32_t>(nn);
     w.counts = cv.take<uint32_t>(nn);
@@ -468,16 +649,17 @@ static SplatWorkspace carve_splat(Carver& cv, int64_t n, int64_t cap)
     w.pkeys_b = cv.take<uint32_t>(cc);
     w.counter = cv.take<uint32_t>(64);
     w.emit_bounds = cv.take<uint32_t>(cc / kEmitPairs + 2);
-    w.owner_rects = cv.take<uint2>(nn);
+    w.owner_reach = reinterpret_cast<ReachRec*>(cv.take<char>(nn * rec));
+    w.owner_rects = reinterpret_cast<uint2*>(w.owner_reach);
     w.sort_scratch = cv.take<char>(radix_scratch_bytes(std::max(nn, cc)));
     w.scan_scratch = cv.take<char>(scan_scratch_bytes(nn));
     return w;
 }
 
-size_t splat_workspace_bytes(int64_t n, int64_t cap)
+size_t splat_workspace_bytes(int64_t n, int64_t cap, bool cull)
 {
     Carver cv(nullptr, 0);
-    carve_splat(cv, n, cap);
+    carve_splat(cv, n, cap, cull);
     return cv.used;
 }
 
@@ -536,15 +718,74 @@ void splat_blend_backward(const dgmr_tiling& t, int n_tiles, hipStream_t st, con
         launch_blend_bwd<32, 8>(t.batch, n_tiles, st, ranges, pl, xy2, co4, rgb, W, H, gx, bg, fb, T, gfb, sg);
 }
 
+struct EmitArgs {
+    int64_t n;
+    unsigned blocks_n;
+    uint32_t cap;
+    int64_t n_tiles;
+    int tile_bits;
+    const uint32_t* sorted_ids;
+    const int4* views;
+    int64_t n_per_view;
+    int n_views;
+    int tile_w, tile_h;
+    bool cull;
+};
+
+// emission, pair sort on the tile bits, tile ranges; pair keys of type KeyT
+template <class KeyT>
+static int emit_sort_ranges(const EmitArgs& a, const SplatWorkspace& w, dgmr_binning* bins, PendingCount* pending, hipStream_t st)
+{
+    // emit into the buffer pair that leaves the sorted ids in bins->point_list
+    const int passes = radix_passes(0, a.tile_bits);
+    KeyT* ka = reinterpret_cast<KeyT*>(w.pkeys_a);
+    KeyT* kb = reinterpret_cast<KeyT*>(w.pkeys_b);
+    SortBuffersT<KeyT> pb;
+    if (passes % 2 == 0)
+        pb = SortBuffersT<KeyT> { ka, bins->point_list, kb, w.pvals_a };
+    else
+        pb = SortBuffersT<KeyT> { ka, w.pvals_a, kb, bins->point_list };
+    const unsigned emit_blocks = unsigned(ceil_div64(a.cap, kEmitPairs));
+    {
+        StageScope sc("bin.emit", st);
+        pending->add_stage(sc.slot, 4.0 + sizeof(KeyT), 28.0 * double(a.n));
+        k_emit_bounds<<<a.blocks_n, 256, 0, st>>>(a.n, w.pair_offsets, w.counter, a.cap, w.emit_bounds);
+        DGMR_HIP_CHECK(hipGetLastError());
+        if (a.cull)
+            k_emit_reach<KeyT><<<emit_blocks, kEmitThreads, 0, st>>>(w.counter, a.cap, a.sorted_ids, w.emit_bounds, w.pair_offsets,
+                w.owner_reach, a.tile_w, a.tile_h, pb.keys_a, pb.vals_a);
+        else
+            k_emit_pairs<KeyT><<<emit_blocks, kEmitThreads, 0, st>>>(a.n, w.counter, a.cap, a.sorted_ids, w.emit_bounds, w.pair_offsets,
+                w.owner_rects, pb.keys_a, pb.vals_a);
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
+    {
+        StageScope sc("bin.pair_sort", st);
+        pending->add_stage(sc.slot, (8.0 + 3.0 * sizeof(KeyT)) * passes, 0.0);
+        DGMR_HIP_CHECK(radix_sort_pairs(pb, a.cap, w.counter, 0, a.tile_bits, w.sort_scratch, st));
+    }
+    const KeyT* sorted_keys = (passes % 2 == 0) ? pb.keys_a : pb.keys_b;
+    const uint32_t* sorted_vals = (passes % 2 == 0) ? pb.vals_a : pb.vals_b;
+    {
+        StageScope sc("bin.ranges", st);
+        pending->add_stage(sc.slot, 0.0, 8.0 * double(a.n_tiles));
+        k_tile_ranges<KeyT><<<unsigned(ceil_div64(a.n_tiles, 256)), 256, 0, st>>>(sorted_keys, sorted_vals, w.counter, a.cap, a.views,
+            a.n_views, a.n_per_view, a.n_tiles, bins->ranges);
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
+    return DGMR_OK;
+}
+
 // binning shared by the splatter and the marcher: depth sort, scan, emission,
 // pair sort, ranges.  depth_keys/ids are the workspace's A buffers, filled by
 // the preprocess.  Returns the pair count through *n_pairs (host).
-int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, bool bin_rects_written, int grid_x,
-    int grid_y, int tile_w, int tile_h, dgmr_binning* bins, Carver& cv, PendingCount* pending, hipStream_t st, const BatchBinning* batch)
+int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const void* xy, bool xy_double, bool bin_rects_written, bool cull,
+    int grid_x, int grid_y, int tile_w, int tile_h, dgmr_binning* bins, Carver& cv, PendingCount* pending, hipStream_t st,
+    const BatchBinning* batch)
 {
     const int4* views = batch ? batch->views : nullptr;
     const int64_t n_per_view = batch ? batch->n_per_view : 0;
-    SplatWorkspace w = carve_splat(cv, n, bins->pair_capacity);
+    SplatWorkspace w = carve_splat(cv, n, bins->pair_capacity, cull);
     // stable sort of Gaussian ids by depth bits.  Large sorts first keep only
     // the visible entries (culled ones carry the key 0xffffffff and would sort
     // last anyway), in order, so the sort and everything after it sees fewer
@@ -570,9 +811,11 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     if (batch && batch->n_views > 1) {
         // stable pass on the view index: order becomes (view, depth bits, id)
         StageScope sc("bin.view_sort", st, 16.0 * double(n));
-        k_view_keys<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_sorted, V, n_per_view, K);
+        // with tight culling the culled entries sort last, under the key n_views
+        const uint32_t culled_view = cull ? uint32_t(batch->n_views) : 0xffffffffu;
+        k_view_keys<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_sorted, V, n_per_view, culled_view, K);
         DGMR_HIP_CHECK(hipGetLastError());
-        const int vbits = int(higher_msb(uint32_t(batch->n_views - 1)));
+        const int vbits = int(higher_msb(cull ? uint32_t(batch->n_views) : uint32_t(batch->n_views - 1)));
         DGMR_HIP_CHECK(radix_sort_pairs(SortBuffers { K, V, K2, V2 }, n, n_sorted, 0, vbits, w.sort_scratch, st));
         if (radix_passes(0, vbits) % 2 == 1) {
             std::swap(K, K2);
@@ -583,7 +826,10 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     const unsigned blocks_n = unsigned(ceil_div64(n, 256));
     {
         StageScope sc("bin.scan", st, 44.0 * double(n));
-        if (bin_rects_written)
+        if (cull)
+            k_gather_reach<<<blocks_n, 256, 0, st>>>(n, n_sorted, K, batch && batch->n_views > 1 ? uint32_t(batch->n_views) : 0xffffffffu,
+                sorted_ids, w.reach, tile_w, tile_h, w.counts, w.owner_reach);
+        else if (bin_rects_written)
             k_gather_owners<float, true><<<blocks_n, 256, 0, st>>>(n, n_sorted, sorted_ids, w.bin_rects, rects, tiles, static_cast<const float*>(xy),
                 grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
         else if (xy_double)
@@ -608,54 +854,27 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
         DGMR_HIP_CHECK(hipMemsetAsync(bins->ranges, 0, size_t(n_tiles) * 2 * sizeof(uint32_t), st));
         return DGMR_OK;
     }
-    // emit into the buffer pair that leaves the sorted ids in bins->point_list
+    // view-local tile ids as keys: 16 bits when they fit (a quarter less
+    // traffic in emission and sort), else 32
     const int tile_bits = int(higher_msb(uint32_t(batch ? batch->max_view_tiles : n_tiles)));
-    const int passes = radix_passes(0, tile_bits);
-    SortBuffers pb;
-    if (passes % 2 == 0)
-        pb = SortBuffers { w.pkeys_a, bins->point_list, w.pkeys_b, w.pvals_a };
-    else
-        pb = SortBuffers { w.pkeys_a, w.pvals_a, w.pkeys_b, bins->point_list };
-    const unsigned emit_blocks = unsigned(ceil_div64(cap, kEmitPairs));
-    {
-        StageScope sc("bin.emit", st);
-        pending->add_stage(sc.slot, 8.0, 28.0 * double(n));
-        k_emit_bounds<<<blocks_n, 256, 0, st>>>(n, w.pair_offsets, w.counter, uint32_t(cap), w.emit_bounds);
-        DGMR_HIP_CHECK(hipGetLastError());
-        k_emit_pairs<<<emit_blocks, kEmitThreads, 0, st>>>(n, w.counter, uint32_t(cap), sorted_ids, w.emit_bounds, w.pair_offsets,
-            w.owner_rects, pb.keys_a, pb.vals_a);
-        DGMR_HIP_CHECK(hipGetLastError());
-    }
-    {
-        StageScope sc("bin.pair_sort", st);
-        pending->add_stage(sc.slot, 20.0 * passes, 0.0);
-        DGMR_HIP_CHECK(radix_sort_pairs(pb, cap, w.counter, 0, tile_bits, w.sort_scratch, st));
-    }
-    const uint32_t* sorted_keys = (passes % 2 == 0) ? pb.keys_a : pb.keys_b;
-    const uint32_t* sorted_vals = (passes % 2 == 0) ? pb.vals_a : pb.vals_b;
-    {
-        StageScope sc("bin.ranges", st);
-        pending->add_stage(sc.slot, 0.0, 8.0 * double(n_tiles));
-        k_tile_ranges<<<unsigned(ceil_div64(n_tiles, 256)), 256, 0, st>>>(sorted_keys, sorted_vals, w.counter, uint32_t(cap), views,
-            batch ? int(batch->n_views) : 0, n_per_view, n_tiles, bins->ranges);
-        DGMR_HIP_CHECK(hipGetLastError());
-    }
-    return DGMR_OK;
+    EmitArgs ea { n, blocks_n, uint32_t(cap), n_tiles, tile_bits, sorted_ids, views, n_per_view, batch ? int(batch->n_views) : 0, tile_w, tile_h,
+        cull };
+    if (tile_bits <= 16)
+        return emit_sort_ranges<uint16_t>(ea, w, bins, pending, st);
+    return emit_sort_ranges<uint32_t>(ea, w, bins, pending, st);
 }
 
-size_t binning_workspace_bytes(int64_t n, int64_t cap) { return splat_workspace_bytes(n, cap); }
-
-// depth keys / ids buffers the preprocess writes (the first two slices of the workspace)
-void binning_key_buffers(void* workspace, int64_t n, uint32_t** keys, uint32_t** ids, uint2** bin_rects)
+// depth keys / ids buffers the preprocess writes and its per-Gaussian
+// binning records (the first three slices of carve_splat)
+void binning_key_buffers(void* workspace, int64_t n, bool cull, uint32_t** keys, uint32_t** ids, void** records)
 {
-    // the first three slices of carve_splat
+    // these slices come first, so they do not depend on the pair capacity
     Carver cv(workspace, ~size_t(0));
-    const int64_t nn = std::max<int64_t>(n, 1);
-    *keys = cv.take<uint32_t>(nn);
-    *ids = cv.take<uint32_t>(nn);
-    uint2* r = cv.take<uint2>(nn);
-    if (bin_rects)
-        *bin_rects = r;
+    const SplatWorkspace w = carve_splat(cv, n, 1, cull);
+    *keys = w.dkeys_a;
+    *ids = w.dvals_a;
+    if (records)
+        *records = w.reach;
 }
 
 } // namespace dgmr
@@ -667,7 +886,7 @@ extern "C" {
 int dgmr_workspace_size(int64_t n, int32_t width, int32_t height, const dgmr_tiling* tiling, int64_t pair_capacity, size_t* bytes)
 {
     DGMR_ARG_CHECK(bytes && n >= 0 && valid_image(width, height) && valid_tiling(tiling) && pair_capacity >= 0, "dgmr_workspace_size: bad argument");
-    *bytes = splat_workspace_bytes(n, pair_capacity);
+    *bytes = splat_workspace_bytes(n, pair_capacity, tiling->cull != 0);
     return DGMR_OK;
 }
 
@@ -683,7 +902,8 @@ int dgmr_splat_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dg
     DGMR_ARG_CHECK(valid_image(cam->width, cam->height), "dgmr_splat_forward: bad image size");
     DGMR_ARG_CHECK(bins->point_list && bins->ranges, "dgmr_splat_forward: null binning buffers");
     const int64_t n = g->n;
-    DGMR_ARG_CHECK(workspace_bytes >= splat_workspace_bytes(n, bins->pair_capacity), "dgmr_splat_forward: workspace too small");
+    const bool cull = tiling->cull != 0;
+    DGMR_ARG_CHECK(workspace_bytes >= splat_workspace_bytes(n, bins->pair_capacity, cull), "dgmr_splat_forward: workspace too small");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const DevCamera<float> dc = make_dev_camera<float>(*cam, *tiling);
     const int64_t n_tiles = int64_t(dc.grid_x) * dc.grid_y;
@@ -700,18 +920,18 @@ int dgmr_splat_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const dg
         return DGMR_OK;
     }
     uint32_t *dkeys, *dids;
-    uint2* brects;
-    binning_key_buffers(workspace, n, &dkeys, &dids, &brects);
+    void* recs;
+    binning_key_buffers(workspace, n, cull, &dkeys, &dids, &recs);
     {
     StageScope sc("splat.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs + 75.0));
     k_splat_preprocess<<<unsigned(ceil_div64(n, kPreThreads)), kPreThreads, 0, st>>>(n, make_params(g), dc,
         SplatOut { geom->rects, geom->tiles_touched, geom->xy, reinterpret_cast<float4*>(geom->conic_opacity), geom->rgb, geom->rgb_clamped,
-            geom->depths, dkeys, dids, brects });
+            geom->depths, dkeys, dids, cull ? nullptr : static_cast<uint2*>(recs), cull ? static_cast<ReachRec*>(recs) : nullptr });
     DGMR_HIP_CHECK(hipGetLastError());
     }
     Carver cv(workspace, workspace_bytes);
     PendingCount pending;
-    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, false, true, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv,
+    const int rc = bin_gaussians(n, geom->rects, geom->tiles_touched, geom->xy, false, true, cull, dc.grid_x, dc.grid_y, dc.tile_w, dc.tile_h, bins, cv,
         &pending, st, nullptr);
     if (rc != DGMR_OK)
         return rc;

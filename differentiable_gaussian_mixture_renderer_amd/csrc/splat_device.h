@@ -71,6 +71,107 @@ __device__ __forceinline__ void activation_backward(const SplatParams& p, int64_
     *g_w *= o * (1.f - o);
 }
 
+// ------------------------------------------------------------ tight culling
+// With tight culling a Gaussian is binned only into the tiles of its
+// rectangle that hold a pixel where its alpha can reach 1/255.  The blend
+// skips every (pixel, Gaussian) with alpha < 1/255 in both directions, so the
+// dropped tiles change no image, transmittance or gradient.  alpha >= 1/255
+// needs w exp(-Q/2) >= 1/255 with Q = a u^2 + 2 b u v + c v^2 (u, v: pixel
+// minus centre, (a, b, c) the conic), i.e. Q <= t = 2 ln(255 w).  The region
+// binned is the ellipse Q <= t' = t (1 + 1e-5 k) + 1e-3, where
+// k = (a + c)^2 / (ac - b^2) bounds the blend's relative rounding of Q, and a
+// tile counts when the ellipse meets its pixel box grown by
+// delta = 1/16 + 2^-20 max(|mx|, |my|) pixels (the rounding of the offsets to
+// the centre).  Both allowances only ever add tiles.
+struct __align__(16) ReachRec {
+    uint32_t origin; // (first tile row << 16) | first tile column of the rectangle
+    uint32_t dims;   // (rows << 16) | columns; 0 when nothing is binned
+    uint32_t grid_x; // tile columns of the Gaussian's view
+    float mx, my;    // centre in pixels
+    float A, B, C;   // (a, b, c) / t'; A < 0: the whole rectangle (degenerate conic)
+};
+static_assert(sizeof(ReachRec) == 32, "ReachRec is two 16-byte words");
+
+struct ReachEllipse {
+    float mx, my, A, inv_A, B, det, vmax, vr, delta;
+    bool whole;
+};
+
+// Both the counting and the emission of pairs evaluate the ellipse through
+// these two functions, with contraction off and the hardware reciprocal and
+// square root (an ulp or two: far inside delta), so they agree bit for bit.
+__device__ __forceinline__ ReachEllipse reach_ellipse(const ReachRec& r)
+{
+#pragma clang fp contract(off)
+    ReachEllipse e;
+    e.mx = r.mx;
+    e.my = r.my;
+    e.A = r.A;
+    e.inv_A = __builtin_amdgcn_rcpf(r.A);
+    e.B = r.B;
+    e.whole = !(r.A > 0.f);
+    e.det = r.A * r.C - r.B * r.B;
+    e.vmax = __builtin_amdgcn_sqrtf(r.A * __builtin_amdgcn_rcpf(e.det));                // |v| of the top and bottom
+    e.vr = -r.B * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(r.C * e.det));           // v of the rightmost point (leftmost: -vr)
+    e.delta = 0.0625f + fmaxf(fabsf(r.mx), fabsf(r.my)) * 0x1p-20f;
+    return e;
+}
+
+// tile columns [x, y] of tile row ty (within the rectangle's columns
+// [x0, x0 + cols)) whose pixel boxes the ellipse meets; x > y when none.
+// Across the row's band of pixel rows the reached x range is an interval:
+// its right end is the concave u_R(v) = (-B v + sqrt(A - det v^2)) / A at the
+// band's clamp of the rightmost point, its left end likewise.
+__device__ __forceinline__ int2 reach_columns(const ReachEllipse& e, int ty, int x0, int cols, int tw, int th)
+{
+#pragma clang fp contract(off)
+    if (e.whole)
+        return make_int2(x0, x0 + cols - 1);
+    const float vb = float(ty * th) - e.my;
+    const float lo = fmaxf(vb - e.delta, -e.vmax), hi = fminf((vb + float(th - 1)) + e.delta, e.vmax);
+    if (!(lo <= hi))
+        return make_int2(1, 0);
+    const float vR = fminf(fmaxf(e.vr, lo), hi), vL = fminf(fmaxf(-e.vr, lo), hi);
+    const float sR = __builtin_amdgcn_sqrtf(fmaxf(e.A - e.det * (vR * vR), 0.f));
+    const float sL = __builtin_amdgcn_sqrtf(fmaxf(e.A - e.det * (vL * vL), 0.f));
+    const float xR = e.mx + (sR - e.B * vR) * e.inv_A;
+    const float xL = e.mx - (sL + e.B * vL) * e.inv_A;
+    // column tx holds pixels tx * tw .. tx * tw + tw - 1 (tw a power of two: the scaling is exact)
+    const float inv_tw = __builtin_amdgcn_rcpf(float(tw));
+    const float c0 = fmaxf(ceilf(((xL - e.delta) - float(tw - 1)) * inv_tw), float(x0));
+    const float c1 = fminf(floorf((xR + e.delta) * inv_tw), float(x0 + cols - 1));
+    if (!(c0 <= c1))
+        return make_int2(1, 0);
+    return make_int2(int(c0), int(c1));
+}
+
+// the record of a visible Gaussian: rectangle x0..x0+cols, y0..y0+rows; conic (a, b, c), weight w
+__device__ __forceinline__ ReachRec make_reach(int x0, int y0, int cols, int rows, int grid_x, float mx, float my, float a, float b, float c,
+    float w)
+{
+    ReachRec r { (uint32_t(y0) << 16) | uint32_t(x0), (uint32_t(rows) << 16) | uint32_t(cols), uint32_t(grid_x), mx, my, 0.f, 0.f, 0.f };
+    // 2 ln(255 w) from the hardware log2 (error ~1e-7, inside the 1e-3 allowance)
+    const float t = 2.f * 0.6931471805599453f * __builtin_amdgcn_logf(255.f * w);
+    if (!(t > 0.f)) {
+        r.dims = 0; // alpha < 1/255 everywhere
+        return r;
+    }
+    const float det = a * c - b * b;
+    const float k = ((a + c) * (a + c)) * __builtin_amdgcn_rcpf(det);
+    const float tp = (t + t * 1e-5f * k) + 1e-3f;
+    if (!(det > 0.f) || !(tp < 3.0e38f) || !(a > 0.f) || !(c > 0.f)) {
+        r.A = -1.f;
+        return r;
+    }
+    const float inv_tp = __builtin_amdgcn_rcpf(tp);
+    r.A = a * inv_tp;
+    r.B = b * inv_tp;
+    r.C = c * inv_tp;
+    if (!(r.A > 0.f) || !(r.C > 0.f) || !(r.A * r.C - r.B * r.B > 0.f))
+        r.A = -1.f;
+    return r;
+}
+
 // per-Gaussian outputs of the preprocess; index o = view * n + i
 struct SplatOut {
     int32_t* rects;
@@ -83,6 +184,7 @@ struct SplatOut {
     uint32_t* depth_keys;
     uint32_t* ids;
     uint2* bin_rects; // (first tile, rows << 16 | columns) for the binning
+    ReachRec* reach;  // tight culling: written instead of bin_rects (null otherwise)
 };
 
 // host side of the splatter preprocess, shared with the rasteriser
@@ -126,7 +228,8 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     out.xy[2 * o + 1] = 0.f;
     out.depth_keys[o] = 0xffffffffu;
     out.ids[o] = uint32_t(o);
-    out.bin_rects[o] = make_uint2(0u, 0u);
+    if (!out.reach) // reach records are read only for entries with a depth key
+        out.bin_rects[o] = make_uint2(0u, 0u);
     const float mu[3] = { p.means[3 * i], p.means[3 * i + 1], p.means[3 * i + 2] };
     float t[3];
     transform_point(cam.view, mu, t, 3);
@@ -149,7 +252,8 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     if (touched == 0)
         return;
     out.tiles[o] = touched;
-    out.bin_rects[o] = make_uint2(uint32_t(y0) * uint32_t(cam.grid_x) + uint32_t(x0), (uint32_t(y1 - y0) << 16) | uint32_t(x1 - x0));
+    if (!out.reach)
+        out.bin_rects[o] = make_uint2(uint32_t(y0) * uint32_t(cam.grid_x) + uint32_t(x0), (uint32_t(y1 - y0) << 16) | uint32_t(x1 - x0));
     out.xy[2 * o] = xy[0];
     out.xy[2 * o + 1] = xy[1];
     const float dv[3] = { cam.pos[0] - mu[0], cam.pos[1] - mu[1], cam.pos[2] - mu[2] };
@@ -186,7 +290,14 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     const float det2 = cov_f[0] * cov_f[2] - cov_f[1] * cov_f[1];
     const float i2 = float(2 * kPi) * sqrt_rn(det2);
     const float weight = w * det3(in.J) * i2prime / i2;
-    out.conic_opacity[o] = make_float4(cov_f[2] / det2, -cov_f[1] / det2, cov_f[0] / det2, weight);
+    const float4 co = make_float4(cov_f[2] / det2, -cov_f[1] / det2, cov_f[0] / det2, weight);
+    out.conic_opacity[o] = co;
+    if (out.reach) {
+        const ReachRec r = make_reach(x0, y0, x1 - x0, y1 - y0, cam.grid_x, xy[0], xy[1], co.x, co.y, co.z, co.w);
+        uint4* d = reinterpret_cast<uint4*>(out.reach + o);
+        d[0] = make_uint4(r.origin, r.dims, r.grid_x, __float_as_uint(r.mx));
+        d[1] = make_uint4(__float_as_uint(r.my), __float_as_uint(r.A), __float_as_uint(r.B), __float_as_uint(r.C));
+    }
 }
 
 // ------------------------------------------------------------------ blending

@@ -26,6 +26,10 @@ from .. import _native as N
 
 # tile shape and LDS batch used when the caller passes none
 DEFAULT_TILING = ((16, 16), 64)
+# binning used when the caller passes none: False keeps the reference's tile
+# rectangles (its b_point_list / i_ranges); True bins only the tiles where
+# alpha can reach 1/255 (same image and gradients, fewer pairs)
+DEFAULT_CULL = False
 
 # cache layout returned by forward (index: content)
 CACHE_FIELDS = ("framebuffer", "rects", "rgb", "rgb_sh_clamped", "depths", "points_xy_image", "conic_opacity",
@@ -34,7 +38,12 @@ CACHE_FIELDS = ("framebuffer", "rects", "rgb", "rgb_sh_clamped", "depths", "poin
 
 def _forward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, proj_matrix, cam_position, background,
              render_width: int, render_height: int, tan_fovx: float, tan_fovy: float, sh_degree: int,
-             tile=None, batch: int = 0, scale_multiplier: float = 1.0, activation: int = N.DGMR_ACT_NONE):
+             tile=None, batch: int = 0, scale_multiplier: float = 1.0, activation: int = N.DGMR_ACT_NONE, cull=None):
+    """cull=True bins each Gaussian only into the tiles where its alpha can
+    reach 1/255 (dgmr_tiling.cull): same image, transmittance and gradients,
+    fewer pairs; b_point_list and i_ranges then differ from the reference's.
+    None: DEFAULT_CULL."""
+    cull = DEFAULT_CULL if cull is None else bool(cull)
     tile = tile or DEFAULT_TILING[0]
     batch = batch or DEFAULT_TILING[1]
     shs, weights, centroids, cov_scales, cov_rotations = C.prepare_inputs(shs, weights, centroids, cov_scales, cov_rotations, True)
@@ -42,7 +51,7 @@ def _forward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, pr
     n = centroids.shape[0]
     W, H = int(render_width), int(render_height)
     lib = N.load()
-    tiling = C.tiling_struct(tile, batch)
+    tiling = C.tiling_struct(tile, batch, cull)
     cam = C.camera_struct(view_matrix, proj_matrix, cam_position, background, W, H, tan_fovx, tan_fovy)
     g = C.gaussians_struct(shs, weights, centroids, cov_scales, cov_rotations, sh_degree, scale_multiplier, activation)
     gx, gy = (W + tiling.tile_w - 1) // tiling.tile_w, (H + tiling.tile_h - 1) // tiling.tile_h
@@ -87,7 +96,8 @@ def _forward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, pr
 
 def _backward(shs, weights, centroids, cov_scales, cov_rotations, view_matrix, proj_matrix, cam_position, background,
               render_width: int, render_height: int, tan_fovx: float, tan_fovy: float, sh_degree: int, grad_framebuffer, cache,
-              tile=None, batch: int = 0, scale_multiplier: float = 1.0, activation: int = N.DGMR_ACT_NONE):
+              tile=None, batch: int = 0, scale_multiplier: float = 1.0, activation: int = N.DGMR_ACT_NONE, cull=None):
+    del cull  # the backward walks whatever lists the forward binned
     tile = tile or DEFAULT_TILING[0]
     batch = batch or DEFAULT_TILING[1]
     shs, weights, centroids, cov_scales, cov_rotations = C.prepare_inputs(shs, weights, centroids, cov_scales, cov_rotations, True)

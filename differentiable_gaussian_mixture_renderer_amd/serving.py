@@ -102,12 +102,16 @@ def make_trace(n_requests: int, rate_hz: float, n_scenes: int, seed: int, backwa
     return reqs
 
 
-def l1_grad(img: torch.Tensor, seed: int) -> torch.Tensor:
-    """dL/dimg of L = mean |img - target| for a seeded uniform-noise target."""
+def l1_target(img: torch.Tensor, seed: int) -> torch.Tensor:
+    """The seeded uniform-noise target image of a gradient request."""
     g = torch.Generator(device=img.device)
     g.manual_seed(seed)
-    target = torch.rand(img.shape, generator=g, device=img.device, dtype=img.dtype)
-    return losses.l1_grad(img, target)
+    return torch.rand(img.shape, generator=g, device=img.device, dtype=img.dtype)
+
+
+def l1_grad(img: torch.Tensor, seed: int) -> torch.Tensor:
+    """dL/dimg of L = mean |img - target| for a seeded uniform-noise target."""
+    return losses.l1_grad(img, l1_target(img, seed))
 
 
 def estimate_bytes(scene: ResidentScene, reqs: List[Request]) -> int:
@@ -171,7 +175,8 @@ class ContinuousBatchingServer:
                 imgs, _ = splatter.forward(*sc.params, [r.camera for r in batch], sh_degree=sc.sh_degree)
                 grads = [None] * len(batch)
                 if n_bwd:
-                    gimg = [l1_grad(img, r.target_seed) if r.backward else None for img, r in zip(imgs, batch)]
+                    gimg = losses.l1_grad_many(imgs[:n_bwd], [l1_target(img, r.target_seed) for img, r in zip(imgs, batch[:n_bwd])])
+                    gimg += [None] * (len(batch) - n_bwd)
                     groups = [k if k < n_bwd else -1 for k in range(len(batch))]
                     grads[:n_bwd] = splatter.backward(gimg, groups=groups)
                 stream.synchronize()

@@ -72,10 +72,20 @@ typedef struct dgmr_camera {
 } dgmr_camera;
 
 /* Tile shape and LDS batching.  The reference fixes 16x16 tiles
- * (constants.h:45-46) and 256-Gaussian batches (splat_forward.cu:268-270). */
+ * (constants.h:45-46) and 256-Gaussian batches (splat_forward.cu:268-270).
+ * cull = 0 bins each Gaussian into its whole 3-sigma tile rectangle, as the
+ * reference does (util::getRect, util.h:42; splat_forward.cu:100, 173), so
+ * point_list and ranges are the reference's.  cull = 1 (splatter only; no
+ * counterpart in the reference) bins it only into the tiles holding a pixel
+ * where its alpha can reach 1/255: the blend skips every other (pixel,
+ * Gaussian), so images, transmittances and gradients are unchanged while
+ * about a fifth fewer pairs are emitted, sorted and blended; point_list and
+ * ranges then differ from the reference's.  The marcher and the rasteriser
+ * take cull = 0 only. */
 typedef struct dgmr_tiling {
     int32_t tile_w, tile_h;   /* 8x8, 16x16 or 32x8 */
     int32_t batch;            /* Gaussians staged in LDS per round: 64, 128 or 256 */
+    int32_t cull;             /* 0: the reference's rectangles; 1: tiles reaching alpha 1/255 */
 } dgmr_tiling;
 
 typedef struct dgmr_stats {
@@ -263,6 +273,22 @@ int dgmr_l1_partials_size(int64_t n, int32_t* count);
 int dgmr_l1_loss_forward(const float* image, const float* target, int64_t n, float scale, float* partials, dgmr_stream stream);
 int dgmr_l1_loss_backward(const float* image, const float* target, int64_t n, float scale, const float* grad_loss, float* grad,
     dgmr_stream stream);
+
+/* The backward over many images in one launch (the views of a batch): item k
+ * writes items[k].grad = sign(image - target) * scale * grad_loss[0], as
+ * dgmr_l1_loss_backward does for one image.  The descriptors are host memory;
+ * the arrays they point to are device memory, float32, 16-byte aligned. */
+typedef struct dgmr_l1_item {
+    const float* image;
+    const float* target;
+    float* grad;
+    const float* grad_loss; /* device scalar dL/dloss */
+    int64_t n;
+    float scale;
+    int32_t reserved;
+} dgmr_l1_item;
+
+int dgmr_l1_loss_backward_many(const dgmr_l1_item* items, int32_t n_items, dgmr_stream stream);
 
 /* ------------------------------------------------------------ sort
  * Stable LSD radix sort of n (uint32 key, uint32 value) pairs on key bits

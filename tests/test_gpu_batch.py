@@ -24,8 +24,11 @@ def scene_and_cams(n=20_000, n_cams=6, seed=4):
     return scene, cams
 
 
-@pytest.mark.parametrize("tile,batch", [((16, 16), 64), ((8, 8), 128), ((32, 8), 256)])
-def test_batch_matches_single_views(tile, batch):
+@pytest.mark.parametrize("tile,batch,cull", [((16, 16), 64, False), ((8, 8), 128, False), ((32, 8), 256, False),
+                                             ((16, 16), 64, True), ((8, 8), 256, True)])
+def test_batch_matches_single_views(tile, batch, cull):
+    """cull=True (tight culling in the batch) against the plain path with the
+    reference binning: the same images and gradients."""
     torch = _torch()
     from differentiable_gaussian_mixture_renderer_amd.batching import BatchSplatter, render_plain
     scene, cams = scene_and_cams()
@@ -33,8 +36,8 @@ def test_batch_matches_single_views(tile, batch):
     t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
     params = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations)]
     bg = (0.1, 0.2, 0.3)
-    print(f"[gpu batch] {len(cams)} views tile={tile} batch={batch}", flush=True)
-    bs = BatchSplatter(tile=tile, batch=batch)
+    print(f"[gpu batch] {len(cams)} views tile={tile} batch={batch} cull={cull}", flush=True)
+    bs = BatchSplatter(tile=tile, batch=batch, cull=cull)
     fbs, Ts = bs.forward(*params, cams, bg=bg)
     rng = np.random.default_rng(1)
     grads = [torch.tensor(rng.normal(size=(3, c.height, c.width)).astype(np.float32) / (3 * c.height * c.width), device=dev) for c in cams]
@@ -78,18 +81,20 @@ def test_gradient_groups_match_per_group_plain_sums():
             assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b) + 1e-12, k
 
 
-def test_large_batch_with_compacted_depth_sort():
+@pytest.mark.parametrize("cull", [False, True])
+def test_large_batch_with_compacted_depth_sort(cull):
     """40 views of 120k Gaussians (4.8M view-Gaussian entries): the batched
     binning compacts the visible entries before its depth sort; images and
-    gradients must still equal the one-camera-per-launch path's."""
+    gradients must still equal the one-camera-per-launch path's (reference
+    binning), with either binning in the batch."""
     torch = _torch()
     from differentiable_gaussian_mixture_renderer_amd.batching import BatchSplatter, render_plain
     scene, cams = scene_and_cams(n=120_000, n_cams=40, seed=6)
     dev = torch.device("cuda:0")
     t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
     params = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations)]
-    print(f"[gpu batch] {len(cams)} views x {scene.n} Gaussians (compacted depth sort)", flush=True)
-    bs = BatchSplatter()
+    print(f"[gpu batch] {len(cams)} views x {scene.n} Gaussians (compacted depth sort) cull={cull}", flush=True)
+    bs = BatchSplatter(cull=cull)
     fbs, _ = bs.forward(*params, cams)
     rng = np.random.default_rng(3)
     grads = [torch.tensor(rng.normal(size=(3, c.height, c.width)).astype(np.float32) / (3 * c.height * c.width), device=dev) for c in cams]
@@ -102,3 +107,29 @@ def test_large_batch_with_compacted_depth_sort():
     for a, b in zip(gb, gp):
         a, b = a.cpu().numpy(), b.cpu().numpy()
         assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b) + 1e-12
+
+
+def test_step_l1_equals_forward_loss_backward():
+    """BatchSplatter.step_l1 (buffers prepared before the forward's wait) is
+    forward + losses.l1_grad_many + backward: the same images, the same
+    gradients up to the order of the atomic sums."""
+    torch = _torch()
+    from differentiable_gaussian_mixture_renderer_amd import losses
+    from differentiable_gaussian_mixture_renderer_amd.batching import BatchSplatter
+    scene, cams = scene_and_cams(seed=9)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    params = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations)]
+    gen = torch.Generator(device=dev).manual_seed(5)
+    targets = [torch.rand((3, c.height, c.width), generator=gen, device=dev) for c in cams]
+    print(f"[gpu batch] step_l1 over {len(cams)} views", flush=True)
+    bs = BatchSplatter()
+    fbs, _ = bs.forward(*params, cams)
+    ref = bs.backward(losses.l1_grad_many(fbs, targets))
+    imgs, got = BatchSplatter().step_l1(*params, cams, targets)
+    torch.cuda.synchronize()
+    for a, b in zip(imgs, fbs):
+        assert torch.equal(a, b)
+    for a, b in zip(got, ref):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b) + 1e-12

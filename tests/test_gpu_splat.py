@@ -28,7 +28,7 @@ def _torch():
     return torch
 
 
-def _run_gpu(scene, cam, bg, tile=(16, 16), batch=64, grad_fb=None):
+def _run_gpu(scene, cam, bg, tile=(16, 16), batch=64, grad_fb=None, cull=False):
     torch = _torch()
     from differentiable_gaussian_mixture_renderer_amd.python_bindings import splatter
     dev = torch.device("cuda:0")
@@ -36,8 +36,8 @@ def _run_gpu(scene, cam, bg, tile=(16, 16), batch=64, grad_fb=None):
     args = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations)]
     cam_args = [t(cam.world_view_transform), t(cam.full_proj_transform), t(cam.camera_center), t(bg), cam.width, cam.height,
                 cam.tan_fovx, cam.tan_fovy, scene.sh_degree]
-    print(f"[gpu splat] n={scene.n} {cam.width}x{cam.height} tile={tile} batch={batch}", flush=True)
-    cache = splatter.cpp_binding.forward(*args, *cam_args, tile=tile, batch=batch)
+    print(f"[gpu splat] n={scene.n} {cam.width}x{cam.height} tile={tile} batch={batch} cull={cull}", flush=True)
+    cache = splatter.cpp_binding.forward(*args, *cam_args, tile=tile, batch=batch, cull=cull)
     out = {k: v.cpu().numpy() for k, v in zip(splatter.CACHE_FIELDS, cache)}
     if grad_fb is not None:
         grads = splatter.cpp_binding.backward(*args, *cam_args, t(grad_fb.astype(np.float32)), cache, tile=tile, batch=batch)
@@ -100,6 +100,53 @@ def test_tilings_match_oracle_of_same_tiling(tile, batch):
     fb64, _, amb, _ = ref64.render(bg)
     ok = amb == 0
     assert np.abs(out["framebuffer"] - fb64).max(axis=0)[ok].max() <= IMG_TOL
+
+
+def test_binning_with_32bit_tile_keys():
+    """2304x2304 with 8x8 tiles is 82 944 tiles: pair keys need 17 bits, so
+    the binning takes its 32-bit key path (16-bit otherwise); bit-exact
+    against the oracle."""
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    import math
+    scene = scenes.make_scene(8_000, seed=21, extent=1.0, sh_mode="rgb")
+    t = math.tan(math.radians(30.0))
+    cam = scenes.look_at_camera([0.0, 0.0, -4.0], [0.0, 0.0, 0.0], 2304, 2304, t, t)
+    out = _run_gpu(scene, cam, np.zeros(3, np.float32), tile=(8, 8), batch=64)
+    ref32 = O.SplatOracle(O.scene_arrays(scene), cam, dbl=False, tile=(8, 8))
+    assert ref32.n_pairs > 0
+    _check_binning(out, ref32)
+
+
+@pytest.mark.parametrize("tile,batch", [((16, 16), 64), ((8, 8), 128), ((32, 8), 256)])
+def test_tight_culling_same_image_fewer_pairs(tile, batch):
+    """cull=1 against the reference binning on the same inputs: image and
+    transmittance bit-identical, gradients equal up to the order of the
+    atomic sums (max 1e-4 of the largest, L2 1e-5: sums that cancel
+    amplify the order's rounding); every tile's list an ordered sublist of the
+    reference's, no shorter than the oracle's count of Gaussians whose alpha
+    reaches 1/255 at some pixel of that tile, and at least 5 % fewer pairs."""
+    scene, cam = _scene_sh3(n=30_000, w=352, h=200)
+    bg = np.array([0.3, 0.1, 0.2], np.float32)
+    grad = np.random.default_rng(3).uniform(-1, 1, (3, cam.height, cam.width)) / (3 * cam.height * cam.width)
+    ref = _run_gpu(scene, cam, bg, tile=tile, batch=batch, grad_fb=grad)
+    got = _run_gpu(scene, cam, bg, tile=tile, batch=batch, grad_fb=grad, cull=True)
+    assert np.array_equal(got["framebuffer"], ref["framebuffer"])
+    assert np.array_equal(got["final_transparency"], ref["final_transparency"])
+    for name, a in ref["grads"].items():
+        d = got["grads"][name] - a
+        print(f"{name}: max {np.abs(d).max() / np.abs(a).max():.3g}, l2 {np.linalg.norm(d) / np.linalg.norm(a):.3g}")
+        assert np.abs(d).max() <= 1e-4 * np.abs(a).max() and np.linalg.norm(d) <= 1e-5 * np.linalg.norm(a), name
+    _, _, reach = O.SplatOracle(O.scene_arrays(scene), cam, dbl=False, tile=tile).tile_work(reachable=True)
+    ra, rb = ref["i_ranges"].view(np.uint32), got["i_ranges"].view(np.uint32)
+    pa, pb = ref["b_point_list"].view(np.uint32), got["b_point_list"].view(np.uint32)
+    for t in range(ra.shape[0]):
+        la, lb = pa[ra[t, 0]:ra[t, 1]], pb[rb[t, 0]:rb[t, 1]]
+        assert len(lb) >= reach[t], t
+        pos = {int(g): i for i, g in enumerate(la)}
+        at = [pos.get(int(g), -1) for g in lb]
+        assert min(at, default=0) >= 0 and all(x < y for x, y in zip(at, at[1:])), t
+    print(f"pairs: reference {len(pa)}, tight {len(pb)}, reachable {int(reach.sum())}")
+    assert len(pb) <= 0.95 * len(pa)
 
 
 @pytest.mark.parametrize("which", ["small", "sh3"])
