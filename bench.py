@@ -184,7 +184,10 @@ def main():
     d_avg_ms = d_ms / max(d_launches, 1)
     achieved = (d_bytes / max(d_launches, 1)) / (d_avg_ms * 1e-3) / 1e9
     p_in = 44 + 12 * 16
-    frame_bytes = args.n * (3 * p_in + 160) + 124 * R + 44 * W * H + 8 * ((W + 15) // 16) * ((H + 15) // 16)
+    # per pair: emission, two pair-sort passes and both blends; 16-bit pair keys
+    # (every view here has fewer than 65 536 tiles) save 2 bytes in the emission
+    # and 6 in each sort pass of the 124 that 32-bit keys move
+    frame_bytes = args.n * (3 * p_in + 160) + 110 * R + 44 * W * H + 8 * ((W + 15) // 16) * ((H + 15) // 16)
     frame_hbm_frac = frame_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     traffic = None
