@@ -153,7 +153,7 @@ __device__ __forceinline__ void splat_ots_grad(S w, const S* mu, const S* s, con
     S g_weight, const S* g_xy, const S* g_cov2, S* g_w, S* g_mu, S* g_s, S* g_q)
 {
     SplatIn<S> in;
-    splat_internals(mu, s, q, cam, &in);
+    splat_internals<S, true>(mu, s, q, cam, &in);
     const S cov_f[3] = { in.cov2[0] + filter, in.cov2[1], in.cov2[2] + filter };
     const S detJ = det3(in.J);
     int i0, i1;
@@ -161,13 +161,13 @@ __device__ __forceinline__ void splat_ots_grad(S w, const S* mu, const S* s, con
     larger2_values(s, &sa, &sb, &i0, &i1);
     const S num = S(2 * kPi) * sa * sb;
     const S det2 = cov_f[0] * cov_f[2] - cov_f[1] * cov_f[1];
-    const S sq = sqrt(det2);
-    const S i2 = S(2 * kPi) * sq;
-    *g_w = g_weight * detJ * num / i2;
-    const S g_det = g_weight * w * num / i2;
-    const S g_num = g_weight * w * detJ / i2;
-    const S g_i2 = -g_weight * w * detJ * num / (i2 * i2);
-    const S f = g_i2 * S(2 * kPi) / (S(2) * sq);
+    const S sq = fast_sqrt(det2);
+    const S inv_i2 = fast_div(S(1), S(2 * kPi) * sq);
+    *g_w = g_weight * detJ * num * inv_i2;
+    const S g_det = g_weight * w * num * inv_i2;
+    const S g_num = g_weight * w * detJ * inv_i2;
+    const S g_i2 = -g_weight * w * detJ * num * (inv_i2 * inv_i2);
+    const S f = g_i2 * S(2 * kPi) * fast_div(S(1), S(2) * sq);
     S g_c2[3] = { g_cov2[0] + f * cov_f[2], g_cov2[1] + f * (S(-2) * cov_f[1]), g_cov2[2] + f * cov_f[0] };
     const S g_a = g_num * S(2 * kPi) * sb, g_b = g_num * S(2 * kPi) * sa;
     const S gs[3] = { i0 == 0 ? g_a : S(0), i0 == 1 ? g_a : (i1 == 1 ? g_b : S(0)), i1 == 2 ? g_b : S(0) };
@@ -179,10 +179,10 @@ __device__ __forceinline__ void splat_ots_grad(S w, const S* mu, const S* s, con
     {
         S pp[4];
         transform_point(cam.proj, mu, pp, 4);
-        const S d = pp[3] + S(0.0000001);
+        const S inv_d = fast_div(S(1), pp[3] + S(0.0000001));
         const S g_nx = g_xy[0] * S(0.5) * S(cam.width);
         const S g_ny = g_xy[1] * S(0.5) * S(cam.height);
-        const S g_pp[4] = { g_nx / d, g_ny / d, 0, -(g_nx * pp[0] + g_ny * pp[1]) / (d * d) };
+        const S g_pp[4] = { g_nx * inv_d, g_ny * inv_d, 0, -(g_nx * pp[0] + g_ny * pp[1]) * (inv_d * inv_d) };
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             g_mu[i] = ((cam.proj[4 * i + 0] * g_pp[0] + cam.proj[4 * i + 1] * g_pp[1]) + cam.proj[4 * i + 2] * g_pp[2]) + cam.proj[4 * i + 3] * g_pp[3];
@@ -195,18 +195,19 @@ __device__ __forceinline__ void splat_ots_grad(S w, const S* mu, const S* s, con
             g_J[r][c] += (g_T[r][0] * in.W[c][0] + g_T[r][1] * in.W[c][1]) + g_T[r][2] * in.W[c][2];
     // make_jakobian adjoint (grad/math.h:94-133)
     const S* t = in.t;
-    const S l = in.l;
-    const S tz2 = t[2] * t[2];
+    const S inv_l = fast_div(S(1), in.l);
+    const S inv_z = fast_div(S(1), t[2]);
+    const S inv_z2 = inv_z * inv_z;
     S g_t[3];
-    g_t[0] = g_J[0][2] * (-cam.focal_x / tz2) + g_J[2][0] / l;
-    g_t[1] = g_J[1][2] * (-cam.focal_y / tz2) + g_J[2][1] / l;
-    g_t[2] = g_J[0][0] * (-cam.focal_x / tz2) + g_J[1][1] * (-cam.focal_y / tz2)
-        + g_J[0][2] * (S(2) * cam.focal_x * t[0] / (tz2 * t[2])) + g_J[1][2] * (S(2) * cam.focal_y * t[1] / (tz2 * t[2]))
-        + g_J[2][2] / l;
-    const S g_l = -(g_J[2][0] * t[0] + g_J[2][1] * t[1] + g_J[2][2] * t[2]) / (l * l);
+    g_t[0] = g_J[0][2] * (-cam.focal_x * inv_z2) + g_J[2][0] * inv_l;
+    g_t[1] = g_J[1][2] * (-cam.focal_y * inv_z2) + g_J[2][1] * inv_l;
+    g_t[2] = g_J[0][0] * (-cam.focal_x * inv_z2) + g_J[1][1] * (-cam.focal_y * inv_z2)
+        + g_J[0][2] * (S(2) * cam.focal_x * t[0] * (inv_z2 * inv_z)) + g_J[1][2] * (S(2) * cam.focal_y * t[1] * (inv_z2 * inv_z))
+        + g_J[2][2] * inv_l;
+    const S g_l = -(g_J[2][0] * t[0] + g_J[2][1] * t[1] + g_J[2][2] * t[2]) * (inv_l * inv_l);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        g_t[i] += g_l * t[i] / l;
+        g_t[i] += g_l * t[i] * inv_l;
     // clamp_to_fov passes the gradient inside the limits
     const S lim_x = S(1.3) * cam.tan_fovx * in.t_un[2];
     const S lim_y = S(1.3) * cam.tan_fovy * in.t_un[2];

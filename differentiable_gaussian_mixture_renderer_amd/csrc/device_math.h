@@ -189,7 +189,17 @@ struct SplatIn {
     S cov2[3]; // unfiltered
 };
 
-template <class S>
+// a / b and sqrt from the hardware reciprocal and square root (about an ulp)
+// for float: for the recomputations inside gradients, where last-bit
+// differences sit far below the tolerance.  Forward quantities that the
+// binning depends on keep the correctly rounded div_rn / sqrt_rn.
+__device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double fast_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float fast_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+__device__ __forceinline__ double fast_sqrt(double v) { return __builtin_sqrt(v); }
+
+// FAST: fast_div / fast_sqrt instead of the correctly rounded forms (gradient recomputation only)
+template <class S, bool FAST = false>
 __device__ __forceinline__ void splat_internals(const S* mu, const S* s, const S* q, const DevCamera<S>& cam, SplatIn<S>* in)
 {
     transform_point(cam.view, mu, in->t_un, 3);
@@ -198,17 +208,29 @@ __device__ __forceinline__ void splat_internals(const S* mu, const S* s, const S
     in->t[0] = dmin(dmax(in->t_un[0], -lim_x), lim_x);
     in->t[1] = dmin(dmax(in->t_un[1], -lim_y), lim_y);
     in->t[2] = in->t_un[2];
-    in->l = length3(in->t);
     const S* t = in->t;
-    in->J[0][0] = div_rn(cam.focal_x, t[2]);
+    if constexpr (FAST) {
+        in->l = fast_sqrt(dot3(t, t));
+        const S inv_z = fast_div(S(1), t[2]), inv_l = fast_div(S(1), in->l);
+        in->J[0][0] = cam.focal_x * inv_z;
+        in->J[0][2] = -(cam.focal_x * t[0]) * (inv_z * inv_z);
+        in->J[1][1] = cam.focal_y * inv_z;
+        in->J[1][2] = -(cam.focal_y * t[1]) * (inv_z * inv_z);
+        in->J[2][0] = t[0] * inv_l;
+        in->J[2][1] = t[1] * inv_l;
+        in->J[2][2] = t[2] * inv_l;
+    } else {
+        in->l = length3(in->t);
+        in->J[0][0] = div_rn(cam.focal_x, t[2]);
+        in->J[0][2] = div_rn(-(cam.focal_x * t[0]), t[2] * t[2]);
+        in->J[1][1] = div_rn(cam.focal_y, t[2]);
+        in->J[1][2] = div_rn(-(cam.focal_y * t[1]), t[2] * t[2]);
+        in->J[2][0] = div_rn(t[0], in->l);
+        in->J[2][1] = div_rn(t[1], in->l);
+        in->J[2][2] = div_rn(t[2], in->l);
+    }
     in->J[0][1] = 0;
-    in->J[0][2] = div_rn(-(cam.focal_x * t[0]), t[2] * t[2]);
     in->J[1][0] = 0;
-    in->J[1][1] = div_rn(cam.focal_y, t[2]);
-    in->J[1][2] = div_rn(-(cam.focal_y * t[1]), t[2] * t[2]);
-    in->J[2][0] = div_rn(t[0], in->l);
-    in->J[2][1] = div_rn(t[1], in->l);
-    in->J[2][2] = div_rn(t[2], in->l);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll

@@ -698,8 +698,8 @@ __device__ __forceinline__ void preprocess_backward_core(const float* mu, const 
     float g_w, g_mu[3], g_s[3], g_q[4];
     splat_ots_grad(w, mu, s, q, cam, 0.3f, sg[5], sg, g_cov2, &g_w, g_mu, g_s, g_q);
     const float v[3] = { mu[0] - cam.pos[0], mu[1] - cam.pos[1], mu[2] - cam.pos[2] };
-    const float len = sqrtf(dot3(v, v));
-    const float dir[3] = { v[0] / len, v[1] / len, v[2] / len };
+    const float inv_len = fast_div(1.f, fast_sqrt(dot3(v, v)));
+    const float dir[3] = { v[0] * inv_len, v[1] * inv_len, v[2] * inv_len };
     const float gr[3] = { clamped3[0] ? sg[6] : 0.f, clamped3[1] ? sg[7] : 0.f, clamped3[2] ? sg[8] : 0.f };
     float Y[16], dY[16][3];
     sh_basis(p.sh_degree, dir, Y);
@@ -735,7 +735,7 @@ __device__ __forceinline__ void preprocess_backward_core(const float* mu, const 
     const float dd = dot3(g_dir, dir);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        g_mu_acc[a] += g_mu[a] + (g_dir[a] - dd * dir[a]) / len;
+        g_mu_acc[a] += g_mu[a] + (g_dir[a] - dd * dir[a]) * inv_len;
         g_s_acc[a] += g_s[a] * m;
     }
 #pragma unroll
