@@ -269,7 +269,7 @@ def adam_workload(params, reps: int = 20):
             "fused_gb_s": round(28.0 * n_el / fused / 1e6, 1)}
 
 
-def batched_workload(n: int, n_cams: int, reps: int = 8):
+def batched_workload(n: int, n_cams: int, reps: int = 20):
     """One scene seen by n_cams cameras of uneven resolution (256^2 .. 1080p) on
     the radius-6 sphere: forward + L1 backward of all views through the batched
     path (autotuned tile shape and LDS batch) against the one-camera-per-launch

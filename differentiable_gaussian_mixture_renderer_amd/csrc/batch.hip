@@ -38,8 +38,12 @@ __global__ __launch_bounds__(256) void k_batch_preprocess(int64_t n, int n_views
     const int64_t i = first + threadIdx.x;
     if (i >= n)
         return;
+    // the view-independent inputs once, then each view
+    GaussIn g;
+    gauss_mean(p, i, g);
+    gauss_rest(p, i, g);
     for (int v = 0; v < n_views; ++v)
-        preprocess_gaussian(i, int64_t(v) * n + i, p, s_sh + threadIdx.x * kShStride, cams[v], out);
+        preprocess_view<true>(g, i, int64_t(v) * n + i, p, s_sh + threadIdx.x * kShStride, cams[v], out);
 }
 
 __device__ __forceinline__ int find_view(const ViewBlend* views, int n_views, int tile)
@@ -86,32 +90,20 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t
     }
 }
 
-// clears the 9 screen-gradient floats of every visible entry: thread q
-// clears floats [4q, 4q + 4) of the flat array with one 16-byte store when
-// either entry they belong to is visible (a culled neighbour's slot may be
-// cleared too; it is never read)
-__global__ __launch_bounds__(256) void k_zero_visible(int64_t m, const uint32_t* __restrict__ tiles, float* __restrict__ screen_grads)
+// clears the screen gradients (9 floats per entry) as one streaming pass of
+// 16-byte stores over the whole slice: the blend backward adds into the
+// visible entries' values and only those are read.  Clearing only the
+// visible entries left culled holes in the written lines and took twice as
+// long (0.96 ms over 64M entries).
+__global__ __launch_bounds__(256) void k_zero_screen_grads(int64_t n_floats, float* __restrict__ screen_grads)
 {
-    const int64_t f = 4 * (int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
-    const int64_t total = 9 * m;
-    if (f >= total)
-        return;
-    const int64_t last = min(f + 3, total - 1);
-    int64_t o0, o1;
-    if (total <= int64_t(UINT32_MAX)) { // 32-bit divisions
-        o0 = uint32_t(f) / 9u;
-        o1 = uint32_t(last) / 9u;
-    } else {
-        o0 = f / 9;
-        o1 = last / 9;
-    }
-    if (tiles[o0] == 0 && tiles[o1] == 0)
-        return;
-    if (f + 4 <= total)
-        *reinterpret_cast<float4*>(screen_grads + f) = make_float4(0.f, 0.f, 0.f, 0.f);
-    else
-        for (int64_t k = f; k < total; ++k)
-            screen_grads[k] = 0.f;
+    const int64_t n4 = n_floats / 4;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    float4* d = reinterpret_cast<float4*>(screen_grads);
+    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n4; k += stride)
+        d[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t k = 4 * n4 + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n_floats; k += stride)
+        screen_grads[k] = 0.f;
 }
 
 // thread per Gaussian, summing its adjoint over the views of each gradient
@@ -133,15 +125,19 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
     const bool valid = i < n;
     float* row = s_sh + threadIdx.x * kShStride;
     const int kc = 3 * p.sh_coeffs;
-    float g_mu[3], g_s[3], g_q[4], g_w, g_shv[48];
-    // the Gaussian's own values are the same for every view: read once
-    float mu[3] = { 0.f, 0.f, 0.f }, s[3] = { 0.f, 0.f, 0.f }, q[4] = { 1.f, 0.f, 0.f, 0.f }, w = 0.f;
+    // per group: the mean, opacity and SH gradients, and the view sums of
+    // dL/dcov3 and dL/d(2 pi sa sb) that become scale and rotation gradients
+    float g_mu[3], g_cov3[6], g_num, g_w, g_shv[48];
+    // the Gaussian's own values are the same for every view: read once, and
+    // its 3D covariance and largest scales computed once
+    float mu[3] = { 0.f, 0.f, 0.f }, s[3] = { 1.f, 1.f, 1.f }, q[4] = { 1.f, 0.f, 0.f, 0.f }, w = 0.f;
     if (valid) {
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             mu[a] = p.means[3 * i + a];
         gaussian_params(p, i, s, q, &w);
     }
+    const OtsPre<float> pre = ots_pre(s, q);
     // per-(view, Gaussian) inputs of view v + 1 are loaded while view v is
     // evaluated; the loads stay in bounds for every view and are unconditional
     struct ViewIn {
@@ -176,6 +172,8 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
                 // go through LDS so the block writes them as one contiguous span
                 __syncthreads(); // every thread has read its SH row
                 if (valid) {
+                    float g_s[3], g_q[4];
+                    preprocess_backward_finish(s, q, pre, p, g_cov3, g_num, g_s, g_q);
                     activation_backward(p, i, g_s, g_q, &g_w);
                     const int64_t o = int64_t(cur) * n + i;
                     for (int a = 0; a < 3; ++a) {
@@ -203,9 +201,10 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
             }
             cur = gv;
             for (int a = 0; a < 3; ++a)
-                g_mu[a] = g_s[a] = 0.f;
-            for (int a = 0; a < 4; ++a)
-                g_q[a] = 0.f;
+                g_mu[a] = 0.f;
+            for (int a = 0; a < 6; ++a)
+                g_cov3[a] = 0.f;
+            g_num = 0.f;
             g_w = 0.f;
 #pragma unroll
             for (int k = 0; k < 48; ++k)
@@ -214,7 +213,7 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
         if (gv < 0 || !valid)
             continue;
         if (in.tiles != 0)
-            preprocess_backward_core(mu, s, q, w, p, row, cams[v], in.A, in.cl, in.sg, g_mu, g_s, g_q, &g_w, g_shv);
+            preprocess_backward_core(mu, pre, w, p, row, cams[v], in.A, in.cl, in.sg, g_mu, g_cov3, &g_num, &g_w, g_shv);
     }
 }
 
@@ -461,10 +460,12 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
     DGMR_HIP_CHECK(hipMemcpyAsync(L.views, vb.data(), sizeof(ViewBlend) * n_views, hipMemcpyHostToDevice, st));
     DGMR_HIP_CHECK(hipMemcpyAsync(L.groups, grp.data(), sizeof(int32_t) * n_views, hipMemcpyHostToDevice, st));
     const int64_t m = n * n_views;
-    // the blend backward adds into the screen gradients of visible entries only,
-    // and only those are read afterwards: clear just them
-    k_zero_visible<<<unsigned(ceil_div64(ceil_div64(9 * m, 4), 256)), 256, 0, st>>>(m, L.tiles, L.screen_grads);
-    DGMR_HIP_CHECK(hipGetLastError());
+    {
+        StageScope sc("batch.clear", st, 36.0 * double(m));
+        const int64_t blocks = std::min<int64_t>(ceil_div64(ceil_div64(9 * m, 4), 256), 8192);
+        k_zero_screen_grads<<<unsigned(std::max<int64_t>(blocks, 1)), 256, 0, st>>>(9 * m, L.screen_grads);
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
     if (n_pairs > 0) {
         StageScope sc("batch.blend_backward", st, 40.0 * double(n_pairs));
         DGMR_HIP_CHECK(hipMemsetAsync(L.queue, 0, sizeof(uint32_t), st));

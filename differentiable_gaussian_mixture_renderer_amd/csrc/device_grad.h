@@ -145,36 +145,60 @@ __device__ __forceinline__ void affine_cut_grad(const S* c, const S M[3][3], con
     }
 }
 
-// grad::splat<Ots> (grad/math.h:333-527, case Ots) with the EWA chain of
-// grad/math.h:497-526.  Inputs: gradients of the 2D Gaussian's weight, centroid
-// and filtered covariance (stored values).
+// The Ots splat's per-Gaussian inputs, the same for every view: the 3D
+// covariance and the two largest scales (splat_ots_grad_view takes them, so
+// a Gaussian seen from many views computes them once).
 template <class S>
-__device__ __forceinline__ void splat_ots_grad(S w, const S* mu, const S* s, const S* q, const DevCamera<S>& cam, S filter,
-    S g_weight, const S* g_xy, const S* g_cov2, S* g_w, S* g_mu, S* g_s, S* g_q)
+struct OtsPre {
+    S cov3[6];
+    S sa, sb;
+    int i0, i1;
+};
+
+template <class S>
+__device__ __forceinline__ OtsPre<S> ots_pre(const S* s, const S* q)
+{
+    OtsPre<S> o;
+    compute_cov(s, q, o.cov3);
+    larger2_values(s, &o.sa, &o.sb, &o.i0, &o.i1);
+    return o;
+}
+
+// Adjoint of the Ots splat (math.h:389-397 and splat_internals) for one
+// view: the weight, centroid and filtered covariance gradients (stored
+// values) give g_w and g_mu for this view, and add dL/dcov3 into g_cov3 and
+// dL/d(2 pi sa sb) into g_num.  Both are linear in what reaches the scales
+// and rotation, so their sums over views go through splat_ots_grad_finish
+// once.  Recomputation uses the hardware reciprocal (gradient tolerance).
+template <class S>
+__device__ __forceinline__ void splat_ots_grad_view(S w, const S* mu, const OtsPre<S>& pre, const DevCamera<S>& cam, S filter, S g_weight,
+    const S* g_xy, const S* g_cov2, S* g_w, S* g_mu, S* g_cov3, S* g_num)
 {
     SplatIn<S> in;
-    splat_internals<S, true>(mu, s, q, cam, &in);
+    splat_projection<S, true>(mu, cam, &in);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        in.cov3[k] = pre.cov3[k];
+    affine_cut(in.cov3, in.T, in.cov2);
     const S cov_f[3] = { in.cov2[0] + filter, in.cov2[1], in.cov2[2] + filter };
     const S detJ = det3(in.J);
-    int i0, i1;
-    S sa, sb;
-    larger2_values(s, &sa, &sb, &i0, &i1);
-    const S num = S(2 * kPi) * sa * sb;
+    const S num = S(2 * kPi) * pre.sa * pre.sb;
     const S det2 = cov_f[0] * cov_f[2] - cov_f[1] * cov_f[1];
     const S sq = fast_sqrt(det2);
     const S inv_i2 = fast_div(S(1), S(2 * kPi) * sq);
     *g_w = g_weight * detJ * num * inv_i2;
     const S g_det = g_weight * w * num * inv_i2;
-    const S g_num = g_weight * w * detJ * inv_i2;
+    *g_num += g_weight * w * detJ * inv_i2;
     const S g_i2 = -g_weight * w * detJ * num * (inv_i2 * inv_i2);
     const S f = g_i2 * S(2 * kPi) * fast_div(S(1), S(2) * sq);
     S g_c2[3] = { g_cov2[0] + f * cov_f[2], g_cov2[1] + f * (S(-2) * cov_f[1]), g_cov2[2] + f * cov_f[0] };
-    const S g_a = g_num * S(2 * kPi) * sb, g_b = g_num * S(2 * kPi) * sa;
-    const S gs[3] = { i0 == 0 ? g_a : S(0), i0 == 1 ? g_a : (i1 == 1 ? g_b : S(0)), i1 == 2 ? g_b : S(0) };
     S g_J[3][3];
     det3_grad(in.J, g_det, g_J);
-    S g_cov3[6], g_T[3][3];
-    affine_cut_grad(in.cov3, in.T, g_c2, g_cov3, g_T, true);
+    S g_c3[6], g_T[3][3];
+    affine_cut_grad(in.cov3, in.T, g_c2, g_c3, g_T, true);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        g_cov3[k] += g_c3[k];
     // centroid through project + ndc2screen
     {
         S pp[4];
@@ -218,6 +242,15 @@ __device__ __forceinline__ void splat_ots_grad(S w, const S* mu, const S* s, con
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         g_mu[i] += (in.W[0][i] * g_t[0] + in.W[1][i] * g_t[1]) + in.W[2][i] * g_t[2];
+}
+
+// scales and rotation gradients from the view sums of dL/dcov3 and
+// dL/d(2 pi sa sb)
+template <class S>
+__device__ __forceinline__ void splat_ots_grad_finish(const S* s, const S* q, const OtsPre<S>& pre, const S* g_cov3, S g_num, S* g_s, S* g_q)
+{
+    const S g_a = g_num * S(2 * kPi) * pre.sb, g_b = g_num * S(2 * kPi) * pre.sa;
+    const S gs[3] = { pre.i0 == 0 ? g_a : S(0), pre.i0 == 1 ? g_a : (pre.i1 == 1 ? g_b : S(0)), pre.i1 == 2 ? g_b : S(0) };
     S gs2[3];
     compute_cov_grad(s, q, g_cov3, gs2, g_q);
 #pragma unroll

@@ -98,11 +98,13 @@ template <class S>
 __device__ __forceinline__ void get_rect(S px, S py, int ext_x, int ext_y, int grid_x, int grid_y, int tile_w, int tile_h,
     int* x0, int* y0, int* x1, int* y1)
 {
+    // tile sides are powers of two, so x / tw is exactly x * (1 / tw)
     const S tw = S(tile_w), th = S(tile_h);
-    const int a = guarded_int(div_rn(px - S(ext_x), tw));
-    const int b = guarded_int(div_rn(py - S(ext_y), th));
-    const int c = guarded_int(div_rn(((px + S(ext_x)) + tw) - S(1), tw));
-    const int d = guarded_int(div_rn(((py + S(ext_y)) + th) - S(1), th));
+    const S itw = div_rn(S(1), tw), ith = div_rn(S(1), th);
+    const int a = guarded_int((px - S(ext_x)) * itw);
+    const int b = guarded_int((py - S(ext_y)) * ith);
+    const int c = guarded_int((((px + S(ext_x)) + tw) - S(1)) * itw);
+    const int d = guarded_int((((py + S(ext_y)) + th) - S(1)) * ith);
     *x0 = min(grid_x, max(0, a));
     *y0 = min(grid_y, max(0, b));
     *x1 = min(grid_x, max(0, c));
@@ -198,9 +200,11 @@ __device__ __forceinline__ double fast_div(double a, double b) { return a / b; }
 __device__ __forceinline__ float fast_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
 __device__ __forceinline__ double fast_sqrt(double v) { return __builtin_sqrt(v); }
 
-// FAST: fast_div / fast_sqrt instead of the correctly rounded forms (gradient recomputation only)
+// view-space centre, Jacobian J, view rotation W and T = J W of the EWA
+// splat (everything but the covariances).  FAST: fast_div / fast_sqrt
+// instead of the correctly rounded forms (gradient recomputation only).
 template <class S, bool FAST = false>
-__device__ __forceinline__ void splat_internals(const S* mu, const S* s, const S* q, const DevCamera<S>& cam, SplatIn<S>* in)
+__device__ __forceinline__ void splat_projection(const S* mu, const DevCamera<S>& cam, SplatIn<S>* in)
 {
     transform_point(cam.view, mu, in->t_un, 3);
     const S lim_x = S(1.3) * cam.tan_fovx * in->t_un[2];
@@ -237,6 +241,12 @@ __device__ __forceinline__ void splat_internals(const S* mu, const S* s, const S
         for (int c = 0; c < 3; ++c)
             in->W[r][c] = cam.view[4 * c + r];
     matmul3(in->J, in->W, in->T);
+}
+
+template <class S, bool FAST = false>
+__device__ __forceinline__ void splat_internals(const S* mu, const S* s, const S* q, const DevCamera<S>& cam, SplatIn<S>* in)
+{
+    splat_projection<S, FAST>(mu, cam, in);
     compute_cov(s, q, in->cov3);
     affine_cut(in->cov3, in->T, in->cov2);
 }

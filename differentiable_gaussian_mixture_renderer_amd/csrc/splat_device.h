@@ -218,7 +218,43 @@ __device__ __forceinline__ void unstage_sh(float* dst, const float* s_sh, int64_
 // preprocess of Gaussian i for one camera (splat_forward.cu:71-118): cull
 // view-z < 0.2, EWA splat (Ots, 0.3 px^2 filter), rect ceil(3 sigma), tiles
 // touched, depth, SH colour, conic.  Writes slot o.  c: this Gaussian's SH row.
+// a Gaussian's view-independent preprocess inputs: centre, activated scales,
+// rotation and opacity, 3D covariance and its two largest scales
+struct GaussIn {
+    float mu[3], s[3], q[4], w, cov3[6], sa, sb;
+};
+
+__device__ __forceinline__ void gauss_mean(const SplatParams& p, int64_t i, GaussIn& g)
+{
+    g.mu[0] = p.means[3 * i];
+    g.mu[1] = p.means[3 * i + 1];
+    g.mu[2] = p.means[3 * i + 2];
+}
+
+__device__ __forceinline__ void gauss_rest(const SplatParams& p, int64_t i, GaussIn& g)
+{
+    gaussian_params(p, i, g.s, g.q, &g.w);
+    compute_cov(g.s, g.q, g.cov3);
+    int i0, i1;
+    larger2_values(g.s, &g.sa, &g.sb, &i0, &i1);
+}
+
+template <bool REST_READY>
+__device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o, const SplatParams& p, const float* c, const DevCamera<float>& cam,
+    const SplatOut& out);
+
+// one view: g.mu set, the rest computed here after the view-space cull
 __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const SplatParams& p, const float* c, const DevCamera<float>& cam,
+    const SplatOut& out)
+{
+    GaussIn g;
+    gauss_mean(p, i, g);
+    preprocess_view<false>(g, i, o, p, c, cam, out);
+}
+
+// REST_READY: g fully set (a Gaussian seen from many views fills it once)
+template <bool REST_READY>
+__device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o, const SplatParams& p, const float* c, const DevCamera<float>& cam,
     const SplatOut& out)
 {
     out.tiles[o] = 0;
@@ -230,15 +266,20 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     out.ids[o] = uint32_t(o);
     if (!out.reach) // reach records are read only for entries with a depth key
         out.bin_rects[o] = make_uint2(0u, 0u);
-    const float mu[3] = { p.means[3 * i], p.means[3 * i + 1], p.means[3 * i + 2] };
+    const float* mu = g.mu;
     float t[3];
     transform_point(cam.view, mu, t, 3);
     if (t[2] < 0.2f)
         return;
-    float s[3], q[4], w;
-    gaussian_params(p, i, s, q, &w);
+    if (!REST_READY)
+        gauss_rest(p, i, g);
+    const float w = g.w;
     SplatIn<float> in;
-    splat_internals(mu, s, q, cam, &in);
+    splat_projection(mu, cam, &in);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        in.cov3[k] = g.cov3[k];
+    affine_cut(in.cov3, in.T, in.cov2);
     const float cov_f[3] = { in.cov2[0] + 0.3f, in.cov2[1], in.cov2[2] + 0.3f };
     float xy[2];
     project_to_screen(mu, cam, xy);
@@ -261,8 +302,11 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     out.depths[o] = depth;
     out.depth_keys[o] = __float_as_uint(depth);
     // colour from SH along normalize(mu - cam) (math.h:233-271)
+    // the colour, conic and weight feed only the blend: hardware reciprocal
+    // and square root (an ulp); centre, depth and extents above, which the
+    // binning depends on, stay correctly rounded
     const float v[3] = { -dv[0], -dv[1], -dv[2] };
-    const float inv_len = __fdiv_rn(1.f, sqrt_rn(dot3(v, v)));
+    const float inv_len = fast_div(1.f, fast_sqrt(dot3(v, v)));
     const float dir[3] = { v[0] * inv_len, v[1] * inv_len, v[2] * inv_len };
     float Y[16];
     sh_basis(p.sh_degree, dir, Y);
@@ -283,14 +327,12 @@ __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const 
     out.clamped[3 * o + 1] = r1 >= 0.f;
     out.clamped[3 * o + 2] = r2 >= 0.f;
     // screen-space weight, Ots formulation (math.h:389-397)
-    int i0, i1;
-    float sa, sb;
-    larger2_values(s, &sa, &sb, &i0, &i1);
-    const float i2prime = float(2 * kPi) * sa * sb;
+    const float i2prime = float(2 * kPi) * g.sa * g.sb;
     const float det2 = cov_f[0] * cov_f[2] - cov_f[1] * cov_f[1];
-    const float i2 = float(2 * kPi) * sqrt_rn(det2);
-    const float weight = w * det3(in.J) * i2prime / i2;
-    const float4 co = make_float4(cov_f[2] / det2, -cov_f[1] / det2, cov_f[0] / det2, weight);
+    const float inv_i2 = fast_div(1.f, float(2 * kPi) * fast_sqrt(det2));
+    const float weight = w * det3(in.J) * i2prime * inv_i2;
+    const float inv_det2 = fast_div(1.f, det2);
+    const float4 co = make_float4(cov_f[2] * inv_det2, -cov_f[1] * inv_det2, cov_f[0] * inv_det2, weight);
     out.conic_opacity[o] = co;
     if (out.reach) {
         const ReachRec r = make_reach(x0, y0, x1 - x0, y1 - y0, cam.grid_x, xy[0], xy[1], co.x, co.y, co.z, co.w);
@@ -667,10 +709,24 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
 enum class ShGrad { Add, Export };
 // the per-Gaussian values (mean, scales times the multiplier, rotation,
 // opacity) passed in, for callers that evaluate one Gaussian for many views
+// One view's adjoint of the preprocess of a Gaussian (pre = ots_pre of its
+// scales and rotation): adds into g_mu_acc, g_w_acc, the SH gradient, and the
+// view sums g_cov3_acc, g_num_acc that preprocess_backward_finish turns into
+// scale and rotation gradients once per Gaussian.
 template <ShGrad MODE = ShGrad::Add>
-__device__ __forceinline__ void preprocess_backward_core(const float* mu, const float* s, const float* q, float w, const SplatParams& p,
-    const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc,
-    float* g_q_acc, float* g_w_acc, float* g_sh);
+__device__ __forceinline__ void preprocess_backward_core(const float* mu, const OtsPre<float>& pre, float w, const SplatParams& p,
+    const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_cov3_acc,
+    float* g_num_acc, float* g_w_acc, float* g_sh);
+
+// scale (times the scale multiplier) and rotation gradients from the view sums
+__device__ __forceinline__ void preprocess_backward_finish(const float* s, const float* q, const OtsPre<float>& pre, const SplatParams& p,
+    const float* g_cov3, float g_num, float* g_s, float* g_q)
+{
+    splat_ots_grad_finish(s, q, pre, g_cov3, g_num, g_s, g_q);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        g_s[a] *= p.scale_multiplier;
+}
 
 template <ShGrad MODE = ShGrad::Add>
 __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const SplatParams& p, const float* c, const DevCamera<float>& cam,
@@ -679,15 +735,24 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
     const float mu[3] = { p.means[3 * i], p.means[3 * i + 1], p.means[3 * i + 2] };
     float s[3], q[4], w;
     gaussian_params(p, i, s, q, &w);
-    preprocess_backward_core<MODE>(mu, s, q, w, p, c, cam, A, clamped3, sg, g_mu_acc, g_s_acc, g_q_acc, g_w_acc, g_sh);
+    const OtsPre<float> pre = ots_pre(s, q);
+    float g_cov3[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f }, g_num = 0.f;
+    preprocess_backward_core<MODE>(mu, pre, w, p, c, cam, A, clamped3, sg, g_mu_acc, g_cov3, &g_num, g_w_acc, g_sh);
+    float g_s[3], g_q[4];
+    preprocess_backward_finish(s, q, pre, p, g_cov3, g_num, g_s, g_q);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        g_s_acc[a] += g_s[a];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        g_q_acc[a] += g_q[a];
 }
 
 template <ShGrad MODE>
-__device__ __forceinline__ void preprocess_backward_core(const float* mu, const float* s, const float* q, float w, const SplatParams& p,
-    const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_s_acc,
-    float* g_q_acc, float* g_w_acc, float* g_sh)
+__device__ __forceinline__ void preprocess_backward_core(const float* mu, const OtsPre<float>& pre, float w, const SplatParams& p,
+    const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_cov3_acc,
+    float* g_num_acc, float* g_w_acc, float* g_sh)
 {
-    const float m = p.scale_multiplier;
     // conic = cov^-1: dL/dcov = -A G A
     const float G00 = sg[2], G01 = 0.5f * sg[3], G11 = sg[4];
     const float t00 = A.x * G00 + A.y * G01, t01 = A.x * G01 + A.y * G11;
@@ -695,8 +760,8 @@ __device__ __forceinline__ void preprocess_backward_core(const float* mu, const 
     const float r00 = t00 * A.x + t01 * A.y, r01 = t00 * A.y + t01 * A.z;
     const float r10 = t10 * A.x + t11 * A.y, r11 = t10 * A.y + t11 * A.z;
     const float g_cov2[3] = { -r00, -(r01 + r10), -r11 };
-    float g_w, g_mu[3], g_s[3], g_q[4];
-    splat_ots_grad(w, mu, s, q, cam, 0.3f, sg[5], sg, g_cov2, &g_w, g_mu, g_s, g_q);
+    float g_w, g_mu[3];
+    splat_ots_grad_view(w, mu, pre, cam, 0.3f, sg[5], sg, g_cov2, &g_w, g_mu, g_cov3_acc, g_num_acc);
     const float v[3] = { mu[0] - cam.pos[0], mu[1] - cam.pos[1], mu[2] - cam.pos[2] };
     const float inv_len = fast_div(1.f, fast_sqrt(dot3(v, v)));
     const float dir[3] = { v[0] * inv_len, v[1] * inv_len, v[2] * inv_len };
@@ -734,13 +799,8 @@ __device__ __forceinline__ void preprocess_backward_core(const float* mu, const 
     }
     const float dd = dot3(g_dir, dir);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < 3; ++a)
         g_mu_acc[a] += g_mu[a] + (g_dir[a] - dd * dir[a]) * inv_len;
-        g_s_acc[a] += g_s[a] * m;
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-        g_q_acc[a] += g_q[a];
     *g_w_acc += g_w;
 }
 
