@@ -172,13 +172,11 @@ class ContinuousBatchingServer:
             n_bwd = sum(r.backward for r in batch)
             with torch.cuda.stream(stream):
                 sc = self.resident[scene_id]
-                imgs, _ = splatter.forward(*sc.params, [r.camera for r in batch], sh_degree=sc.sh_degree)
-                grads = [None] * len(batch)
-                if n_bwd:
-                    gimg = losses.l1_grad_many(imgs[:n_bwd], [l1_target(img, r.target_seed) for img, r in zip(imgs, batch[:n_bwd])])
-                    gimg += [None] * (len(batch) - n_bwd)
-                    groups = [k if k < n_bwd else -1 for k in range(len(batch))]
-                    grads[:n_bwd] = splatter.backward(gimg, groups=groups)
+                if len(batch) == 1:
+                    # a lone request (low load): the one-camera path has less per-launch setup
+                    imgs, grads = self._single(sc, batch[0])
+                else:
+                    imgs, grads = self._batched(splatter, sc, batch, n_bwd)
                 stream.synchronize()
             t = self.now()
             with self._lock:
@@ -191,6 +189,28 @@ class ContinuousBatchingServer:
                 self.stats.batches += 1
                 self._in_flight -= cost
                 self._lock.notify_all()
+
+    def _single(self, sc, r):
+        from .python_bindings import splatter
+        cam = r.camera
+        args = (*sc.params[:5], torch.tensor(cam.world_view_transform), torch.tensor(cam.full_proj_transform),
+                torch.tensor(cam.camera_center), torch.zeros(3), cam.width, cam.height, cam.tan_fovx, cam.tan_fovy, sc.sh_degree)
+        cache = splatter.cpp_binding.forward(*args, tile=self.tile, batch=self.lds_batch)
+        grads = None
+        if r.backward:
+            grads = tuple(splatter.cpp_binding.backward(*args, l1_grad(cache[0], r.target_seed), cache, tile=self.tile, batch=self.lds_batch))
+        return [cache[0]], [grads]
+
+    @staticmethod
+    def _batched(splatter, sc, batch, n_bwd):
+        imgs, _ = splatter.forward(*sc.params, [r.camera for r in batch], sh_degree=sc.sh_degree)
+        grads = [None] * len(batch)
+        if n_bwd:
+            gimg = losses.l1_grad_many(imgs[:n_bwd], [l1_target(img, r.target_seed) for img, r in zip(imgs, batch[:n_bwd])])
+            gimg += [None] * (len(batch) - n_bwd)
+            groups = [k if k < n_bwd else -1 for k in range(len(batch))]
+            grads[:n_bwd] = splatter.backward(gimg, groups=groups)
+        return imgs, grads
 
     # -------------------------------------------------------------- scheduler
     def _schedule(self):
