@@ -154,12 +154,17 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const KeyT* __restr
     __syncthreads();
     if (base < n) {
         const unsigned wave = threadIdx.x / 64;
-#pragma unroll 4
+        // every load first (all in flight together), then the LDS counts
+        uint32_t k[kSortItems];
+#pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
             const int64_t i = base + int64_t(j) * kSortThreads + threadIdx.x;
-            if (i < n)
-                atomicAdd(&s_hist[wave][(uint32_t(keys[i]) >> shift) & mask], 1u);
+            k[j] = i < n ? uint32_t(keys[i]) : 0xffffffffu;
         }
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j)
+            if (base + int64_t(j) * kSortThreads + threadIdx.x < n)
+                atomicAdd(&s_hist[wave][(k[j] >> shift) & mask], 1u);
     }
     __syncthreads();
     if (threadIdx.x <= mask) {
