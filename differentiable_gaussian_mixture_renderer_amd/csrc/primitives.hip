@@ -310,8 +310,8 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_count(const uint32_
     __shared__ uint32_t s_wave[kCompactThreads / 64];
     const int64_t base = int64_t(blockIdx.x) * kCompactTile;
     uint32_t c = 0;
-#pragma unroll 4
-    for (int j = 0; j < kCompactItems; ++j) {
+#pragma unroll
+    for (int j = 0; j < kCompactItems; ++j) { // every load in flight at once
         const int64_t i = base + int64_t(j) * kCompactThreads + threadIdx.x;
         c += (i < n && keys[i] != skip_key) ? 1u : 0u;
     }
