@@ -37,8 +37,16 @@ using SortBuffers = SortBuffersT<uint32_t>;
 using SortBuffers16 = SortBuffersT<uint16_t>;
 int radix_passes(int begin_bit, int end_bit);
 size_t radix_scratch_bytes(int64_t n_max);
+// The last pass may write, instead of each sorted key, the view of its value
+// (value / n_per_view; culled_view for the key 0xffffffff when culled_view is
+// not ~0): the keys of a following stable pass on the view.  n_per_view 0
+// keeps the keys.
+struct KeyOut {
+    uint32_t n_per_view = 0;
+    uint32_t culled_view = 0xffffffffu;
+};
 hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit,
-    void* scratch, hipStream_t stream);
+    void* scratch, hipStream_t stream, KeyOut last_pass_keys = KeyOut {});
 hipError_t radix_sort_pairs(const SortBuffers16& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit,
     void* scratch, hipStream_t stream);
 

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const KeyT* __restr
 template <int ITEMS, class KeyT, int RADIX>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     KeyT* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int bits, int n_blocks,
-    const uint32_t* __restrict__ offsets)
+    const uint32_t* __restrict__ offsets, KeyOut ko)
 {
     constexpr int kSortTile = kSortThreads * ITEMS;
     constexpr int kWaves = kSortThreads / 64;
@@ -294,8 +294,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __re
         const KeyT key = s_keys[q];
         const uint32_t d = (uint32_t(key) >> shift) & mask;
         const uint32_t dst = s_global[d] + (uint32_t(q) - s_local[d]);
-        keys_out[dst] = key;
-        vals_out[dst] = s_vals[q];
+        const uint32_t val = s_vals[q];
+        if (ko.n_per_view == 0)
+            keys_out[dst] = key;
+        else // the view of the id replaces the key (culled_view for a culled key 0xffffffff when set)
+            keys_out[dst] = KeyT((ko.culled_view != 0xffffffffu && uint32_t(key) == 0xffffffffu) ? ko.culled_view : val / ko.n_per_view);
+        vals_out[dst] = val;
     }
 }
 
@@ -413,7 +417,7 @@ size_t radix_scratch_bytes(int64_t n_max)
 
 template <class KeyT>
 static hipError_t radix_sort_impl(const SortBuffersT<KeyT>& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit, void* scratch,
-    hipStream_t stream)
+    hipStream_t stream, KeyOut last = KeyOut {})
 {
     const int passes = radix_passes(begin_bit, end_bit);
     if (n_max <= 0 || passes == 0)
@@ -437,18 +441,19 @@ static hipError_t radix_sort_impl(const SortBuffersT<KeyT>& buf, int64_t n_max, 
         hipError_t e = scan_u32(hist, hist, nullptr, int64_t(blocks) << bits, false, scan_scratch, stream);
         if (e != hipSuccess)
             return e;
+        const KeyOut ko = p + 1 == passes ? last : KeyOut {};
         if (large && bits <= 7)
             k_radix_scatter<kLargeItems, KeyT, 128><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks,
-                hist);
+                hist, ko);
         else if (large)
             k_radix_scatter<kLargeItems, KeyT, 256><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks,
-                hist);
+                hist, ko);
         else if (bits <= 7)
             k_radix_scatter<kSmallItems, KeyT, 128><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks,
-                hist);
+                hist, ko);
         else
             k_radix_scatter<kSmallItems, KeyT, 256><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks,
-                hist);
+                hist, ko);
         e = hipGetLastError();
         if (e != hipSuccess)
             return e;
@@ -457,9 +462,9 @@ static hipError_t radix_sort_impl(const SortBuffersT<KeyT>& buf, int64_t n_max, 
 }
 
 hipError_t radix_sort_pairs(const SortBuffers& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit, void* scratch,
-    hipStream_t stream)
+    hipStream_t stream, KeyOut last_pass_keys)
 {
-    return radix_sort_impl(buf, n_max, n_dev, begin_bit, end_bit, scratch, stream);
+    return radix_sort_impl(buf, n_max, n_dev, begin_bit, end_bit, scratch, stream, last_pass_keys);
 }
 
 hipError_t radix_sort_pairs(const SortBuffers16& buf, int64_t n_max, const uint32_t* n_dev, int begin_bit, int end_bit, void* scratch,

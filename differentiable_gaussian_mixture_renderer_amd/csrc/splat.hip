@@ -596,18 +596,6 @@ __global__ __launch_bounds__(64) void k_selftest_wave_sum(const float* __restric
     out[64 + l] = wave_sum_to_last(in[512 + l]);
 }
 
-// view index of each sorted entry, in place of its depth key; with
-// culled_view != ~0 an entry still holding the culled depth key 0xffffffff
-// gets culled_view instead (past every view: it sorts last)
-__global__ void k_view_keys(int64_t n, const uint32_t* __restrict__ n_sorted, const uint32_t* __restrict__ ids, int64_t n_per_view,
-    uint32_t culled_view, uint32_t* __restrict__ keys)
-{
-    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j < n && (!n_sorted || j < int64_t(*n_sorted))) {
-        const uint32_t v = ids[j] / uint32_t(n_per_view); // ids < 2^32: a 32-bit division
-        keys[j] = (culled_view != 0xffffffffu && keys[j] == 0xffffffffu) ? culled_view : v;
-    }
-}
 } // namespace
 
 // ------------------------------------------------------------------ host side
@@ -800,21 +788,22 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
         std::swap(V, V2);
         n_sorted = count;
     }
+    const bool multi_view = batch && batch->n_views > 1;
+    // with tight culling the culled entries sort last in the view pass, under the key n_views
+    const uint32_t culled_view = cull && multi_view ? uint32_t(batch->n_views) : 0xffffffffu;
     {
         StageScope sc("bin.depth_sort", st, 16.0 * double(n) * radix_passes(0, 32));
-        DGMR_HIP_CHECK(radix_sort_pairs(SortBuffers { K, V, K2, V2 }, n, n_sorted, 0, 32, w.sort_scratch, st));
+        // several views: the last depth pass writes each entry's view as its key
+        const KeyOut view_keys = multi_view ? KeyOut { uint32_t(n_per_view), culled_view } : KeyOut {};
+        DGMR_HIP_CHECK(radix_sort_pairs(SortBuffers { K, V, K2, V2 }, n, n_sorted, 0, 32, w.sort_scratch, st, view_keys));
         if (radix_passes(0, 32) % 2 == 1) {
             std::swap(K, K2);
             std::swap(V, V2);
         }
     }
-    if (batch && batch->n_views > 1) {
+    if (multi_view) {
         // stable pass on the view index: order becomes (view, depth bits, id)
         StageScope sc("bin.view_sort", st, 16.0 * double(n));
-        // with tight culling the culled entries sort last, under the key n_views
-        const uint32_t culled_view = cull ? uint32_t(batch->n_views) : 0xffffffffu;
-        k_view_keys<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_sorted, V, n_per_view, culled_view, K);
-        DGMR_HIP_CHECK(hipGetLastError());
         const int vbits = int(higher_msb(cull ? uint32_t(batch->n_views) : uint32_t(batch->n_views - 1)));
         DGMR_HIP_CHECK(radix_sort_pairs(SortBuffers { K, V, K2, V2 }, n, n_sorted, 0, vbits, w.sort_scratch, st));
         if (radix_passes(0, vbits) % 2 == 1) {
