@@ -450,7 +450,8 @@ def serving_workload(n_requests: int, rate_hz: float = 200.0):
     log-uniform), cameras 256^2..1080p, 30 % forward+backward: continuous
     batching against a FIFO server that renders one request at a time through
     the plain path.  Also both servers with every request present at t=0
-    (saturated), which measures sustainable throughput."""
+    (saturated, median of three runs), which measures sustainable
+    throughput."""
     import copy
 
     import numpy as np
@@ -481,8 +482,16 @@ def serving_workload(n_requests: int, rate_hz: float = 200.0):
     sat = copy.deepcopy(trace)
     for r in sat:
         r.arrival = 0.0
-    out["continuous_saturated"] = run(serving.ContinuousBatchingServer, copy.deepcopy(sat), hbm_budget_bytes=budget)
-    out["fifo_plain_saturated"] = run(serving.FifoServer, copy.deepcopy(sat))
+
+    def saturated(server_cls, **kw):
+        # a saturated run lasts a fraction of a second: the median of three
+        runs = sorted((run(server_cls, copy.deepcopy(sat), **kw) for _ in range(3)), key=lambda st: st["mpixel_per_s"])
+        med = dict(runs[1])
+        med["mpixel_per_s_runs"] = [round(st["mpixel_per_s"], 1) for st in runs]
+        return med
+
+    out["continuous_saturated"] = saturated(serving.ContinuousBatchingServer, hbm_budget_bytes=budget)
+    out["fifo_plain_saturated"] = saturated(serving.FifoServer)
     out["saturated_throughput_ratio"] = round(out["continuous_saturated"]["mpixel_per_s"] / out["fifo_plain_saturated"]["mpixel_per_s"], 3)
     return out
 
