@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--torch-loss", action="store_true", help="L1 loss as torch's composite (sub, abs, mean) instead of the fused kernel")
     ap.add_argument("--no-marcher", action="store_true", help="skip the volumetric marcher workload")
     ap.add_argument("--no-raster", action="store_true", help="skip the volumetric rasteriser workload")
+    ap.add_argument("--no-tight", action="store_true", help="skip the tight-culling run of the training step")
     ap.add_argument("--batched-cams", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -158,26 +159,10 @@ def main():
     n_visible = int((cache[7] > 0).sum())
     del cache
 
-    # the same step with tight culling (dgmr_tiling.cull = 1): the same image
-    # and gradients from fewer pairs; the reference binning above stays the
-    # reported value, since the drop-in returns the reference's point list
-    splatter.DEFAULT_CULL = True
-    el_cull = timed()
-    lib.dgmr_profile_enable(1)
-    for _ in range(prof_steps):
-        step()
-    torch.cuda.synchronize()
-    lib.dgmr_profile_enable(0)
-    stages_cull = stage_times(lib)
-    with torch.no_grad():
-        cache = splatter.cpp_binding.forward(sh, torch.sigmoid(logits), means, torch.exp(log_scales),
-                                             torch.nn.functional.normalize(quats, dim=-1), view, proj, campos, bg, W, H, cam.tan_fovx,
-                                             cam.tan_fovy, 3)
-    tight = {"value": round(W * H * args.steps / el_cull / 1e6, 3), "ms_per_step": round(1e3 * el_cull / args.steps, 4),
-             "pairs": int(cache[-1][0]),
-             "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(stages_cull.items(), key=lambda kv: -kv[1][0])}}
-    del cache
-    splatter.DEFAULT_CULL = False
+    tight = None
+    if not args.no_tight:
+        tight = tight_cull_run(args, lib, splatter, step, timed, prof_steps,
+                               (sh, logits, means, log_scales, quats, view, proj, campos, bg, W, H, cam))
 
     dominant = max(stages.items(), key=lambda kv: kv[1][0])
     d_name, (d_ms, d_launches, d_bytes) = dominant
@@ -239,6 +224,32 @@ def main():
         "raster": raster,
     }
     print(json.dumps(line))
+
+
+def tight_cull_run(args, lib, splatter, step, timed, prof_steps, frame):
+    """The training step with tight culling (dgmr_tiling.cull = 1): the same
+    image and gradients from fewer pairs.  The reference binning stays the
+    reported value, since the drop-in returns the reference's point list."""
+    import torch
+    sh, logits, means, log_scales, quats, view, proj, campos, bg, W, H, cam = frame
+    splatter.DEFAULT_CULL = True
+    el_cull = timed()
+    lib.dgmr_profile_enable(1)
+    for _ in range(prof_steps):
+        step()
+    torch.cuda.synchronize()
+    lib.dgmr_profile_enable(0)
+    stages_cull = stage_times(lib)
+    with torch.no_grad():
+        cache = splatter.cpp_binding.forward(sh, torch.sigmoid(logits), means, torch.exp(log_scales),
+                                             torch.nn.functional.normalize(quats, dim=-1), view, proj, campos, bg, W, H, cam.tan_fovx,
+                                             cam.tan_fovy, 3)
+    tight = {"value": round(W * H * args.steps / el_cull / 1e6, 3), "ms_per_step": round(1e3 * el_cull / args.steps, 4),
+             "pairs": int(cache[-1][0]),
+             "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(stages_cull.items(), key=lambda kv: -kv[1][0])}}
+    del cache
+    splatter.DEFAULT_CULL = False
+    return tight
 
 
 def adam_workload(params, reps: int = 20):
