@@ -51,22 +51,27 @@ __global__ __launch_bounds__(kPreThreads) void k_splat_preprocess(int64_t n, Spl
 // first owner and an upper bound of its last come from k_emit_bounds; the
 // owners in between (culled Gaussians carry the largest depth key, so the
 // owners of a block's pairs are contiguous and each owns at least one pair)
-// are staged in LDS with their tile rectangle, and every thread writes 8
-// consecutive pairs.  A window wider than the LDS stage (only possible with
-// zero-pair Gaussians among visible ones) takes a per-pair search instead.
+// are staged in LDS with their tile rectangle, and every thread writes
+// kEmitPer consecutive pairs.  A window wider than the LDS stage (more than
+// kEmitWindow owners in one block's pairs, or zero-pair Gaussians among
+// visible ones) takes a per-pair search instead.
 // views: per-view (grid_x, grid_y, first global tile, -) when ids span several
 // views of n_per_view Gaussians each; null for a single view.
 constexpr int kEmitThreads = 256;
-constexpr int kEmitPer = 8;
-constexpr int kEmitPairs = kEmitThreads * kEmitPer;
+// pairs per thread: 8, or 16 from kEmitLarge pairs up (the per-thread search
+// and setup spread over more pairs: 434M pairs 1.13 -> 1.06 ms; at 15M, with
+// half the blocks, 16 was slower, 0.042 against 0.032 ms)
+constexpr int kEmitPerSmall = 8, kEmitPerLarge = 16;
+constexpr int64_t kEmitLarge = int64_t(48) << 20;
+constexpr int kEmitWindow = 2048; // owners a block stages in LDS (more take the per-pair search)
 
 // First owners of the emission blocks: bounds[b] is the Gaussian (position
-// in depth order) that owns pair b * kEmitPairs, for every block that emits,
+// in depth order) that owns pair b * block_pairs, for every block that emits,
 // and bounds[n_blocks] owns the last emitted pair.  Position k owns pairs
 // [offsets[k], offsets[k + 1]) and writes the entries whose pair falls in its
 // range, so each entry is written exactly once and no block searches.
 __global__ void k_emit_bounds(int64_t n, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ n_pairs, uint32_t capacity,
-    uint32_t* __restrict__ bounds)
+    uint32_t block_pairs, uint32_t* __restrict__ bounds)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n)
@@ -77,10 +82,10 @@ __global__ void k_emit_bounds(int64_t n, const uint32_t* __restrict__ offsets, c
     const uint64_t hi = min(k + 1 < n ? uint64_t(offsets[k + 1]) : total, R);
     if (lo >= hi)
         return;
-    for (uint64_t b = (lo + kEmitPairs - 1) / kEmitPairs; b * kEmitPairs < hi; ++b)
+    for (uint64_t b = (lo + block_pairs - 1) / block_pairs; b * block_pairs < hi; ++b)
         bounds[b] = uint32_t(k);
     if (R - 1 < hi)
-        bounds[(R + kEmitPairs - 1) / kEmitPairs] = uint32_t(k);
+        bounds[(R + block_pairs - 1) / block_pairs] = uint32_t(k);
 }
 
 // tile rectangle of Gaussian id: first tile and (grid_x << 16 | width); 0 shape when it has no tiles
@@ -145,17 +150,32 @@ __global__ __launch_bounds__(256) void k_gather_owners(int64_t n, const uint32_t
     owner_rects[j] = make_uint2(start, shape);
 }
 
-// eight consecutive keys at a 16-byte aligned address: 32-bit keys in two
-// stores, 16-bit keys packed into one
-__device__ __forceinline__ void store_keys8(uint32_t* out, const uint32_t (&kv)[kEmitPer])
+// kEmitPer consecutive pairs at a 16-byte aligned position: 32-bit keys and
+// the values four to a store, 16-bit keys packed eight to a store
+template <int kEmitPer>
+__device__ __forceinline__ void store_run(uint32_t* keys, uint32_t* vals, const uint32_t (&kv)[kEmitPer], const uint32_t (&vv)[kEmitPer])
 {
-    uint4* d = reinterpret_cast<uint4*>(out);
-    d[0] = make_uint4(kv[0], kv[1], kv[2], kv[3]);
-    d[1] = make_uint4(kv[4], kv[5], kv[6], kv[7]);
+    static_assert(kEmitPer % 8 == 0, "pairs per thread: a multiple of 8");
+    uint4* kd = reinterpret_cast<uint4*>(keys);
+    uint4* vd = reinterpret_cast<uint4*>(vals);
+#pragma unroll
+    for (int j = 0; j < kEmitPer / 4; ++j) {
+        kd[j] = make_uint4(kv[4 * j], kv[4 * j + 1], kv[4 * j + 2], kv[4 * j + 3]);
+        vd[j] = make_uint4(vv[4 * j], vv[4 * j + 1], vv[4 * j + 2], vv[4 * j + 3]);
+    }
 }
-__device__ __forceinline__ void store_keys8(uint16_t* out, const uint32_t (&kv)[kEmitPer])
+template <int kEmitPer>
+__device__ __forceinline__ void store_run(uint16_t* keys, uint32_t* vals, const uint32_t (&kv)[kEmitPer], const uint32_t (&vv)[kEmitPer])
 {
-    *reinterpret_cast<uint4*>(out) = make_uint4(kv[0] | (kv[1] << 16), kv[2] | (kv[3] << 16), kv[4] | (kv[5] << 16), kv[6] | (kv[7] << 16));
+    uint4* kd = reinterpret_cast<uint4*>(keys);
+    uint4* vd = reinterpret_cast<uint4*>(vals);
+#pragma unroll
+    for (int j = 0; j < kEmitPer / 8; ++j)
+        kd[j] = make_uint4(kv[8 * j] | (kv[8 * j + 1] << 16), kv[8 * j + 2] | (kv[8 * j + 3] << 16), kv[8 * j + 4] | (kv[8 * j + 5] << 16),
+            kv[8 * j + 6] | (kv[8 * j + 7] << 16));
+#pragma unroll
+    for (int j = 0; j < kEmitPer / 4; ++j)
+        vd[j] = make_uint4(vv[4 * j], vv[4 * j + 1], vv[4 * j + 2], vv[4 * j + 3]);
 }
 
 // Tight culling: per position j of the depth order, the pair count of
@@ -238,12 +258,13 @@ __device__ __forceinline__ uint32_t cursor_tile(ReachCursor& cu, uint32_t* local
 
 // Tight-culling emission: the same block layout as k_emit_pairs; each thread
 // writes kEmitPer consecutive pairs, walking its owners' reached rows.
-template <class KeyT>
+template <class KeyT, int kEmitPer>
 __global__ __launch_bounds__(kEmitThreads) void k_emit_reach(const uint32_t* __restrict__ n_pairs, uint32_t capacity,
     const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ bounds, const uint32_t* __restrict__ offsets,
     const ReachRec* __restrict__ owners, int tile_w, int tile_h, KeyT* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
 {
-    __shared__ uint32_t s_off[kEmitPairs];
+    constexpr int kEmitPairs = kEmitThreads * kEmitPer;
+    __shared__ uint32_t s_off[kEmitWindow];
     const uint32_t R = min(*n_pairs, capacity);
     const uint32_t first = blockIdx.x * uint32_t(kEmitPairs);
     if (first >= R)
@@ -251,7 +272,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_reach(const uint32_t* __r
     const uint32_t end = min(R, first + uint32_t(kEmitPairs));
     const uint32_t g_lo = bounds[blockIdx.x], g_hi = bounds[blockIdx.x + 1];
     const uint32_t window = g_hi - g_lo + 1;
-    const bool staged = window <= uint32_t(kEmitPairs);
+    const bool staged = window <= uint32_t(kEmitWindow);
     if (staged) {
         for (uint32_t k = threadIdx.x; k < window; k += kEmitThreads)
             s_off[k] = offsets[g_lo + k];
@@ -303,10 +324,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_reach(const uint32_t* __r
     }
     const bool aligned = ((reinterpret_cast<uintptr_t>(keys_out) | reinterpret_cast<uintptr_t>(vals_out)) & 15u) == 0;
     if (aligned && p0 + kEmitPer <= end) {
-        store_keys8(keys_out + p0, kv);
-        uint4* vd = reinterpret_cast<uint4*>(vals_out + p0);
-        vd[0] = make_uint4(vv[0], vv[1], vv[2], vv[3]);
-        vd[1] = make_uint4(vv[4], vv[5], vv[6], vv[7]);
+        store_run(keys_out + p0, vals_out + p0, kv, vv);
     } else {
 #pragma unroll
         for (int q = 0; q < kEmitPer; ++q)
@@ -317,12 +335,13 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_reach(const uint32_t* __r
     }
 }
 
-template <class KeyT>
+template <class KeyT, int kEmitPer>
 __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const uint32_t* __restrict__ n_pairs, uint32_t capacity,
     const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ bounds, const uint32_t* __restrict__ offsets,
     const uint2* __restrict__ owner_rects, KeyT* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
 {
-    __shared__ uint32_t s_off[kEmitPairs], s_id[kEmitPairs], s_start[kEmitPairs], s_shape[kEmitPairs];
+    constexpr int kEmitPairs = kEmitThreads * kEmitPer;
+    __shared__ uint32_t s_off[kEmitWindow], s_id[kEmitWindow], s_start[kEmitWindow], s_shape[kEmitWindow];
     const uint32_t R = min(*n_pairs, capacity); // pairs past the capacity are not written
     const uint32_t first = blockIdx.x * uint32_t(kEmitPairs);
     if (first >= R)
@@ -332,7 +351,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const ui
     // of the last pair): the owners of this block's pairs lie in between
     const uint32_t g_lo = bounds[blockIdx.x], g_hi = bounds[blockIdx.x + 1];
     const uint32_t window = g_hi - g_lo + 1;
-    if (window <= uint32_t(kEmitPairs)) {
+    if (window <= uint32_t(kEmitWindow)) {
         for (uint32_t k = threadIdx.x; k < window; k += kEmitThreads) {
             const uint2 r = owner_rects[g_lo + k];
             s_off[k] = offsets[g_lo + k];
@@ -381,11 +400,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_pairs(int64_t n, const ui
         }
         const bool aligned = ((reinterpret_cast<uintptr_t>(keys_out) | reinterpret_cast<uintptr_t>(vals_out)) & 15u) == 0;
         if (aligned && p0 + kEmitPer <= end) {
-            // 16-byte aligned: p0 is a multiple of 8
-            store_keys8(keys_out + p0, kv);
-            uint4* vd = reinterpret_cast<uint4*>(vals_out + p0);
-            vd[0] = make_uint4(vv[0], vv[1], vv[2], vv[3]);
-            vd[1] = make_uint4(vv[4], vv[5], vv[6], vv[7]);
+            // 16-byte aligned: p0 is a multiple of kEmitPer
+            store_run(keys_out + p0, vals_out + p0, kv, vv);
         } else {
 #pragma unroll
             for (int q = 0; q < kEmitPer; ++q)
@@ -636,7 +652,7 @@ static SplatWorkspace carve_splat(Carver& cv, int64_t n, int64_t cap, bool cull)
     w.pvals_a = cv.take<uint32_t>(cc);
     w.pkeys_b = cv.take<uint32_t>(cc);
     w.counter = cv.take<uint32_t>(64);
-    w.emit_bounds = cv.take<uint32_t>(cc / kEmitPairs + 2);
+    w.emit_bounds = cv.take<uint32_t>(cc / (kEmitThreads * kEmitPerSmall) + 2);
     w.owner_reach = reinterpret_cast<ReachRec*>(cv.take<char>(nn * rec));
     w.owner_rects = reinterpret_cast<uint2*>(w.owner_reach);
     w.sort_scratch = cv.take<char>(radix_scratch_bytes(std::max(nn, cc)));
@@ -733,18 +749,26 @@ static int emit_sort_ranges(const EmitArgs& a, const SplatWorkspace& w, dgmr_bin
         pb = SortBuffersT<KeyT> { ka, bins->point_list, kb, w.pvals_a };
     else
         pb = SortBuffersT<KeyT> { ka, w.pvals_a, kb, bins->point_list };
-    const unsigned emit_blocks = unsigned(ceil_div64(a.cap, kEmitPairs));
+    const int per = int64_t(a.cap) >= kEmitLarge ? kEmitPerLarge : kEmitPerSmall;
+    const uint32_t block_pairs = uint32_t(kEmitThreads * per);
+    const unsigned emit_blocks = unsigned(ceil_div64(a.cap, block_pairs));
     {
         StageScope sc("bin.emit", st);
         pending->add_stage(sc.slot, 4.0 + sizeof(KeyT), 28.0 * double(a.n));
-        k_emit_bounds<<<a.blocks_n, 256, 0, st>>>(a.n, w.pair_offsets, w.counter, a.cap, w.emit_bounds);
+        k_emit_bounds<<<a.blocks_n, 256, 0, st>>>(a.n, w.pair_offsets, w.counter, a.cap, block_pairs, w.emit_bounds);
         DGMR_HIP_CHECK(hipGetLastError());
-        if (a.cull)
-            k_emit_reach<KeyT><<<emit_blocks, kEmitThreads, 0, st>>>(w.counter, a.cap, a.sorted_ids, w.emit_bounds, w.pair_offsets,
-                w.owner_reach, a.tile_w, a.tile_h, pb.keys_a, pb.vals_a);
+        if (a.cull && per == kEmitPerLarge)
+            k_emit_reach<KeyT, kEmitPerLarge><<<emit_blocks, kEmitThreads, 0, st>>>(w.counter, a.cap, a.sorted_ids, w.emit_bounds,
+                w.pair_offsets, w.owner_reach, a.tile_w, a.tile_h, pb.keys_a, pb.vals_a);
+        else if (a.cull)
+            k_emit_reach<KeyT, kEmitPerSmall><<<emit_blocks, kEmitThreads, 0, st>>>(w.counter, a.cap, a.sorted_ids, w.emit_bounds,
+                w.pair_offsets, w.owner_reach, a.tile_w, a.tile_h, pb.keys_a, pb.vals_a);
+        else if (per == kEmitPerLarge)
+            k_emit_pairs<KeyT, kEmitPerLarge><<<emit_blocks, kEmitThreads, 0, st>>>(a.n, w.counter, a.cap, a.sorted_ids, w.emit_bounds,
+                w.pair_offsets, w.owner_rects, pb.keys_a, pb.vals_a);
         else
-            k_emit_pairs<KeyT><<<emit_blocks, kEmitThreads, 0, st>>>(a.n, w.counter, a.cap, a.sorted_ids, w.emit_bounds, w.pair_offsets,
-                w.owner_rects, pb.keys_a, pb.vals_a);
+            k_emit_pairs<KeyT, kEmitPerSmall><<<emit_blocks, kEmitThreads, 0, st>>>(a.n, w.counter, a.cap, a.sorted_ids, w.emit_bounds,
+                w.pair_offsets, w.owner_rects, pb.keys_a, pb.vals_a);
         DGMR_HIP_CHECK(hipGetLastError());
     }
     {

@@ -135,6 +135,24 @@ class ServerStats:
                 "requests": len(self.latencies), "batches": self.batches, "wall_s": self.wall}
 
 
+# Worker streams, one per (device, worker index), kept for the process: the
+# caching allocator reuses a freed block only on the stream that allocated
+# it, so fresh streams per server would leave every earlier server's
+# workspaces unusable (reserved memory grew by 60 GB per saturated run until
+# an allocation retry freed everything, a 5.8 s stall).
+_STREAMS = {}
+_STREAMS_LOCK = threading.Lock()
+
+
+def _worker_stream(device: torch.device, k: int) -> torch.cuda.Stream:
+    with _STREAMS_LOCK:
+        key = (str(device), k)
+        st = _STREAMS.get(key)
+        if st is None:
+            st = _STREAMS[key] = torch.cuda.Stream(device=device)  # default priority
+        return st
+
+
 class ContinuousBatchingServer:
     def __init__(self, resident: List[ResidentScene], hbm_budget_bytes: int, workers: int = 3, max_batch_views: int = 64,
                  tile=(16, 16), lds_batch: int = 64, device="cuda:0"):
@@ -157,8 +175,8 @@ class ContinuousBatchingServer:
         return time.perf_counter() - self._t0
 
     # ---------------------------------------------------------------- workers
-    def _worker(self):
-        stream = torch.cuda.Stream(device=self.device)
+    def _worker(self, k: int):
+        stream = _worker_stream(self.device, k)
         splatter = BatchSplatter(tile=self.tile, batch=self.lds_batch, device=self.device)
         while True:
             with self._lock:
@@ -240,7 +258,7 @@ class ContinuousBatchingServer:
 
     def run(self, trace: List[Request]) -> ServerStats:
         self._t0 = time.perf_counter()
-        workers = [threading.Thread(target=self._worker, daemon=True) for _ in range(self.n_workers)]
+        workers = [threading.Thread(target=self._worker, args=(k,), daemon=True) for k in range(self.n_workers)]
         for w in workers:
             w.start()
         pending = deque(sorted(trace, key=lambda r: r.arrival))
