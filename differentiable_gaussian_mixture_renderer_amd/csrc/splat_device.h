@@ -488,10 +488,11 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
                     const bool blend = live[k] && alpha >= 1.0f / 255.0f;
                     const bool stop = blend && test_T < 0.0001f;
                     const bool use = blend && !stop;
-                    const float wgt = alpha * T[k];
-                    C0[k] = use ? fmaf(c.z, wgt, C0[k]) : C0[k];
-                    C1[k] = use ? fmaf(c.w, wgt, C1[k]) : C1[k];
-                    C2[k] = use ? fmaf(blue, wgt, C2[k]) : C2[k];
+                    // one select on the weight: fma(c, 0, C) is C exactly (colours are finite, >= 0)
+                    const float wgt = use ? alpha * T[k] : 0.f;
+                    C0[k] = fmaf(c.z, wgt, C0[k]);
+                    C1[k] = fmaf(c.w, wgt, C1[k]);
+                    C2[k] = fmaf(blue, wgt, C2[k]);
                     T[k] = use ? test_T : T[k];
                     live[k] = live[k] && !stop;
                     still |= live[k];
@@ -658,12 +659,13 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                     const float raw = c.y * Gk[k];
                     const float tk = 1.f - alphak[k];
                     const float test_T = T[k] * tk;
-                    const float wgt = alphak[k] * T[k];
+                    // one select on the weight: fma(x, 0, s) is s exactly for finite x
+                    const float wgt = use[k] ? alphak[k] * T[k] : 0.f;
                     const float grgb = gC0[k] * c.z + gC1[k] * c.w + gC2[k] * blue; // <dL/dC, rgb>
-                    D[k] = use[k] ? fmaf(-grgb, wgt, D[k]) : D[k];
-                    sr = use[k] ? fmaf(gC0[k], wgt, sr) : sr;
-                    sg = use[k] ? fmaf(gC1[k], wgt, sg) : sg;
-                    sbl = use[k] ? fmaf(gC2[k], wgt, sbl) : sbl;
+                    D[k] = fmaf(-grgb, wgt, D[k]);
+                    sr = fmaf(gC0[k], wgt, sr);
+                    sg = fmaf(gC1[k], wgt, sg);
+                    sbl = fmaf(gC2[k], wgt, sbl);
                     // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
                     const float inv_tk = __builtin_amdgcn_rcpf(tk);
                     const float g_alpha = grgb * T[k] - D[k] * inv_tk;
