@@ -124,12 +124,15 @@ hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t 
 namespace {
 
 constexpr int kSortThreads = 256;
-// items per thread: 16 (4 096 per block) below kLargeSort keys, 32 above,
-// where the longer per-digit runs of an 8 192-item block write more whole
-// lines (433M keys: 6.6 -> 5.4 ms) and the grid is large enough to fill the
-// chip at the lower occupancy (15M keys are faster with 16)
+// items per thread: 8 (2 048 per block) below kSmallSort keys, so small
+// sorts still give every CU several workgroups; 16 (4 096) below kLargeSort;
+// 32 above, where the longer per-digit runs of an 8 192-item block write more
+// whole lines (433M keys: 6.6 -> 5.4 ms) and the grid is large enough to fill
+// the chip at the lower occupancy (15M keys are faster with 16)
+constexpr int kTinyItems = 8;
 constexpr int kSmallItems = 16;
 constexpr int kLargeItems = 32;
+constexpr int64_t kSmallSort = int64_t(4) << 20;
 constexpr int64_t kLargeSort = int64_t(48) << 20;
 constexpr int kRadixMax = 256;
 
@@ -409,10 +412,25 @@ static int digit_bits(int begin_bit, int end_bit)
 
 size_t radix_scratch_bytes(int64_t n_max)
 {
-    // sized for the smaller tile, so it covers any sort of up to n_max keys
-    const int64_t blocks = std::max<int64_t>(1, ceil_div(n_max, int64_t(kSortThreads) * kSmallItems));
+    // sized for the smallest tile, so it covers any sort of up to n_max keys
+    const int64_t blocks = std::max<int64_t>(1, ceil_div(n_max, int64_t(kSortThreads) * kTinyItems));
     const size_t hist = size_t(blocks) * kRadixMax * sizeof(uint32_t);
     return ((hist + 255) / 256) * 256 + scan_scratch_bytes(blocks * kRadixMax);
+}
+
+template <int ITEMS, class KeyT>
+static void radix_pass(int blocks, const KeyT* kin, const uint32_t* vin, KeyT* kout, uint32_t* vout, int64_t n_max, const uint32_t* n_dev,
+    int shift, int bits, uint32_t* hist, void* scan_scratch, KeyOut ko, hipStream_t stream, hipError_t* err)
+{
+    k_radix_hist<ITEMS, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist);
+    *err = scan_u32(hist, hist, nullptr, int64_t(blocks) << bits, false, scan_scratch, stream);
+    if (*err != hipSuccess)
+        return;
+    if (bits <= 7)
+        k_radix_scatter<ITEMS, KeyT, 128><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist, ko);
+    else
+        k_radix_scatter<ITEMS, KeyT, 256><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist, ko);
+    *err = hipGetLastError();
 }
 
 template <class KeyT>
@@ -423,8 +441,11 @@ static hipError_t radix_sort_impl(const SortBuffersT<KeyT>& buf, int64_t n_max, 
     if (n_max <= 0 || passes == 0)
         return hipSuccess;
     const int width = digit_bits(begin_bit, end_bit);
-    const bool large = n_max >= kLargeSort; // 16-bit keys too: 4 096-item blocks sort 434M of them in 5.36 ms, 8 192 in 5.13
-    const int blocks = int(std::max<int64_t>(1, ceil_div(n_max, int64_t(kSortThreads) * (large ? kLargeItems : kSmallItems))));
+    // items per block: 2 048 below 4M keys (1M keys in 4 096-item blocks left
+    // the 256 CUs with one workgroup each), 4 096, and 8 192 from 48M keys up
+    // (16-bit keys too: 434M of them sort in 5.13 ms against 5.36 with 4 096)
+    const int items = n_max < kSmallSort ? kTinyItems : (n_max < kLargeSort ? kSmallItems : kLargeItems);
+    const int blocks = int(std::max<int64_t>(1, ceil_div(n_max, int64_t(kSortThreads) * items)));
     uint32_t* hist = static_cast<uint32_t*>(scratch);
     void* scan_scratch = static_cast<char*>(scratch) + ((size_t(blocks) * kRadixMax * sizeof(uint32_t) + 255) / 256) * 256;
     for (int p = 0; p < passes; ++p) {
@@ -434,27 +455,14 @@ static hipError_t radix_sort_impl(const SortBuffersT<KeyT>& buf, int64_t n_max, 
         const uint32_t* vin = (p % 2 == 0) ? buf.vals_a : buf.vals_b;
         KeyT* kout = (p % 2 == 0) ? buf.keys_b : buf.keys_a;
         uint32_t* vout = (p % 2 == 0) ? buf.vals_b : buf.vals_a;
-        if (large)
-            k_radix_hist<kLargeItems, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist);
-        else
-            k_radix_hist<kSmallItems, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist);
-        hipError_t e = scan_u32(hist, hist, nullptr, int64_t(blocks) << bits, false, scan_scratch, stream);
-        if (e != hipSuccess)
-            return e;
         const KeyOut ko = p + 1 == passes ? last : KeyOut {};
-        if (large && bits <= 7)
-            k_radix_scatter<kLargeItems, KeyT, 128><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks,
-                hist, ko);
-        else if (large)
-            k_radix_scatter<kLargeItems, KeyT, 256><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks,
-                hist, ko);
-        else if (bits <= 7)
-            k_radix_scatter<kSmallItems, KeyT, 128><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks,
-                hist, ko);
+        hipError_t e = hipSuccess;
+        if (items == kTinyItems)
+            radix_pass<kTinyItems>(blocks, kin, vin, kout, vout, n_max, n_dev, shift, bits, hist, scan_scratch, ko, stream, &e);
+        else if (items == kSmallItems)
+            radix_pass<kSmallItems>(blocks, kin, vin, kout, vout, n_max, n_dev, shift, bits, hist, scan_scratch, ko, stream, &e);
         else
-            k_radix_scatter<kSmallItems, KeyT, 256><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks,
-                hist, ko);
-        e = hipGetLastError();
+            radix_pass<kLargeItems>(blocks, kin, vin, kout, vout, n_max, n_dev, shift, bits, hist, scan_scratch, ko, stream, &e);
         if (e != hipSuccess)
             return e;
     }
