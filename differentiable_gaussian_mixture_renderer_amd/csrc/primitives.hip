@@ -128,7 +128,8 @@ constexpr int kSortThreads = 256;
 // sorts still give every CU several workgroups; 16 (4 096) below kLargeSort;
 // 32 above, where the longer per-digit runs of an 8 192-item block write more
 // whole lines (433M keys: 6.6 -> 5.4 ms) and the grid is large enough to fill
-// the chip at the lower occupancy (15M keys are faster with 16)
+// the chip at the lower occupancy (15M keys are faster with 16: 0.155 ms,
+// against 0.171 with 8 and more with 32)
 constexpr int kTinyItems = 8;
 constexpr int kSmallItems = 16;
 constexpr int kLargeItems = 32;
