@@ -81,3 +81,57 @@ def _load_fresh(N, path):
         N.load(path)
     finally:
         N._lib = saved
+
+
+def test_tight_culling_flag_validation_and_workspace():
+    """dgmr_tiling.cull takes 0 or 1; with 1 the workspace holds 32-byte
+    reach records (preprocess and depth-ordered copies) instead of 8-byte
+    rectangles; the marcher refuses it."""
+    from differentiable_gaussian_mixture_renderer_amd import _native as N
+    lib = N.load()
+    n, cap = 100_000, 1_000_000
+    sizes = {}
+    for cull in (0, 1):
+        t = N.Tiling(16, 16, 64, cull)
+        size = ctypes.c_size_t(0)
+        assert lib.dgmr_workspace_size(n, 256, 256, ctypes.byref(t), cap, ctypes.byref(size)) == N.DGMR_OK
+        sizes[cull] = size.value
+    assert sizes[1] - sizes[0] >= 2 * 24 * n
+    bad = N.Tiling(16, 16, 64, 2)
+    size = ctypes.c_size_t(0)
+    assert lib.dgmr_workspace_size(n, 256, 256, ctypes.byref(bad), cap, ctypes.byref(size)) == N.DGMR_ERR_ARGUMENT
+    g = N.Gaussians()
+    g.n = 10
+    g.dtype = N.DGMR_F32
+    g.sh_coeffs, g.sh_degree = 1, 0
+    cam = N.Camera()
+    cam.width = cam.height = 64
+    opt = N.MarchOptions()
+    geom = N.MarchGeometry()
+    bins = N.Binning()
+    tight = N.Tiling(16, 16, 64, 1)
+    buf = ctypes.create_string_buffer(64)
+    rc = lib.dgmr_march_forward(ctypes.byref(g), ctypes.byref(cam), ctypes.byref(tight), ctypes.byref(opt), ctypes.byref(geom),
+                                ctypes.byref(bins), buf, buf, buf, 64, None, None)
+    assert rc == N.DGMR_ERR_ARGUMENT
+    assert b"bad tiling" in lib.dgmr_last_error()
+
+
+def test_l1_many_argument_checks_without_gpu():
+    """losses.l1_grad_many rejects mismatched shapes and dtypes before any
+    launch; the C entry point rejects null and misaligned arrays."""
+    import torch
+    from differentiable_gaussian_mixture_renderer_amd import _native as N
+    from differentiable_gaussian_mixture_renderer_amd import losses
+    with pytest.raises(ValueError):
+        losses.l1_grad_many([torch.zeros(3, 4, 4)], [torch.zeros(3, 4, 5)])
+    with pytest.raises(ValueError):
+        losses.l1_grad_many([torch.zeros(3, 4, 4, dtype=torch.float64)], [torch.zeros(3, 4, 4, dtype=torch.float64)])
+    assert losses.l1_grad_many([], []) == []
+    lib = N.load()
+    items = (N.L1Item * 1)()
+    assert lib.dgmr_l1_loss_backward_many(items, 1, None) == N.DGMR_ERR_ARGUMENT  # null arrays
+    items[0].image = items[0].target = items[0].grad = items[0].grad_loss = 4100  # not 16-byte aligned
+    items[0].n = 8
+    assert lib.dgmr_l1_loss_backward_many(items, 1, None) == N.DGMR_ERR_ARGUMENT
+    assert lib.dgmr_l1_loss_backward_many(items, 0, None) == N.DGMR_OK  # nothing to do

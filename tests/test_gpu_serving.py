@@ -48,3 +48,29 @@ def test_results_same_alone_and_in_the_mix():
         else:
             assert r.grads is None
     torch.cuda.synchronize()
+
+
+def test_servers_share_worker_streams():
+    """A second server reuses the first one's worker streams (the caching
+    allocator reuses freed blocks only on their own stream), so repeated
+    servers do not keep reserving new memory."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import torch
+    from differentiable_gaussian_mixture_renderer_amd import serving
+    resident = serving.make_resident_scenes(2, seed=9, n_min=5_000, n_max=20_000)
+    print("[gpu serving] two servers in a row", flush=True)
+    reserved = []
+    for _ in range(2):
+        trace = serving.make_trace(n_requests=12, rate_hz=5_000.0, n_scenes=2, seed=4, backward_fraction=0.3, min_res=(64, 64),
+                                   max_res=(128, 128))
+        for r in trace:
+            r.arrival = 0.0
+        stats = serving.ContinuousBatchingServer(resident, hbm_budget_bytes=1 << 30, workers=2, max_batch_views=8).run(trace)
+        torch.cuda.synchronize()
+        assert len(stats.latencies) == len(trace)
+        reserved.append(torch.cuda.memory_reserved())
+    dev = torch.device("cuda:0")
+    assert serving._worker_stream(dev, 0) is serving._worker_stream(dev, 0)
+    assert sum(1 for key in serving._STREAMS if key[0] == str(dev)) >= 2
+    assert reserved[1] <= reserved[0] * 1.5 + (64 << 20)
