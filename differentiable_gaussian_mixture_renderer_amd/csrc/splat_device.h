@@ -257,20 +257,26 @@ template <bool REST_READY>
 __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o, const SplatParams& p, const float* c, const DevCamera<float>& cam,
     const SplatOut& out)
 {
-    out.tiles[o] = 0;
-    out.rects[2 * o] = 0;
-    out.rects[2 * o + 1] = 0;
-    out.xy[2 * o] = 0.f;
-    out.xy[2 * o + 1] = 0.f;
-    out.depth_keys[o] = 0xffffffffu;
     out.ids[o] = uint32_t(o);
-    if (!out.reach) // reach records are read only for entries with a depth key
-        out.bin_rects[o] = make_uint2(0u, 0u);
+    // a culled entry: no tiles, centre 0, the culled depth key (each output is
+    // written once, here or on the visible path)
+    const auto culled = [&]() {
+        out.tiles[o] = 0;
+        out.xy[2 * o] = 0.f;
+        out.xy[2 * o + 1] = 0.f;
+        out.depth_keys[o] = 0xffffffffu;
+        if (!out.reach) // reach records are read only for entries with a depth key
+            out.bin_rects[o] = make_uint2(0u, 0u);
+    };
     const float* mu = g.mu;
     float t[3];
     transform_point(cam.view, mu, t, 3);
-    if (t[2] < 0.2f)
+    if (t[2] < 0.2f) {
+        out.rects[2 * o] = 0;
+        out.rects[2 * o + 1] = 0;
+        culled();
         return;
+    }
     if (!REST_READY)
         gauss_rest(p, i, g);
     const float w = g.w;
@@ -290,8 +296,10 @@ __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o
     int x0, y0, x1, y1;
     get_rect(xy[0], xy[1], ext_x, ext_y, cam.grid_x, cam.grid_y, cam.tile_w, cam.tile_h, &x0, &y0, &x1, &y1);
     const uint32_t touched = uint32_t(x1 - x0) * uint32_t(y1 - y0);
-    if (touched == 0)
+    if (touched == 0) {
+        culled(); // the extents stay in rects, as the reference leaves them
         return;
+    }
     out.tiles[o] = touched;
     if (!out.reach)
         out.bin_rects[o] = make_uint2(uint32_t(y0) * uint32_t(cam.grid_x) + uint32_t(x0), (uint32_t(y1 - y0) << 16) | uint32_t(x1 - x0));
