@@ -225,13 +225,11 @@ struct BatchLayout {
     ViewBlend* views;
     int4* grids;
     uint32_t* queue;
-    int32_t* rects;
     uint32_t* tiles;
     float* xy;
     float4* conic_opacity;
     float* rgb;
     uint8_t* clamped;
-    float* depths;
     uint32_t* point_list;
     uint32_t* ranges;
     float* screen_grads;
@@ -254,13 +252,11 @@ BatchLayout carve_batch(void* ws, int64_t n, const dgmr_batch_view* views, int n
     L.views = cv.take<ViewBlend>(n_views);
     L.grids = cv.take<int4>(n_views);
     L.queue = cv.take<uint32_t>(64);
-    L.rects = cv.take<int32_t>(2 * m);
     L.tiles = cv.take<uint32_t>(m);
     L.xy = cv.take<float>(2 * m);
     L.conic_opacity = cv.take<float4>(m);
     L.rgb = cv.take<float>(3 * m);
     L.clamped = cv.take<uint8_t>(3 * m);
-    L.depths = cv.take<float>(m);
     L.point_list = cv.take<uint32_t>(std::max<int64_t>(cap, 1));
     L.ranges = cv.take<uint32_t>(2 * std::max<int64_t>(total_tiles, 1));
     L.screen_grads = cv.take<float>(9 * m);
@@ -367,11 +363,11 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     void* recs;
     binning_key_buffers(L.binning, m, cull, &dkeys, &dids, &recs);
     if (n > 0) {
-        StageScope sc("batch.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 67.0);
+        StageScope sc("batch.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 55.0);
         const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
             static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), g->activation };
         k_batch_preprocess<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams,
-            SplatOut { L.rects, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, L.depths, dkeys, dids,
+            SplatOut { nullptr, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, nullptr, dkeys, dids,
                 cull ? nullptr : static_cast<uint2*>(recs), cull ? static_cast<ReachRec*>(recs) : nullptr });
         DGMR_HIP_CHECK(hipGetLastError());
     }
@@ -385,7 +381,7 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     const BatchBinning bb { L.grids, std::max<int64_t>(n, 1), n_views, L.total_tiles, max_view_tiles };
     PendingCount pending;
     if (m > 0) {
-        const int rc = bin_gaussians(m, L.rects, L.tiles, L.xy, false, true, cull, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &pending, st, &bb);
+        const int rc = bin_gaussians(m, nullptr, L.tiles, L.xy, false, true, cull, 0, 0, tiling->tile_w, tiling->tile_h, &bins, cv, &pending, st, &bb);
         if (rc != DGMR_OK)
             return rc;
     } else {

@@ -133,7 +133,9 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     const BatchBinning* batch);
 // the preprocess's outputs in the workspace: depth keys, ids and the
 // per-Gaussian records (cull: ReachRec; otherwise uint2 bin rectangles
-// (first tile y0 * grid_x + x0, (y1 - y0) << 16 | (x1 - x0)), zero when culled)
+// (first tile y0 * grid_x + x0, (y1 - y0) << 16 | (x1 - x0)), zero when culled).
+// ids comes back null when the binning numbers the entries itself (large n):
+// the preprocess then writes none
 void binning_key_buffers(void* workspace, int64_t n, bool cull, uint32_t** keys, uint32_t** ids, void** records = nullptr);
 size_t splat_workspace_bytes(int64_t n, int64_t cap, bool cull = false);
 

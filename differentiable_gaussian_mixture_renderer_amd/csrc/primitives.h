@@ -55,6 +55,7 @@ hipError_t radix_sort_pairs(const SortBuffers16& buf, int64_t n_max, const uint3
 // and block_offsets hold compact_blocks(n) entries; scratch holds
 // scan_scratch_bytes(compact_blocks(n)) bytes.
 int64_t compact_blocks(int64_t n);
+// vals_in may be null: the values are then the input indices
 hipError_t compact_pairs(const uint32_t* keys_in, const uint32_t* vals_in, int64_t n, uint32_t skip_key, uint32_t* keys_out,
     uint32_t* vals_out, uint32_t* count, uint32_t* block_counts, uint32_t* block_offsets, void* scratch, hipStream_t stream);
 

@@ -339,7 +339,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_count(const uint32_
 
 // Each wave owns a contiguous quarter of the block's items and walks it in
 // rounds of 64; kept items take ranks by ballot prefix, so the output keeps
-// the input order.
+// the input order.  vals null: each item's value is its input index.
 __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     int64_t n, uint32_t skip_key, const uint32_t* __restrict__ block_offsets, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
 {
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(const uint3
     for (int r = 0; r < kRounds; ++r) {
         const int64_t i = base + int64_t(wave) * (kRounds * 64) + r * 64 + lane;
         k[r] = i < n ? keys[i] : skip_key;
-        v[r] = i < n ? vals[i] : 0u;
+        v[r] = i < n ? (vals ? vals[i] : uint32_t(i)) : 0u;
         keep[r] = i < n && k[r] != skip_key;
         wave_count += uint32_t(__popcll(__ballot(keep[r])));
     }

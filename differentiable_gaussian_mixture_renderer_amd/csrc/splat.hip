@@ -807,7 +807,8 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     if (n >= kCompactMin) {
         StageScope sc("bin.compact", st, 16.0 * double(n));
         uint32_t* count = w.counter + 1;
-        DGMR_HIP_CHECK(compact_pairs(K, V, n, 0xffffffffu, K2, V2, count, w.counts, w.pair_offsets, w.scan_scratch, st));
+        // the preprocess left the ids unwritten: the compaction numbers the entries
+        DGMR_HIP_CHECK(compact_pairs(K, nullptr, n, 0xffffffffu, K2, V2, count, w.counts, w.pair_offsets, w.scan_scratch, st));
         std::swap(K, K2);
         std::swap(V, V2);
         n_sorted = count;
@@ -885,7 +886,8 @@ void binning_key_buffers(void* workspace, int64_t n, bool cull, uint32_t** keys,
     Carver cv(workspace, ~size_t(0));
     const SplatWorkspace w = carve_splat(cv, n, 1, cull);
     *keys = w.dkeys_a;
-    *ids = w.dvals_a;
+    // sorts that compact first take the entry index as the id
+    *ids = n >= kCompactMin ? nullptr : w.dvals_a;
     if (records)
         *records = w.reach;
 }

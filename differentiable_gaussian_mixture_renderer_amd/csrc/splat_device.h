@@ -173,6 +173,9 @@ __device__ __forceinline__ ReachRec make_reach(int x0, int y0, int cols, int row
 }
 
 // per-Gaussian outputs of the preprocess; index o = view * n + i
+// rects, depths and ids may be null (not written): the batched path reads
+// neither extents nor depths, and ids are not needed when the binning's
+// compaction numbers the entries itself
 struct SplatOut {
     int32_t* rects;
     uint32_t* tiles;
@@ -257,7 +260,8 @@ template <bool REST_READY>
 __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o, const SplatParams& p, const float* c, const DevCamera<float>& cam,
     const SplatOut& out)
 {
-    out.ids[o] = uint32_t(o);
+    if (out.ids)
+        out.ids[o] = uint32_t(o);
     // a culled entry: no tiles, centre 0, the culled depth key (each output is
     // written once, here or on the visible path)
     const auto culled = [&]() {
@@ -272,8 +276,10 @@ __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o
     float t[3];
     transform_point(cam.view, mu, t, 3);
     if (t[2] < 0.2f) {
-        out.rects[2 * o] = 0;
-        out.rects[2 * o + 1] = 0;
+        if (out.rects) {
+            out.rects[2 * o] = 0;
+            out.rects[2 * o + 1] = 0;
+        }
         culled();
         return;
     }
@@ -291,8 +297,10 @@ __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o
     project_to_screen(mu, cam, xy);
     const int ext_x = guarded_int(ceilf(3.f * sqrt_rn(cov_f[0])));
     const int ext_y = guarded_int(ceilf(3.f * sqrt_rn(cov_f[2])));
-    out.rects[2 * o] = ext_x;
-    out.rects[2 * o + 1] = ext_y;
+    if (out.rects) {
+        out.rects[2 * o] = ext_x;
+        out.rects[2 * o + 1] = ext_y;
+    }
     int x0, y0, x1, y1;
     get_rect(xy[0], xy[1], ext_x, ext_y, cam.grid_x, cam.grid_y, cam.tile_w, cam.tile_h, &x0, &y0, &x1, &y1);
     const uint32_t touched = uint32_t(x1 - x0) * uint32_t(y1 - y0);
@@ -307,7 +315,8 @@ __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o
     out.xy[2 * o + 1] = xy[1];
     const float dv[3] = { cam.pos[0] - mu[0], cam.pos[1] - mu[1], cam.pos[2] - mu[2] };
     const float depth = length3(dv);
-    out.depths[o] = depth;
+    if (out.depths)
+        out.depths[o] = depth;
     out.depth_keys[o] = __float_as_uint(depth);
     // colour from SH along normalize(mu - cam) (math.h:233-271)
     // the colour, conic and weight feed only the blend: hardware reciprocal
