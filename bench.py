@@ -189,7 +189,12 @@ def main():
     del means, log_scales, quats, logits, sh, params, target
     torch.cuda.empty_cache()
     batched = None if args.no_batched else batched_workload(args.n, args.batched_cams)
+    # each section starts from an empty allocator cache: blocks cached on one
+    # stream cannot serve another stream's requests, and a full cache makes
+    # the allocator free everything and retry in the middle of a timed run
+    torch.cuda.empty_cache()
     served = None if args.no_serving else serving_workload(args.serving_requests)
+    torch.cuda.empty_cache()
     marcher = None if args.no_marcher else marcher_workload(lib, with_cpu=not args.no_cpu)
     raster = None if args.no_raster else raster_workload(lib, with_cpu=not args.no_cpu)
 
@@ -304,7 +309,7 @@ def batched_workload(n: int, n_cams: int, reps: int = 20):
     bs = BatchSplatter(tile=tile, batch=lds)
 
     def batched_step():
-        # forward, one-launch L1 gradient of every view, backward
+        # forward, then the backward with the L1 gradient formed inside the blend backward
         return bs.step_l1(*params, cams, targets)
 
     batched_step()
@@ -475,6 +480,7 @@ def serving_workload(n_requests: int, rate_hz: float = 200.0):
     out = {"workload": f"{n_requests} requests, Poisson {rate_hz:g}/s, 8 resident scenes of 50k..3M Gaussians, 256^2..1080p, "
                        "30% fwd+bwd", "scenes_gaussians": [sc.n for sc in resident]}
     budget = 48 << 30
+    retries0 = torch.cuda.memory_stats().get("num_alloc_retries", 0)
 
     def run(server_cls, reqs, **kw):
         srv = server_cls(resident, **kw)
@@ -506,6 +512,8 @@ def serving_workload(n_requests: int, rate_hz: float = 200.0):
     out["continuous_saturated"] = saturated(serving.ContinuousBatchingServer, hbm_budget_bytes=budget)
     out["fifo_plain_saturated"] = saturated(serving.FifoServer)
     out["saturated_throughput_ratio"] = round(out["continuous_saturated"]["mpixel_per_s"] / out["fifo_plain_saturated"]["mpixel_per_s"], 3)
+    # times the caching allocator had to free its cache and retry (each stalls every stream)
+    out["allocator_retries"] = int(torch.cuda.memory_stats().get("num_alloc_retries", 0) - retries0)
     return out
 
 

@@ -122,6 +122,9 @@ class BatchView(ctypes.Structure):
         ("framebuffer", c_void_p),
         ("final_transmittance", c_void_p),
         ("grad_framebuffer", c_void_p),
+        ("l1_target", c_void_p),
+        ("l1_scale", ctypes.c_float),
+        ("reserved", ctypes.c_int32),
     ]
 
 

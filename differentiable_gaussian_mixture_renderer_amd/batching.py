@@ -19,7 +19,6 @@ import torch
 
 from . import _common as C
 from . import _native as N
-from . import losses
 
 
 def _cam_struct(cam, bg):
@@ -48,6 +47,7 @@ class BatchSplatter:
             arr[v].framebuffer = fbs[v].data_ptr()
             arr[v].final_transmittance = Ts[v].data_ptr()
             arr[v].grad_framebuffer = None
+            arr[v].l1_target = None
         return arr
 
     def _prepare(self, shs, opacity, means, scales, rotations, cameras, bg, sh_degree):
@@ -100,12 +100,13 @@ class BatchSplatter:
     def step_l1(self, shs, opacity, means, scales, rotations, cameras: Sequence, targets: Sequence[torch.Tensor], bg=(0.0, 0.0, 0.0),
                 sh_degree: int = 3):
         """One multi-view training step with the L1 loss: the forward of every
-        view, dL/dimage = sign(image - target) / numel for each view (one
-        launch), and the gradients summed over the views.  Equals
-        forward + losses.l1_grad_many + backward; every buffer and descriptor
-        is prepared before the forward waits for its pair count, so the loss
-        and backward launches follow that wait at once.  Returns (images,
-        (g_sh, g_opacity, g_means, g_scales, g_rotations))."""
+        view and the gradients summed over the views, with dL/dimage =
+        sign(image - target) / numel formed inside the blend backward from
+        the image and the target (dgmr_batch_view.l1_target), so no gradient
+        image is written or read.  Equals forward + losses.l1_grad_many +
+        backward; every descriptor is prepared before the forward waits for
+        its pair count, so the backward launches follow that wait at once.
+        Returns (images, (g_sh, g_opacity, g_means, g_scales, g_rotations))."""
         dev = self._prepare(shs, opacity, means, scales, rotations, cameras, bg, sh_degree)
         if len(targets) != len(self.cameras):
             raise ValueError("step_l1: one target per camera")
@@ -113,20 +114,14 @@ class BatchSplatter:
         for f, t in zip(self._fbs, targets):
             if f.shape != t.shape:
                 raise ValueError("step_l1: each target has its view's image shape")
-        sizes = [f.numel() for f in self._fbs]
-        flat = torch.empty(sum(sizes) + 4 * len(sizes), dtype=torch.float32, device=dev)
-        grads, off = [], 0
-        for f, k in zip(self._fbs, sizes):
-            grads.append(flat[off:off + k].view(f.shape))
-            off += (k + 3) // 4 * 4  # 16-byte aligned views
-        items = losses.l1_items(self._fbs, targets, grads)
-        for v, g in enumerate(grads):
-            self._views[v].grad_framebuffer = g.data_ptr()
+        for v, (f, t) in enumerate(zip(self._fbs, targets)):
+            self._views[v].grad_framebuffer = None
+            self._views[v].l1_target = t.data_ptr()
+            self._views[v].l1_scale = 1.0 / max(f.numel(), 1)  # the scale losses.l1_items passes
         out = self._grad_buffers(1)
         self._launch_forward(dev)
-        N.check(N.load().dgmr_l1_loss_backward_many(items, len(items), C.stream_handle(dev)), "dgmr_l1_loss_backward_many")
         self._launch_backward(None, 1, out)
-        self._step_keep = (targets, flat)  # alive until the next step's launches are queued behind these
+        self._step_keep = targets  # alive until the next step's launches are queued behind these
         return self._fbs, tuple(t[0] for t in out)
 
     @property
@@ -139,17 +134,42 @@ class BatchSplatter:
         Otherwise groups[v] in {-1, 0..G-1} (a group's views consecutive, -1 =
         no gradient, its grad image may be None) and a list of G such tuples is
         returned, one per group."""
+        return self._backward(grad_images, None, groups)
+
+    def backward_l1(self, targets: List[Optional[torch.Tensor]], groups: Optional[Sequence[int]] = None):
+        """backward() with dL/dimage = sign(image - target) / numel of each
+        view formed inside the blend backward (dgmr_batch_view.l1_target):
+        the same gradients as backward(losses.l1_grad_many(images, targets)),
+        without writing or reading gradient images.  targets[v] None: view v
+        has a zero gradient."""
+        if len(targets) != len(self.cameras):
+            raise ValueError("backward_l1: one target (or None) per camera")
+        return self._backward([None] * len(targets), targets, groups)
+
+    def _backward(self, grad_images, targets, groups):
         G = 1 if groups is None else max(1, max(groups) + 1)
+        dev = self._fbs[0].device
         zero = None
-        grads = []
-        for v, g in enumerate(grad_images):
+        keep = []
+        for v, g in enumerate(grad_images):  # the forward's descriptors; only the gradient inputs are new
+            t = None if targets is None else targets[v]
+            if t is not None:
+                t = t.to(device=dev, dtype=torch.float32).contiguous()
+                if t.shape != self._fbs[v].shape:
+                    raise ValueError("backward_l1: each target has its view's image shape")
+                self._views[v].grad_framebuffer = None
+                self._views[v].l1_target = t.data_ptr()
+                self._views[v].l1_scale = 1.0 / max(t.numel(), 1)  # the scale losses.l1_items passes
+                keep.append(t)
+                continue
             if g is None:
                 if zero is None or zero.shape != self._fbs[v].shape:
                     zero = torch.zeros_like(self._fbs[v])
                 g = zero
-            grads.append(g.to(torch.float32).contiguous())
-        for v, g in enumerate(grads):  # the forward's descriptors; only the gradient images are new
+            g = g.to(torch.float32).contiguous()
+            keep.append(g)
             self._views[v].grad_framebuffer = g.data_ptr()
+            self._views[v].l1_target = None
         out = self._grad_buffers(G)
         self._launch_backward(groups, G, out)
         out = [(g_sh, g_op, g_means, g_scales, g_rot) for g_sh, g_op, g_means, g_scales, g_rot in zip(*[list(t) for t in out])]

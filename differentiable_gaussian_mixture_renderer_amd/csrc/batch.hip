@@ -25,6 +25,8 @@ struct ViewBlend {
     float* framebuffer;
     float* final_T;
     const float* grad_fb;
+    const float* l1_target; // non-null: dL/dimage = sign(image - target) * l1_scale instead of grad_fb
+    float l1_scale;
 };
 
 // thread per Gaussian: its parameters are read once and projected into every view
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t
         const int px0 = (local % vb.grid_x) * TW, py0 = (local / vb.grid_x) * TH;
         if constexpr (BACKWARD)
             blend_tile_backward<TW, TH, B>(sm, uniform(ranges[2 * tile]), uniform(ranges[2 * tile + 1]), px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
-                vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T, vb.grad_fb, screen_grads);
+                vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T, vb.grad_fb, screen_grads, vb.l1_target, vb.l1_scale);
         else
             blend_tile_forward<TW, TH, B>(sm, uniform(ranges[2 * tile]), uniform(ranges[2 * tile + 1]), px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
                 vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T);
@@ -348,7 +350,7 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
         cams[v] = make_dev_camera<float>(views[v].cam, *tiling);
         vb[v] = ViewBlend { views[v].cam.width, views[v].cam.height, cams[v].grid_x, int(base),
             { float(views[v].cam.background[0]), float(views[v].cam.background[1]), float(views[v].cam.background[2]) }, views[v].framebuffer,
-            views[v].final_transmittance, views[v].grad_framebuffer };
+            views[v].final_transmittance, views[v].grad_framebuffer, views[v].l1_target, views[v].l1_scale };
         grids[v] = make_int4(cams[v].grid_x, cams[v].grid_y, int(base), 0);
         base += int64_t(cams[v].grid_x) * cams[v].grid_y;
     }
@@ -435,7 +437,8 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
         n_groups = 1;
     }
     for (int v = 0; v < n_views; ++v)
-        DGMR_ARG_CHECK(grp[v] < 0 || views[v].grad_framebuffer, "dgmr_splat_batch_backward: a view with a gradient group needs grad_framebuffer");
+        DGMR_ARG_CHECK(grp[v] < 0 || views[v].grad_framebuffer || views[v].l1_target,
+            "dgmr_splat_batch_backward: a view with a gradient group needs grad_framebuffer or l1_target");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int64_t n = g->n;
     size_t need = 0;
@@ -450,7 +453,7 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
         const DevCamera<float> c = make_dev_camera<float>(views[v].cam, *tiling);
         vb[v] = ViewBlend { views[v].cam.width, views[v].cam.height, c.grid_x, int(base),
             { float(views[v].cam.background[0]), float(views[v].cam.background[1]), float(views[v].cam.background[2]) }, views[v].framebuffer,
-            views[v].final_transmittance, views[v].grad_framebuffer };
+            views[v].final_transmittance, views[v].grad_framebuffer, views[v].l1_target, views[v].l1_scale };
         base += int64_t(c.grid_x) * c.grid_y;
     }
     DGMR_HIP_CHECK(hipMemcpyAsync(L.views, vb.data(), sizeof(ViewBlend) * n_views, hipMemcpyHostToDevice, st));

@@ -15,6 +15,9 @@
 
 namespace dgmr {
 
+// torch.sign semantics for the L1 gradient: 0 for equal values, NaN propagates
+__device__ __forceinline__ float l1_sign(float d) { return d > 0.f ? 1.f : (d < 0.f ? -1.f : d); }
+
 constexpr double kPi = 3.14159265358979323846;
 
 // real SH constants (constants.h:26-43)

@@ -6,6 +6,7 @@
 // leaves one partial sum per workgroup (summed in fixed order by the caller,
 // so the loss is deterministic); the backward reads them once more and
 // writes the gradient scaled by dL/dloss, which it reads from device memory.
+#include "device_math.h"
 #include "host_common.h"
 
 namespace dgmr {
@@ -44,8 +45,6 @@ __global__ __launch_bounds__(kLossThreads) void k_l1_forward(const float* __rest
     }
 }
 
-// torch.sign semantics: 0 for equal values, NaN propagates
-__device__ __forceinline__ float sign_of(float d) { return d > 0.f ? 1.f : (d < 0.f ? -1.f : d); }
 
 __global__ __launch_bounds__(kLossThreads) void k_l1_backward(const float* __restrict__ a, const float* __restrict__ b, int64_t n, float scale,
     const float* __restrict__ grad_loss, float* __restrict__ grad)
@@ -58,10 +57,10 @@ __global__ __launch_bounds__(kLossThreads) void k_l1_backward(const float* __res
     float4* g4 = reinterpret_cast<float4*>(grad);
     for (int64_t i = int64_t(blockIdx.x) * kLossThreads + threadIdx.x; i < n4; i += stride) {
         const float4 x = a4[i], y = b4[i];
-        g4[i] = make_float4(sign_of(x.x - y.x) * s, sign_of(x.y - y.y) * s, sign_of(x.z - y.z) * s, sign_of(x.w - y.w) * s);
+        g4[i] = make_float4(l1_sign(x.x - y.x) * s, l1_sign(x.y - y.y) * s, l1_sign(x.z - y.z) * s, l1_sign(x.w - y.w) * s);
     }
     for (int64_t i = 4 * n4 + int64_t(blockIdx.x) * kLossThreads + threadIdx.x; i < n; i += stride)
-        grad[i] = sign_of(a[i] - b[i]) * s;
+        grad[i] = l1_sign(a[i] - b[i]) * s;
 }
 
 // Many images in one launch (the views of a batch): each workgroup takes
@@ -105,10 +104,10 @@ __global__ __launch_bounds__(kLossThreads) void k_l1_backward_many(const L1Many 
         if (i + 4 <= n) {
             const float4 x = *reinterpret_cast<const float4*>(a + i), y = *reinterpret_cast<const float4*>(b + i);
             *reinterpret_cast<float4*>(g + i) =
-                make_float4(sign_of(x.x - y.x) * s, sign_of(x.y - y.y) * s, sign_of(x.z - y.z) * s, sign_of(x.w - y.w) * s);
+                make_float4(l1_sign(x.x - y.x) * s, l1_sign(x.y - y.y) * s, l1_sign(x.z - y.z) * s, l1_sign(x.w - y.w) * s);
         } else {
             for (int64_t j = i; j < n; ++j)
-                g[j] = sign_of(a[j] - b[j]) * s;
+                g[j] = l1_sign(a[j] - b[j]) * s;
         }
     }
 }

@@ -561,7 +561,8 @@ template <int TW, int TH, int B>
 __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_t first, uint32_t last, int px0, int py0, int width,
     int height, float bg0, float bg1, float bg2, const uint32_t* __restrict__ point_list, const float2* __restrict__ xy,
     const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, const float* __restrict__ framebuffer,
-    const float* __restrict__ final_T, const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
+    const float* __restrict__ final_T, const float* __restrict__ grad_fb, float* __restrict__ screen_grads,
+    const float* __restrict__ l1_target = nullptr, float l1_scale = 0.f)
 {
     constexpr int NP = TW * TH / 64;
     constexpr int RS = 64 / TW;
@@ -587,10 +588,18 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
         const bool inside = px < width && py < height;
         if (inside) {
             const int64_t p = int64_t(py) * width + px;
-            gC0[k] = grad_fb[p];
-            gC1[k] = grad_fb[plane + p];
-            gC2[k] = grad_fb[2 * plane + p];
-            D[k] = gC0[k] * framebuffer[p] + gC1[k] * framebuffer[plane + p] + gC2[k] * framebuffer[2 * plane + p];
+            const float f0 = framebuffer[p], f1 = framebuffer[plane + p], f2 = framebuffer[2 * plane + p];
+            if (l1_target) {
+                // fused L1 loss: dL/dC = sign(C - target) * scale, as the L1 kernel writes it
+                gC0[k] = l1_sign(f0 - l1_target[p]) * l1_scale;
+                gC1[k] = l1_sign(f1 - l1_target[plane + p]) * l1_scale;
+                gC2[k] = l1_sign(f2 - l1_target[2 * plane + p]) * l1_scale;
+            } else {
+                gC0[k] = grad_fb[p];
+                gC1[k] = grad_fb[plane + p];
+                gC2[k] = grad_fb[2 * plane + p];
+            }
+            D[k] = gC0[k] * f0 + gC1[k] * f1 + gC2[k] * f2;
         }
         T[k] = 1.f;
         live[k] = inside && !(gC0[k] == 0.f && gC1[k] == 0.f && gC2[k] == 0.f);

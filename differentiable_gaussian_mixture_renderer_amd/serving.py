@@ -224,10 +224,10 @@ class ContinuousBatchingServer:
         imgs, _ = splatter.forward(*sc.params, [r.camera for r in batch], sh_degree=sc.sh_degree)
         grads = [None] * len(batch)
         if n_bwd:
-            gimg = losses.l1_grad_many(imgs[:n_bwd], [l1_target(img, r.target_seed) for img, r in zip(imgs, batch[:n_bwd])])
-            gimg += [None] * (len(batch) - n_bwd)
+            # the L1 gradient is formed inside the blend backward
+            targets = [l1_target(img, r.target_seed) for img, r in zip(imgs, batch[:n_bwd])] + [None] * (len(batch) - n_bwd)
             groups = [k if k < n_bwd else -1 for k in range(len(batch))]
-            grads[:n_bwd] = splatter.backward(gimg, groups=groups)
+            grads[:n_bwd] = splatter.backward_l1(targets, groups=groups)
         return imgs, grads
 
     # -------------------------------------------------------------- scheduler

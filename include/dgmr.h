@@ -187,6 +187,13 @@ typedef struct dgmr_batch_view {
     float* framebuffer;           /* [3,H,W] output of the forward */
     float* final_transmittance;   /* [H,W]   output of the forward */
     const float* grad_framebuffer;/* [3,H,W] input of the backward */
+    /* Fused L1 loss (no counterpart in the reference): l1_target non-NULL
+     * makes the backward use dL/dimage = sign(framebuffer - l1_target) *
+     * l1_scale at every pixel, bit for bit what dgmr_l1_loss_backward writes
+     * with scale l1_scale and dL/dloss = 1, and ignore grad_framebuffer. */
+    const float* l1_target;       /* [3,H,W] or NULL */
+    float l1_scale;               /* 1 / (3 H W) for the mean */
+    int32_t reserved;
 } dgmr_batch_view;
 
 int dgmr_batch_workspace_size(int64_t n, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,

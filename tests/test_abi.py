@@ -135,3 +135,33 @@ def test_l1_many_argument_checks_without_gpu():
     items[0].n = 8
     assert lib.dgmr_l1_loss_backward_many(items, 1, None) == N.DGMR_ERR_ARGUMENT
     assert lib.dgmr_l1_loss_backward_many(items, 0, None) == N.DGMR_OK  # nothing to do
+
+
+def test_ctypes_descriptors_match_the_header(tmp_path):
+    """Sizes and field offsets of the ctypes mirrors equal what a C compiler
+    lays out from include/dgmr.h (gcc on a probe that prints offsetof)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    from differentiable_gaussian_mixture_renderer_amd import _native as N
+    mirrors = {"dgmr_gaussians": N.Gaussians, "dgmr_camera": N.Camera, "dgmr_tiling": N.Tiling, "dgmr_stats": N.Stats,
+               "dgmr_splat_geometry": N.SplatGeometry, "dgmr_binning": N.Binning, "dgmr_march_geometry": N.MarchGeometry,
+               "dgmr_march_options": N.MarchOptions, "dgmr_batch_view": N.BatchView, "dgmr_adam_tensor": N.AdamTensor,
+               "dgmr_stage_time": N.StageTime, "dgmr_raster_options": N.RasterOptions, "dgmr_l1_item": N.L1Item}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dgmr.h"', 'int main(void) {']
+    for cname, cls in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['return 0;', '}']
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict((" ".join(l.split()[:2]), int(l.split()[2])) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                                                      text=True).stdout.splitlines())
+    for cname, cls in mirrors.items():
+        assert got[f"{cname} size"] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[f"{cname} {fname}"] == getattr(cls, fname).offset, (cname, fname)
