@@ -240,3 +240,36 @@ def test_fused_activations_match_torch_activations():
         nb = float(b.norm())
         assert nb > 0, name
         assert float((a - b).norm()) <= 1e-3 * nb, (name, float((a - b).norm()), nb)
+
+
+def test_compacted_depth_sort_single_view_equals_uncompacted_subset():
+    """4.3M Gaussians (at least 4M: the binning compacts the visible entries
+    and numbers them itself, the preprocess writes no ids) around a camera at
+    the origin looking along +z, so every Gaussian with z <= 0 is culled.  The
+    same render of only the Gaussians with z > 0 (2.15M, no compaction, ids
+    written by the preprocess) gives the same image and transmittance bit for
+    bit, the same tile ranges, and the same point list up to the subset's
+    numbering: dropping culled Gaussians changes nothing else."""
+    torch = _torch()
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    from differentiable_gaussian_mixture_renderer_amd.python_bindings import splatter
+    scene = scenes.make_scene(4_300_000, seed=31, extent=5.0, sh_degree=0, sh_mode="rgb")
+    cam = scenes.look_at_camera((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 320, 240, 0.6, 0.45)
+    keep = np.nonzero(scene.means[:, 2] > 0.0)[0]
+    assert keep.size < (4 << 20) <= scene.n
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    cam_args = [t(cam.world_view_transform), t(cam.full_proj_transform), t(cam.camera_center), t(np.zeros(3, np.float32)), cam.width,
+                cam.height, cam.tan_fovx, cam.tan_fovy, 0]
+    outs = []
+    for sel in (slice(None), keep):
+        args = [t(scene.sh[sel]), t(scene.opacity[sel]), t(scene.means[sel]), t(scene.scales[sel]), t(scene.rotations[sel])]
+        print(f"[gpu splat] compaction check n={args[2].shape[0]}", flush=True)
+        cache = splatter.cpp_binding.forward(*args, *cam_args, cull=False)
+        outs.append([c.cpu() for c in cache])
+    torch.cuda.synchronize()
+    full, sub = outs
+    assert torch.equal(full[0], sub[0]) and torch.equal(full[11], sub[11])  # framebuffer, transmittance
+    assert int(full[12][0]) == int(sub[12][0]) > 0                          # pair count
+    assert torch.equal(full[9], sub[9])                                      # tile ranges
+    np.testing.assert_array_equal(full[10].numpy(), keep[sub[10].numpy()].astype(np.int32))
