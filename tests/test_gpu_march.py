@@ -240,3 +240,35 @@ def test_chunk_size_and_tiles_do_not_change_results():
         vol_marcher.DEFAULT_OPTIONS.update(saved)
     np.testing.assert_array_equal(out[16][0], out[32][0])
     np.testing.assert_array_equal(out[16][1], out[32][1])
+
+
+def test_compacted_binning_equals_uncompacted_subset():
+    """The marcher above the compaction threshold (4.3M Gaussians; the
+    preprocess leaves the ids to the compaction) against the same render of
+    only the Gaussians with z > 0 (no compaction), camera at the origin
+    looking along +z so that every dropped Gaussian is culled: the same image
+    and remaining transmittance bit for bit, the same tile ranges, and the
+    same point list up to the subset's numbering."""
+    torch = _torch()
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    from differentiable_gaussian_mixture_renderer_amd.python_bindings import vol_marcher as vm
+    scene = scenes.make_scene(4_300_000, seed=37, extent=5.0, sh_degree=0, sh_mode="rgb")
+    cam = scenes.look_at_camera((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 96, 64, 0.2, 0.15)  # ~40k Gaussians in view
+    keep = np.nonzero(scene.means[:, 2] > 0.0)[0]
+    assert keep.size < (4 << 20) <= scene.n
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    cam_args = [t(cam.world_view_transform), t(cam.full_proj_transform), t(cam.camera_center), t(np.zeros(3, np.float32)), cam.width,
+                cam.height, cam.tan_fovx, cam.tan_fovy, 0]
+    outs = []
+    for sel in (slice(None), keep):
+        args = [t(scene.sh[sel]), t(scene.opacity[sel]), t(scene.means[sel]), t(scene.scales[sel]), t(scene.rotations[sel])]
+        print(f"[gpu march] compaction check n={args[2].shape[0]}", flush=True)
+        cache = vm.cpp_binding.forward(*args, *cam_args)
+        outs.append([c.cpu() for c in cache])
+    torch.cuda.synchronize()
+    full, sub = outs
+    assert torch.equal(full[0], sub[0]) and torch.equal(full[12], sub[12])  # framebuffer, remaining transmittance
+    assert torch.equal(full[10], sub[10])                                    # tile ranges
+    assert full[11].numel() == sub[11].numel() > 0
+    np.testing.assert_array_equal(full[11].numpy(), keep[sub[11].numpy()].astype(np.int32))
