@@ -544,7 +544,12 @@ struct BlendSmemBwd {
     float4 geo[B];   // x, y, A, B
     float4 col[B];   // C, opacity, r, g
     float blue[B];
-    float acc[9][B]; // per-batch gradient sums of the tile: xy 2, conic 3, opacity 1, rgb 3
+    // per-batch gradient sums of the tile, Gaussian-major: acc[9 i + q] is
+    // value q (xy 2, conic 3, opacity 1, rgb 3) of batch Gaussian i, so the
+    // nine lanes storing one Gaussian's sums and the flush's lanes both touch
+    // consecutive words (value-major rows 64 words apart put a Gaussian's
+    // nine values in one bank)
+    float acc[9 * B];
     uint32_t gid[B]; // the batch's Gaussian ids, for the flush
 };
 
@@ -623,8 +628,8 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
     const auto flush = [&]() {
         const int total = cnt * 9;
         for (int f = lane; f < total; f += 64) {
-            const int i = f / 9, q = f - 9 * (f / 9);
-            const float v = sm.acc[q][i];
+            const int i = f / 9, q = f - 9 * i;
+            const float v = sm.acc[f];
             if (v != 0.f)
                 unsafeAtomicAdd(screen_grads + int64_t(sm.gid[i]) * 9 + q, v);
         }
@@ -646,7 +651,7 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             f.stage(sm.geo, sm.col, sm.blue, s);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
-                sm.acc[q][i] = 0.f;
+                sm.acc[q * B + i] = 0.f; // 9 B consecutive words over the lanes and steps
         }
         wave_lds_sync();
         // unconditional (clamped) so the loaded registers need no merge after a branch
@@ -714,7 +719,7 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                 const float t8 = wave_sum8(gx, gy, ga, gb, gc, so, sr, sg);
                 const float t9 = wave_sum_to_last(sbl);
                 if (slot8 >= 0)
-                    sm.acc[slot8][j] = slot8 == 8 ? t9 : t8;
+                    sm.acc[9 * j + slot8] = slot8 == 8 ? t9 : t8;
             }
             if (__ballot(still) == 0)
                 break;
