@@ -141,7 +141,7 @@ class BatchSplatter:
         view formed inside the blend backward (dgmr_batch_view.l1_target):
         the same gradients as backward(losses.l1_grad_many(images, targets)),
         without writing or reading gradient images.  targets[v] None: view v
-        has a zero gradient."""
+        has a zero gradient (its tiles are skipped)."""
         if len(targets) != len(self.cameras):
             raise ValueError("backward_l1: one target (or None) per camera")
         return self._backward([None] * len(targets), targets, groups)
@@ -149,7 +149,6 @@ class BatchSplatter:
     def _backward(self, grad_images, targets, groups):
         G = 1 if groups is None else max(1, max(groups) + 1)
         dev = self._fbs[0].device
-        zero = None
         keep = []
         for v, g in enumerate(grad_images):  # the forward's descriptors; only the gradient inputs are new
             t = None if targets is None else targets[v]
@@ -162,14 +161,13 @@ class BatchSplatter:
                 self._views[v].l1_scale = 1.0 / max(t.numel(), 1)  # the scale losses.l1_items passes
                 keep.append(t)
                 continue
-            if g is None:
-                if zero is None or zero.shape != self._fbs[v].shape:
-                    zero = torch.zeros_like(self._fbs[v])
-                g = zero
+            self._views[v].l1_target = None
+            if g is None:  # no gradient image: the view adds nothing (dgmr_splat_batch_backward skips its tiles)
+                self._views[v].grad_framebuffer = None
+                continue
             g = g.to(torch.float32).contiguous()
             keep.append(g)
             self._views[v].grad_framebuffer = g.data_ptr()
-            self._views[v].l1_target = None
         out = self._grad_buffers(G)
         self._launch_backward(groups, G, out)
         out = [(g_sh, g_op, g_means, g_scales, g_rot) for g_sh, g_op, g_means, g_scales, g_rot in zip(*[list(t) for t in out])]

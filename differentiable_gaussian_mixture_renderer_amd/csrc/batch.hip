@@ -82,12 +82,19 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t
         const ViewBlend& vb = views[find_view(views, n_views, tile)];
         const int local = tile - vb.tile_base;
         const int px0 = (local % vb.grid_x) * TW, py0 = (local / vb.grid_x) * TH;
-        if constexpr (BACKWARD)
-            blend_tile_backward<TW, TH, B>(sm, uniform(ranges[2 * tile]), uniform(ranges[2 * tile + 1]), px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
-                vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T, vb.grad_fb, screen_grads, vb.l1_target, vb.l1_scale);
-        else
+        if constexpr (BACKWARD) {
+            // a view with neither a gradient image nor an L1 target has a zero
+            // dL/dimage (or no gradient group): its tiles add nothing, so they
+            // are skipped (the view record is the same for the whole wave)
+            if (!vb.grad_fb && !vb.l1_target)
+                continue;
+            blend_tile_backward<TW, TH, B>(sm, uniform(ranges[2 * tile]), uniform(ranges[2 * tile + 1]), px0, py0, vb.width, vb.height, vb.bg[0],
+                vb.bg[1], vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T, vb.grad_fb, screen_grads, vb.l1_target,
+                vb.l1_scale);
+        } else {
             blend_tile_forward<TW, TH, B>(sm, uniform(ranges[2 * tile]), uniform(ranges[2 * tile + 1]), px0, py0, vb.width, vb.height, vb.bg[0], vb.bg[1],
                 vb.bg[2], point_list, xy, conic_opacity, rgb, vb.framebuffer, vb.final_T);
+        }
         wave_lds_sync();
     }
 }
@@ -436,9 +443,6 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
     } else {
         n_groups = 1;
     }
-    for (int v = 0; v < n_views; ++v)
-        DGMR_ARG_CHECK(grp[v] < 0 || views[v].grad_framebuffer || views[v].l1_target,
-            "dgmr_splat_batch_backward: a view with a gradient group needs grad_framebuffer or l1_target");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int64_t n = g->n;
     size_t need = 0;

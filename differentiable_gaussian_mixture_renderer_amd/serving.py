@@ -168,6 +168,7 @@ class ContinuousBatchingServer:
         self._in_flight = 0
         self._work: deque = deque()
         self._stop = False
+        self._error: Optional[BaseException] = None  # the first exception a worker raised
         self._t0 = 0.0
 
     # ------------------------------------------------------------------ clock
@@ -176,6 +177,22 @@ class ContinuousBatchingServer:
 
     # ---------------------------------------------------------------- workers
     def _worker(self, k: int):
+        """Runs batches until stopped; the first exception (a DgmrError, an
+        out-of-memory error, a failure to set up the stream) is recorded, the
+        queued work is dropped and run() re-raises it."""
+        try:
+            self._worker_loop(k)
+        except BaseException as exc:  # noqa: BLE001 -- handed to run(), which re-raises it
+            with self._lock:
+                if self._error is None:
+                    self._error = exc
+                self._stop = True
+                for _, _, cost in self._work:
+                    self._in_flight -= cost
+                self._work.clear()
+                self._lock.notify_all()
+
+    def _worker_loop(self, k: int):
         stream = _worker_stream(self.device, k)
         splatter = BatchSplatter(tile=self.tile, batch=self.lds_batch, device=self.device)
         while True:
@@ -188,14 +205,19 @@ class ContinuousBatchingServer:
             # gradient requests first, one gradient group each; forward-only views after them
             batch = [r for r in batch if r.backward] + [r for r in batch if not r.backward]
             n_bwd = sum(r.backward for r in batch)
-            with torch.cuda.stream(stream):
-                sc = self.resident[scene_id]
-                if len(batch) == 1:
-                    # a lone request (low load): the one-camera path has less per-launch setup
-                    imgs, grads = self._single(sc, batch[0])
-                else:
-                    imgs, grads = self._batched(splatter, sc, batch, n_bwd)
-                stream.synchronize()
+            try:
+                with torch.cuda.stream(stream):
+                    sc = self.resident[scene_id]
+                    if len(batch) == 1:
+                        # a lone request (low load): the one-camera path has less per-launch setup
+                        imgs, grads = self._single(sc, batch[0])
+                    else:
+                        imgs, grads = self._batched(splatter, sc, batch, n_bwd)
+                    stream.synchronize()
+            except BaseException:
+                with self._lock:
+                    self._in_flight -= cost  # the failed batch's admission is released
+                raise
             t = self.now()
             with self._lock:
                 for r, img, g in zip(batch, imgs, grads):
@@ -265,6 +287,8 @@ class ContinuousBatchingServer:
         total = len(pending)
         while True:
             with self._lock:
+                if self._error is not None:
+                    break
                 t = self.now()
                 while pending and pending[0].arrival <= t:
                     self._waiting.append(pending.popleft())
@@ -281,6 +305,9 @@ class ContinuousBatchingServer:
         for w in workers:
             w.join()
         self.stats.wall = self.now()
+        if self._error is not None:
+            # a failed batch (e.g. a DgmrError or an out-of-memory error) ends the run instead of leaving it waiting
+            raise RuntimeError(f"serving worker failed: {self._error!r}") from self._error
         return self.stats
 
 

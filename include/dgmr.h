@@ -207,7 +207,9 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
  * entries) gives each view a group in [0, n_groups) or -1 (no gradient); a
  * group's views are consecutive, and grad_* hold n_groups consecutive blocks
  * ([n_groups][n,3] means, ...), each the sum over its group's views (zero for
- * a group without views).  Outputs are overwritten. */
+ * a group without views).  A view whose grad_framebuffer and l1_target are
+ * both NULL contributes nothing, as a zero dL/dimage would (group -1 views
+ * need neither).  Outputs are overwritten. */
 int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* views, int32_t n_views, const dgmr_tiling* tiling,
     void* workspace, size_t workspace_bytes, int64_t pair_capacity, int64_t n_pairs, const int32_t* view_group, int32_t n_groups,
     float* grad_means, float* grad_scales, float* grad_rotations, float* grad_opacity, float* grad_sh, dgmr_stream stream);

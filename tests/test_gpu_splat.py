@@ -273,3 +273,63 @@ def test_compacted_depth_sort_single_view_equals_uncompacted_subset():
     assert int(full[12][0]) == int(sub[12][0]) > 0                          # pair count
     assert torch.equal(full[9], sub[9])                                      # tile ranges
     np.testing.assert_array_equal(full[10].numpy(), keep[sub[10].numpy()].astype(np.int32))
+
+
+def test_backward_against_finite_differences_of_the_fp64_forward():
+    """The HIP splat backward (float32) against central differences of the
+    FP64 render (float binning, as the GPU bins), 32 random parameters over
+    the five tensors (the reference's pattern, grad_inria_splatter_render.cu:160-228).
+    dL/dimg is zeroed on pixels the oracle flags as float-ambiguous; h = 1e-6
+    max(1, |theta|) (1e-3 for SH coefficients, which enter linearly) moves no
+    discrete decision (the two step sizes agree).  Bar per parameter:
+    |gpu - fd| <= 1e-3 |fd| + 1e-4 s + e (s the largest |fd| drawn for that
+    tensor, covering float accumulation on small gradients, as the max-relative
+    bar of test_backward_parity; e the quotient's round-off)."""
+    torch = _torch()
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    scene = scenes.make_scene(2_000, seed=47, extent=1.0, sh_degree=3)
+    scene.scales = (scene.scales * 2.0).astype(np.float32)
+    cam = scenes.look_at_camera([0.3, -0.2, -4.0], [0.0, 0.1, 0.0], 96, 64, 0.6, 0.4)
+    bg = np.array([0.2, 0.1, 0.3], np.float32)
+    base = [scene.means.astype(np.float64), scene.scales.astype(np.float64), scene.rotations.astype(np.float64),
+            scene.opacity.reshape(-1).astype(np.float64), O.scene_arrays(scene)[4].astype(np.float64)]
+    ref = O.SplatOracle((*base, 3), cam, dbl=True, float_binning=True)
+    fb64, _, amb, _ = ref.render(bg)
+    rng = np.random.default_rng(23)
+    grad = rng.normal(size=fb64.shape)
+    grad[:, amb != 0] = 0.0
+    out = _run_gpu(scene, cam, bg, grad_fb=grad)
+
+    def loss(params):
+        fb, _, _, _ = O.SplatOracle((*params, 3), cam, dbl=True, float_binning=True).render(bg)
+        return float((fb * grad).sum())
+
+    names = ("means", "scales", "rotations", "opacity", "sh")
+    gpu = {k: out["grads"][k].reshape(scene.n, -1) for k in names}
+    rows = []
+    for r in range(32):
+        t = r % 5
+        name = names[t]
+        live = np.nonzero(np.abs(gpu[name]).max(axis=1) > 0)[0]
+        i = int(live[rng.integers(live.size)])
+        j = int(rng.integers(gpu[name].shape[1]))
+        v = float(base[t].reshape(scene.n, -1)[i, j])
+        fds = []
+        for step in ((1e-3, 5e-4) if name == "sh" else (1e-6, 5e-7)):
+            hh = step * max(1.0, abs(v))
+            vals = []
+            for sgn in (1, -1):
+                p2 = [p.copy() for p in base]
+                p2[t].reshape(scene.n, -1)[i, j] = v + sgn * hh
+                vals.append(loss(p2))
+            fds.append((vals[0] - vals[1]) / (2 * hh))
+        e = 8 * np.sqrt(fb64.size) * np.finfo(np.float64).eps * np.abs(grad * fb64).max() / hh
+        rows.append((name, i, j, fds[0], fds[1], float(gpu[name][i, j]), e))
+    for name in names:
+        sel = [r for r in rows if r[0] == name]
+        s = max(abs(r[3]) for r in sel)
+        assert s > 0, name
+        for _, i, j, fd, fd2, g, e in sel:
+            print(f"{name}[{i},{j}]: fd {fd:.7g} (half step {fd2:.7g}), gpu {g:.7g}, rel {abs(g - fd) / max(abs(fd), 1e-30):.2g}")
+            assert abs(fd - fd2) <= 1e-5 * abs(fd) + e, (name, i, j, "decision moved")
+            assert abs(g - fd) <= 1e-3 * abs(fd) + 1e-4 * s + e, (name, i, j, fd, g)

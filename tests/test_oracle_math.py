@@ -168,6 +168,74 @@ def test_sh_basis_is_orthonormal_on_the_sphere():
     np.testing.assert_allclose(gram, np.eye(16), atol=2e-3)
 
 
+def _real_sh_scipy(l, m, d):
+    """Real spherical harmonic (l, m) at unit direction d, built from scipy's
+    complex Y_l^m (which carries the Condon-Shortley phase): sqrt(2) Re Y_l^m
+    for m > 0, sqrt(2) Im Y_l^|m| for m < 0, Y_l^0 for m = 0."""
+    from scipy.special import sph_harm_y
+    theta = math.acos(max(-1.0, min(1.0, d[2])))
+    phi = math.atan2(d[1], d[0])
+    y = complex(sph_harm_y(l, abs(m), theta, phi))
+    if m > 0:
+        return math.sqrt(2.0) * y.real
+    if m < 0:
+        return math.sqrt(2.0) * y.imag
+    return y.real
+
+
+def test_sh_basis_equals_scipy_real_harmonics():
+    """Each of the 16 basis functions of sh_to_colour (constants.h:26-43,
+    math.h:233-271) equals, value and sign, the real harmonic (l, m) with
+    coefficient k = l^2 + l + m, from scipy's complex harmonics; a flipped
+    sign, a swapped pair or a wrong constant fails at the first direction."""
+    for _ in range(40):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        for l in range(4):
+            for m in range(-l, l + 1):
+                k = l * l + l + m
+                sh = np.zeros((16, 3))
+                sh[k] = [0.1, -0.1, 0.05]
+                rgb, ok = sh_colour(sh, 3, d)
+                want = _real_sh_scipy(l, m, d)
+                np.testing.assert_allclose((rgb - 0.5) / np.array([0.1, -0.1, 0.05]), want, rtol=0, atol=1e-12, err_msg=f"l={l} m={m}")
+                assert ok.all()
+    # the active degree cuts the series: coefficients above it contribute nothing
+    d = np.array([0.48, -0.6, 0.64])
+    sh = rng.normal(size=(16, 3)) * 0.05
+    for deg in range(4):
+        rgb, _ = sh_colour(sh, deg, d)
+        kk = (deg + 1) ** 2
+        want = sum(sh[k] * _real_sh_scipy(int(math.isqrt(k)), k - int(math.isqrt(k)) ** 2 - int(math.isqrt(k)), d) for k in range(kk))
+        np.testing.assert_allclose(rgb, want + 0.5, atol=1e-12)
+
+
+@pytest.mark.parametrize("scales,expect", [
+    ((0.30, 0.20, 0.05), 1.0),             # smallest scale along the view axis: the weight is the opacity
+    ((0.30, 0.05, 0.20), 0.30 * 0.20 / (0.30 * 0.05)),  # smallest across it: the screen footprint shrinks, the weight grows
+    ((0.05, 0.25, 0.10), 0.25 * 0.10 / (0.05 * 0.25)),
+])
+@pytest.mark.parametrize("spin", [0.0, 0.7])
+def test_ots_weight_closed_form_on_the_optical_axis(scales, expect, spin):
+    """Ots weight = w det(SJ) I2(larger2(s)) / I2(cov2d) (math.h:389-397).
+    For an axis-aligned Gaussian at the centre of a camera looking down the
+    world z axis, with filter 0 (optionally spun about the view axis, which
+    leaves the footprint's area unchanged), det(SJ) = f_x f_y / z^2 and
+    I2(cov2d) = 2 pi (f_x s_x / z)(f_y s_y / z), so the weight is
+    w s_a s_b / (s_x s_y) with (s_a, s_b) the two largest scales and (s_x, s_y)
+    the in-image ones: w itself when the smallest scale faces the camera."""
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    c = scenes.look_at_camera([0.0, 0.0, -4.0], [0.0, 0.0, 0.0], 128, 96, 0.5, 0.375)
+    cam = (np.ascontiguousarray(c.world_view_transform, np.float64), np.ascontiguousarray(c.full_proj_transform, np.float64),
+           np.ascontiguousarray(c.camera_center, np.float64), 0.5, 0.375, 128, 96)
+    q = np.array([math.cos(spin / 2), 0.0, 0.0, math.sin(spin / 2)])
+    w = 0.37
+    g = splat2d(3, w, [0.0, 0.0, 0.0], scales, q, cam, 0.0)
+    np.testing.assert_allclose(g[0], w * expect, rtol=1e-9)
+    # and with the 0.3 px^2 filter the footprint grows, so the weight drops
+    assert splat2d(3, w, [0.0, 0.0, 0.0], scales, q, cam, 0.3)[0] < g[0]
+
+
 @pytest.mark.parametrize("deg", [0, 1, 2, 3])
 def test_sh_grad(deg):
     """grad_math.cu:392-431, 795-813"""

@@ -90,3 +90,102 @@ def test_blend_equals_cumulative_product_compositing():
         np.testing.assert_allclose(fb[:, py, px], C + T_end * bg, rtol=0, atol=1e-12)
         assert abs(T[py, px] - T_end) <= 1e-12
     assert saturated > 0
+
+
+def _fd_scene(n=300, seed=41):
+    """A small SH-3 scene seen at 64x48 with Gaussians large enough to overlap
+    (the splat backward's blend adjoint then carries real occlusion terms)."""
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    scene = scenes.make_scene(n, seed=seed, extent=1.0, sh_degree=3)
+    scene.scales = scene.scales * 2.5
+    cam = scenes.look_at_camera([0.3, -0.2, -4.0], [0.0, 0.1, 0.0], 64, 48, 0.6, 0.45)
+    arrays = [np.asarray(a, np.float64) for a in (scene.means, scene.scales, scene.rotations, scene.opacity.reshape(-1))]
+    sh16 = np.zeros((n, 16, 3))
+    sh16[:, :scene.sh.shape[1]] = scene.sh
+    return arrays + [sh16], scene.sh_degree, cam
+
+
+def _fd_loss(params, deg, cam, bg, grad_fb):
+    """L = <dL/dimg, image> of the FP64 splatter (double decisions throughout)."""
+    h = O.SplatOracle((*params, deg), cam, dbl=True, float_binning=False)
+    fb, _, _, _ = h.render(bg)
+    return float((fb * grad_fb).sum())
+
+
+def test_backward_matches_finite_differences():
+    """The composed splat backward (blend adjoint, conic -> covariance inverse
+    adjoint, grad::splat<Ots>, SH and direction adjoints) against central
+    differences of the FP64 render, for all five parameter tensors, 8 random
+    parameters each (the reference's pattern, grad_inria_splatter_render.cu:160-228,
+    extended from the per-pixel blend to the whole frame).  The blend's discrete
+    decisions (alpha >= 1/255, the 0.99 clamp, the stop before T < 1e-4, tile
+    rectangles) are piecewise constant; h = 1e-6 max(1, |theta|) moves no
+    decision at these parameters (the two step sizes agree), so the render is
+    differentiable there.  SH coefficients enter the image linearly (up to the
+    clamp at 0), so they take h = 1e-3, where the quotient is exact.
+    Bar: |fd - analytic| <= 1e-5 |fd| + e, with e = 8 sqrt(N) eps max|g img| / h
+    the difference quotient's round-off (N image values, eps = 2^-52)."""
+    params, deg, cam = _fd_scene()
+    bg = np.array([0.2, 0.1, 0.3])
+    rng = np.random.default_rng(17)
+    grad_fb = rng.normal(size=(3, cam.height, cam.width))
+    h = O.SplatOracle((*params, deg), cam, dbl=True, float_binning=False)
+    fb, T, _, _ = h.render(bg)
+    an = h.backward(bg, fb, T, grad_fb)
+    names = ("means", "scales", "rotations", "opacity", "sh")
+    checked = 0
+    for t, name in enumerate(names):
+        flat_an = an[name].reshape(params[t].shape[0], -1)
+        if name == "sh":
+            flat_an = flat_an[:, :3 * (deg + 1) ** 2]
+        live = np.nonzero(np.abs(flat_an).max(axis=1) > 0)[0]
+        assert live.size > 20, name
+        rows = []
+        for _ in range(8):
+            i = int(live[rng.integers(live.size)])
+            j = int(rng.integers(flat_an.shape[1]))
+            v = float(params[t].reshape(params[t].shape[0], -1)[i, j])
+            fds = []
+            for step in ((1e-3, 5e-4) if name == "sh" else (1e-6, 5e-7)):
+                hh = step * max(1.0, abs(v))
+                vals = []
+                for sgn in (1, -1):
+                    p2 = [p.copy() for p in params]
+                    p2[t].reshape(p2[t].shape[0], -1)[i, j] = v + sgn * hh
+                    vals.append(_fd_loss(p2, deg, cam, bg, grad_fb))
+                fds.append((vals[0] - vals[1]) / (2 * hh))
+            # round-off of the half-step quotient (the larger of the two)
+            e = 8 * np.sqrt(fb.size) * np.finfo(np.float64).eps * np.abs(grad_fb * fb).max() / hh
+            rows.append((i, j, fds[0], fds[1], float(flat_an[i, j]), e))
+        assert max(abs(r[2]) for r in rows) > 0, name
+        for i, j, fd, fd2, a, e in rows:
+            print(f"{name}[{i},{j}]: fd {fd:.9g} (half step {fd2:.9g}), analytic {a:.9g}, round-off {e:.2g}")
+            assert abs(fd - fd2) <= 1e-5 * abs(fd) + e, (name, i, j, "decision moved")
+            assert abs(fd - a) <= 1e-5 * abs(fd) + e, (name, i, j, fd, a)
+            checked += 1
+    assert checked == 40
+
+
+def test_backward_finite_difference_check_catches_a_planted_error():
+    """The FD bar above is tight enough to see a plausible slip: the analytic
+    opacity gradient with the 0.99-clamp branch ignored (gradient passed
+    through where alpha is clamped) or scaled by (1 - 1/255) fails it."""
+    params, deg, cam = _fd_scene(n=120, seed=43)
+    bg = np.zeros(3)
+    rng = np.random.default_rng(5)
+    grad_fb = rng.normal(size=(3, cam.height, cam.width))
+    h = O.SplatOracle((*params, deg), cam, dbl=True, float_binning=False)
+    fb, T, _, _ = h.render(bg)
+    an = h.backward(bg, fb, T, grad_fb)["opacity"]
+    i = int(np.argmax(np.abs(an)))
+    v = float(params[3][i])
+    hh = 1e-6
+    vals = []
+    for sgn in (1, -1):
+        p2 = [p.copy() for p in params]
+        p2[3][i] = v + sgn * hh
+        vals.append(_fd_loss(p2, deg, cam, bg, grad_fb))
+    fd = (vals[0] - vals[1]) / (2 * hh)
+    assert abs(fd - an[i]) <= 1e-5 * abs(fd)
+    planted = an[i] * (1 - 1 / 255)
+    assert abs(fd - planted) > 1e-5 * abs(fd)

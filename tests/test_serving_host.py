@@ -1,0 +1,75 @@
+"""Host logic of the continuous-batching server (CPU only): a batch that
+fails on a worker ends run() with that error instead of leaving it waiting for
+latencies that never come, and the admission budget is released."""
+import contextlib
+import threading
+
+import numpy as np
+import pytest
+import torch
+
+from differentiable_gaussian_mixture_renderer_amd import serving
+from differentiable_gaussian_mixture_renderer_amd._native import DgmrError
+
+
+class _FakeStream:
+    def synchronize(self):
+        pass
+
+
+@pytest.fixture
+def host_only(monkeypatch):
+    """Streams and the batched splatter replaced by host stand-ins."""
+    monkeypatch.setattr(serving, "_worker_stream", lambda device, k: _FakeStream())
+    monkeypatch.setattr(serving.torch.cuda, "stream", lambda s: contextlib.nullcontext())
+    monkeypatch.setattr(serving, "BatchSplatter", lambda **kw: None)
+
+
+def _server(n_requests=12):
+    resident = [serving.ResidentScene([torch.zeros(10, 1, 3), torch.zeros(10, 1), torch.zeros(10, 3), torch.zeros(10, 3),
+                                       torch.zeros(10, 4)], 0)]
+    trace = serving.make_trace(n_requests, rate_hz=2000.0, n_scenes=1, seed=1, min_res=(16, 16), max_res=(32, 32))
+    return serving.ContinuousBatchingServer(resident, hbm_budget_bytes=1 << 30, workers=2, device="cpu"), trace
+
+
+def test_worker_failure_ends_run_with_the_error(host_only, monkeypatch):
+    calls = {"n": 0}
+    lock = threading.Lock()
+
+    def flaky_single(self, sc, r):
+        with lock:
+            calls["n"] += 1
+            k = calls["n"]
+        if k == 3:
+            raise DgmrError(2, "dgmr_splat_forward", "pair capacity kept growing")
+        return [None], [None]
+
+    monkeypatch.setattr(serving.ContinuousBatchingServer, "_single", flaky_single)
+    monkeypatch.setattr(serving.ContinuousBatchingServer, "_batched", staticmethod(lambda sp, sc, batch, n_bwd: ([None] * len(batch),) * 2))
+    srv, trace = _server()
+    done = {}
+
+    def go():
+        try:
+            srv.run(trace)
+            done["result"] = "returned"
+        except RuntimeError as exc:
+            done["result"] = exc
+
+    th = threading.Thread(target=go, daemon=True)
+    th.start()
+    th.join(timeout=30.0)
+    assert not th.is_alive(), "run() kept waiting after a worker failed"
+    assert isinstance(done["result"], RuntimeError)
+    assert isinstance(done["result"].__cause__, DgmrError)
+    assert srv._in_flight == 0
+
+
+def test_all_batches_complete_without_failures(host_only, monkeypatch):
+    monkeypatch.setattr(serving.ContinuousBatchingServer, "_single", lambda self, sc, r: ([None], [None]))
+    monkeypatch.setattr(serving.ContinuousBatchingServer, "_batched", staticmethod(lambda sp, sc, batch, n_bwd: ([None] * len(batch),) * 2))
+    srv, trace = _server()
+    stats = srv.run(trace)
+    assert len(stats.latencies) == len(trace)
+    assert srv._in_flight == 0
+    assert np.all(np.array(stats.latencies) >= 0)
