@@ -169,11 +169,13 @@ def main():
     d_avg_ms = d_ms / max(d_launches, 1)
     achieved = (d_bytes / max(d_launches, 1)) / (d_avg_ms * 1e-3) / 1e9
     p_in = 44 + 12 * 16
-    # per pair: emission, two pair-sort passes and both blends; 16-bit pair keys
-    # (every view here has fewer than 65 536 tiles) save 2 bytes in the emission
-    # and 6 in each sort pass of the 124 that 32-bit keys move
-    frame_bytes = args.n * (3 * p_in + 160) + 110 * R + 44 * W * H + 8 * ((W + 15) // 16) * ((H + 15) // 16)
-    frame_hbm_frac = frame_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
+    # the shared frame byte model: per pair emission 12, sort 24, ranges 8 and
+    # both blend reads 40 + 40 (124 B); with 16-bit pair keys (every view here
+    # has fewer than 65 536 tiles) the emission and each of the two sort
+    # passes move 2 + 6 + 6 bytes less (110 B)
+    frame_rest = args.n * (3 * p_in + 160) + 44 * W * H + 8 * ((W + 15) // 16) * ((H + 15) // 16)
+    frame_hbm_frac = (frame_rest + 124 * R) / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
+    frame_hbm_frac_16bit = (frame_rest + 110 * R) / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     traffic = None
     tr_path = os.path.join(ROOT, "profiles", "traffic_splat_1M_1080p.json")
@@ -186,6 +188,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, cam, args.cpu_rows)
+    roofline = blend_roofline(d_name, d_avg_ms, d_bytes / max(d_launches, 1), achieved, traffic, cpu)
     del means, log_scales, quats, logits, sh, params, target
     torch.cuda.empty_cache()
     batched = None if args.no_batched else batched_workload(args.n, args.batched_cams)
@@ -215,10 +218,9 @@ def main():
                    "lds_batch": args.batch, "pairs": R, "visible": n_visible,
                    "activations": "torch" if args.torch_activations else "fused",
                    "l1_loss": "torch" if args.torch_loss else "fused"},
-        "roofline": {"bound": "hbm", "kernel": d_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "avg_launch_ms": round(d_avg_ms, 4), "algorithmic_bytes_per_launch": d_bytes / max(d_launches, 1)},
+        "roofline": roofline,
         "frame_hbm_fraction": round(frame_hbm_frac, 4),
+        "frame_hbm_fraction_16bit_keys": round(frame_hbm_frac_16bit, 4),
         "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0])},
         "cpu_baseline": cpu,
         "tight_cull": tight,
@@ -229,6 +231,38 @@ def main():
         "raster": raster,
     }
     print(json.dumps(line))
+
+
+# FLOP per (pixel, pair) evaluation of the blend: forward 12 + one exp,
+# backward 36 + one exp (the exp counted as one operation)
+BLEND_FLOP_PER_EVAL = {"splat.blend_forward": 13, "splat.blend_backward": 37}
+VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
+
+
+def blend_roofline(name, avg_ms, bytes_per_launch, achieved_gbs, traffic, cpu):
+    """Roofline of the dominant stage against both ceilings: algorithmic
+    HBM bytes per launch / duration against 8 TB/s, and algorithmic FLOP per
+    launch (the oracle's count of (pixel, pair) evaluations of the frame times
+    the per-evaluation FLOP) / duration against the FP32 vector peak.  `bound`
+    names the ceiling the kernel is closer to; `traffic` is the measured HBM
+    bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)."""
+    hbm = {"achieved_gbs": round(achieved_gbs, 2), "peak_gbs": HBM_PEAK_GBS, "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+           "algorithmic_bytes_per_launch": bytes_per_launch}
+    if traffic:
+        hbm["measured_traffic_frac"] = round(traffic / (avg_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
+    valu = None
+    work = (cpu or {}).get("blend_work")
+    if work and name in BLEND_FLOP_PER_EVAL:
+        flop = work["evaluations"] * BLEND_FLOP_PER_EVAL[name]
+        tflops = flop / (avg_ms * 1e-3) / 1e12
+        valu = {"achieved_tflops": round(tflops, 3), "peak_tflops": VALU_PEAK_TFLOPS, "frac": round(tflops / VALU_PEAK_TFLOPS, 4),
+                "evaluations_per_launch": work["evaluations"], "flop_per_evaluation": BLEND_FLOP_PER_EVAL[name],
+                "tile_evaluations_per_launch": work["tile_evaluations"]}
+    if valu is not None and valu["frac"] > hbm["frac"]:
+        head = {"bound": "valu", "achieved": valu["achieved_tflops"], "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": valu["frac"]}
+    else:
+        head = {"bound": "hbm", "achieved": hbm["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm["frac"]}
+    return {**head, "kernel": name, "traffic": traffic, "avg_launch_ms": round(avg_ms, 4), "hbm": hbm, "valu": valu}
 
 
 def tight_cull_run(args, lib, splatter, step, timed, prof_steps, frame):
@@ -409,9 +443,9 @@ def marcher_workload(lib, with_cpu: bool, steps: int = 3, max_large_steps: int =
         h.backward(bg, fb, T, grad, crop=(0, y0, W, y0 + rows), max_large_steps=max_large_steps)
         secs = time.perf_counter() - t0
         threads = os.environ.get("OMP_NUM_THREADS", str(os.cpu_count()))
-        out["cpu_baseline"] = {"value": round(W * rows / secs / 1e6, 5), "unit": "Mrays/s", "kind": "port",
-                               "sample": f"float restatement: preprocess+binning of all Gaussians, forward of {W}x{rows} rays on "
-                                         f"{threads} OpenMP threads, backward of the same rays on 1 thread, {secs:.1f} s"}
+        out["cpu_baseline"] = {"value": round(W * rows / secs / 1e6, 5), "unit": "Mrays/s", "kind": "port", "cores": int(threads),
+                               "sample": f"float restatement: preprocess+binning of all Gaussians, forward and backward of {W}x{rows} "
+                                         f"rays, on {threads} OpenMP threads, {secs:.1f} s"}
     return out
 
 
@@ -535,6 +569,12 @@ def cpu_baseline(scene, cam, rows: int):
     threads = max(1, min(16, os.cpu_count() or 1))
     secs_mt = h.time_fwd_bwd(bg, grad, crop=(0, y0, W, y0 + rows), threads=threads)
     secs_st = h.time_fwd_bwd(bg, grad, crop=(0, y0, W, y0 + rows), threads=1)
+    # the blend's algorithmic work on the whole frame: (pixel, pair)
+    # evaluations up to each pixel's stop, and list entries a tile traverses
+    # until all its pixels have stopped (times the tile's pixels)
+    _, _, _, n_evals = h.render(bg)
+    traversed, _ = h.tile_work()
+    tile_px = h.tile[0] * h.tile[1]
     px = W * rows
     model = ""
     try:
@@ -545,7 +585,9 @@ def cpu_baseline(scene, cam, rows: int):
     sample = (f"float restatement (g++ -O3, OpenMP): preprocess+binning of {scene.n} Gaussians, blend fwd+bwd of {W}x{rows} "
               f"pixels, preprocess bwd; {secs_mt:.1f} s on {threads} threads, {secs_st:.1f} s on 1 thread; host: {model}")
     return {"value": round(px / secs_mt / 1e6, 4), "unit": "Mpixel/s", "cores": threads, "kind": "port", "sample": sample,
-            "single_thread": {"value": round(px / secs_st / 1e6, 4), "unit": "Mpixel/s", "cores": 1}}
+            "host_cpus": os.cpu_count(), "host_model": model,
+            "single_thread": {"value": round(px / secs_st / 1e6, 4), "unit": "Mpixel/s", "cores": 1},
+            "blend_work": {"evaluations": int(n_evals), "tile_evaluations": int(traversed.astype(np.int64).sum()) * tile_px}}
 
 
 if __name__ == "__main__":

@@ -45,8 +45,8 @@ def lib() -> ctypes.CDLL:
         L.orc_splat_backward.argtypes = [vp, vp, i32, i32, i32, i32] + [vp] * 12
         L.orc_splat_tile_work.argtypes = [vp, vp, vp, vp]
         L.orc_march_get_geometry.argtypes = [vp] * 9
-        L.orc_march_render.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp]
-        L.orc_march_backward.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp] + [vp] * 9
+        L.orc_march_render.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, vp]
+        L.orc_march_backward.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, i32, vp, vp, vp] + [vp] * 9
         L.orc_time_splat_fwd_bwd.restype = f64
         L.orc_time_splat_fwd_bwd.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32]
         L.orc_portable_log_f.restype = ctypes.c_float
@@ -211,17 +211,29 @@ class MarchOracle(_Handle):
         arr = np.ascontiguousarray(frozen, np.float64).reshape(-1, 3)
         return arr.shape[0], arr
 
-    def render(self, bg, crop=None, max_large_steps=0, frozen=None):
+    def render(self, bg, crop=None, max_large_steps=0, frozen=None, with_ambiguity=False, float_sampling=False):
+        """(fb [3,H,W], T [H,W], large steps per pixel [H,W]) and, with
+        with_ambiguity, a [H,W] uint8 mask of pixels whose discrete decisions
+        lie within float32 rounding (meaningful for dbl=True): bit 0 section
+        comparisons, 1 step sizes, 2 border snaps, 3 mass thresholds, 4 mass
+        in the bins, 5 transmittance.  float_sampling (dbl=True): the bin
+        borders come from the float32 pipeline, everything on them is FP64."""
         x0, y0, x1, y1 = crop or (0, 0, self.W, self.H)
         fb = np.zeros((3, self.H, self.W), self.dt)
         T = np.ones((self.H, self.W), self.dt)
         nl = np.zeros((self.H, self.W), np.int32)
+        amb = np.zeros((self.H, self.W), np.uint8)
         nf, fr = self._frozen(frozen)
         bgc = _c(bg, self.dt)
-        lib().orc_march_render(self.h, P(bgc), x0, y0, x1, y1, max_large_steps, nf, P(fr), P(fb), P(T), P(nl))
-        return fb, T, nl
+        lib().orc_march_render(self.h, P(bgc), x0, y0, x1, y1, max_large_steps, nf, P(fr), int(float_sampling), P(fb), P(T), P(nl),
+                               P(amb) if with_ambiguity else None)
+        return (fb, T, nl, amb) if with_ambiguity else (fb, T, nl)
 
-    def backward(self, bg, fb, T, grad_fb, crop=None, max_large_steps=0, frozen=None):
+    def backward(self, bg, fb, T, grad_fb, crop=None, max_large_steps=0, frozen=None, float_sampling=False, sh_stored=16):
+        """Gradients of sum(grad_fb * image) (pixels with zero grad_fb are
+        skipped).  sh_stored < 16: the caller passed that many SH coefficients
+        per Gaussian, and the preprocess backward reads the rest as ones, as
+        the reference binding pads them (python_bindings/vol_marcher.cu:177)."""
         x0, y0, x1, y1 = crop or (0, 0, self.W, self.H)
         n, dt = self.n, self.dt
         out = dict(means=np.zeros((n, 3), dt), scales=np.zeros((n, 3), dt), rotations=np.zeros((n, 4), dt), opacity=np.zeros(n, dt),
@@ -229,9 +241,9 @@ class MarchOracle(_Handle):
                    r_inv_cov=np.zeros((n, 6), dt))
         nf, fr = self._frozen(frozen)
         bgc, fbc, Tc, gc = _c(bg, dt), _c(fb, dt), _c(T, dt), _c(grad_fb, dt)
-        lib().orc_march_backward(self.h, P(bgc), x0, y0, x1, y1, max_large_steps, nf, P(fr), P(fbc), P(Tc), P(gc), P(out["means"]),
-                                 P(out["scales"]), P(out["rotations"]), P(out["opacity"]), P(out["sh"]), P(out["r_mass"]),
-                                 P(out["r_rgb"]), P(out["r_means"]), P(out["r_inv_cov"]))
+        lib().orc_march_backward(self.h, P(bgc), x0, y0, x1, y1, max_large_steps, nf, P(fr), int(float_sampling), int(sh_stored), P(fbc),
+                                 P(Tc), P(gc), P(out["means"]), P(out["scales"]), P(out["rotations"]), P(out["opacity"]), P(out["sh"]),
+                                 P(out["r_mass"]), P(out["r_rgb"]), P(out["r_means"]), P(out["r_inv_cov"]))
         return out
 
 

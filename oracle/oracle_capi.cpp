@@ -342,13 +342,18 @@ void orc_march_get_geometry(void* p, int32_t* rect, uint32_t* tiles, void* xy, v
 }
 
 void orc_march_render(void* p, const void* bg, int x0, int y0, int x1, int y1, int max_large_steps, int n_frozen,
-    const double* frozen, void* fb, void* T_final, int32_t* n_large)
+    const double* frozen, int float_sampling, void* fb, void* T_final, int32_t* n_large, uint8_t* ambiguous)
 {
     auto* h = static_cast<MarchHandle*>(p);
     MarchOptions opt;
     opt.max_large_steps = max_large_steps;
     opt.n_frozen = n_frozen;
     opt.frozen = frozen;
+    float ipf[16];
+    for (int i = 0; i < 16; ++i)
+        ipf[i] = float(h->inv_proj[i]);
+    const FloatSampling fsamp { &h->gf, &h->sf.scene, &h->sf.cam, ipf };
+    const FloatSampling* fs = (h->dbl && float_sampling) ? &fsamp : nullptr;
     const auto go = [&](auto& st, auto& g, auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         crop_guard<S>(st.cam.width, st.cam.height, &x0, &y0, &x1, &y1);
@@ -365,8 +370,13 @@ void orc_march_render(void* p, const void* bg, int x0, int y0, int x1, int y1, i
             for (int px = x0; px < x1; ++px) {
                 S C[3], T;
                 int nl;
-                march_pixel(g, h->bins, st.scene, st.cam, ip, px, py, opt, C, &T, &nl);
+                TieWatch watch;
+                tie_watch = ambiguous ? &watch : nullptr;
+                march_pixel(g, h->bins, st.scene, st.cam, ip, px, py, opt, C, &T, &nl, fs);
+                tie_watch = nullptr;
                 const size_t q = size_t(py) * W + px;
+                if (ambiguous)
+                    ambiguous[q] = uint8_t(watch.hit);
                 for (int c = 0; c < 3; ++c)
                     out[c * plane + q] = C[c] + T * bgs[c];
                 Tout[q] = T;
@@ -381,14 +391,19 @@ void orc_march_render(void* p, const void* bg, int x0, int y0, int x1, int y1, i
 }
 
 void orc_march_backward(void* p, const void* bg, int x0, int y0, int x1, int y1, int max_large_steps, int n_frozen,
-    const double* frozen, const void* fb, const void* T_final, const void* grad_fb, void* g_means, void* g_scales,
-    void* g_rotations, void* g_opacity, void* g_sh, void* r_mass, void* r_rgb, void* r_means, void* r_inv_cov)
+    const double* frozen, int float_sampling, int sh_stored, const void* fb, const void* T_final, const void* grad_fb, void* g_means,
+    void* g_scales, void* g_rotations, void* g_opacity, void* g_sh, void* r_mass, void* r_rgb, void* r_means, void* r_inv_cov)
 {
     auto* h = static_cast<MarchHandle*>(p);
     MarchOptions opt;
     opt.max_large_steps = max_large_steps;
     opt.n_frozen = n_frozen;
     opt.frozen = frozen;
+    float ipf[16];
+    for (int i = 0; i < 16; ++i)
+        ipf[i] = float(h->inv_proj[i]);
+    const FloatSampling fsamp { &h->gf, &h->sf.scene, &h->sf.cam, ipf };
+    const FloatSampling* fs = (h->dbl && float_sampling) ? &fsamp : nullptr;
     const auto go = [&](auto& st, auto& g, auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         crop_guard<S>(st.cam.width, st.cam.height, &x0, &y0, &x1, &y1);
@@ -401,22 +416,60 @@ void orc_march_backward(void* p, const void* bg, int x0, int y0, int x1, int y1,
         const S* fbs = static_cast<const S*>(fb);
         const S* Ts = static_cast<const S*>(T_final);
         const S* gs = static_cast<const S*>(grad_fb);
-        MarchRenderGrads<S> gr;
-        gr.mass.assign(n, 0);
-        gr.rgb.assign(3 * n, 0);
-        gr.means.assign(3 * n, 0);
-        gr.inv_cov.assign(6 * n, 0);
-        for (int py = y0; py < y1; ++py)
-            for (int px = x0; px < x1; ++px) {
-                const size_t q = size_t(py) * W + px;
-                const S gC[3] = { gs[q], gs[plane + q], gs[2 * plane + q] };
-                if (gC[0] == 0 && gC[1] == 0 && gC[2] == 0)
-                    continue;
-                const S fpx[3] = { fbs[q], fbs[plane + q], fbs[2 * plane + q] };
-                march_pixel_backward(g, h->bins, st.scene, st.cam, ip, px, py, opt, fpx, Ts[q], gC, static_cast<const S*>(bg), &gr);
+        const auto zeroed = [n](MarchRenderGrads<S>& a) {
+            a.mass.assign(n, 0);
+            a.rgb.assign(3 * n, 0);
+            a.means.assign(3 * n, 0);
+            a.inv_cov.assign(6 * n, 0);
+        };
+        // rows in fixed contiguous blocks per thread, each thread summing into
+        // its own buffers, reduced in thread order: deterministic for a given
+        // thread count
+        const int n_threads = std::max(1, std::min(omp_get_max_threads(), y1 - y0));
+        std::vector<MarchRenderGrads<S>> part(n_threads);
+#pragma omp parallel for schedule(static, 1) num_threads(n_threads)
+        for (int t = 0; t < n_threads; ++t) {
+            zeroed(part[t]);
+            const int r0 = y0 + int(int64_t(y1 - y0) * t / n_threads), r1 = y0 + int(int64_t(y1 - y0) * (t + 1) / n_threads);
+            for (int py = r0; py < r1; ++py)
+                for (int px = x0; px < x1; ++px) {
+                    const size_t q = size_t(py) * W + px;
+                    const S gC[3] = { gs[q], gs[plane + q], gs[2 * plane + q] };
+                    if (gC[0] == 0 && gC[1] == 0 && gC[2] == 0)
+                        continue;
+                    const S fpx[3] = { fbs[q], fbs[plane + q], fbs[2 * plane + q] };
+                    march_pixel_backward(g, h->bins, st.scene, st.cam, ip, px, py, opt, fpx, Ts[q], gC, static_cast<const S*>(bg),
+                        &part[t], fs);
+                }
+        }
+        MarchRenderGrads<S> gr = std::move(part[0]);
+        for (int t = 1; t < n_threads; ++t) {
+            for (size_t i = 0; i < n; ++i)
+                gr.mass[i] += part[t].mass[i];
+            for (size_t i = 0; i < 3 * n; ++i) {
+                gr.rgb[i] += part[t].rgb[i];
+                gr.means[i] += part[t].means[i];
             }
+            for (size_t i = 0; i < 6 * n; ++i)
+                gr.inv_cov[i] += part[t].inv_cov[i];
+        }
+        // The reference binding pads SH coefficients the caller did not pass
+        // (sh_stored of 16) with ones for the backward, zeros for the forward
+        // (python_bindings/vol_marcher.cu:92 vs :177); the render backward
+        // replays with the forward's colours, only the preprocess backward
+        // sees the ones.
+        Scene<S> bsc = st.scene;
+        std::vector<S> sh_ones;
+        if (sh_stored > 0 && sh_stored < 16) {
+            sh_ones.assign(st.scene.sh, st.scene.sh + 48 * n);
+            for (size_t i = 0; i < n; ++i)
+                for (int k = sh_stored; k < 16; ++k)
+                    for (int c = 0; c < 3; ++c)
+                        sh_ones[48 * i + 3 * k + c] = S(1);
+            bsc.sh = sh_ones.data();
+        }
         ParamGrads<S> pg;
-        march_preprocess_backward(st.scene, st.cam, g, gr, &pg);
+        march_preprocess_backward(bsc, st.cam, g, gr, &pg);
         store_out(pg.means, g_means);
         store_out(pg.scales, g_scales);
         store_out(pg.rotations, g_rotations);

@@ -27,6 +27,60 @@ struct MarchConfig {
     static constexpr float relevance_sigma = 3.f;
 };
 
+// ---------------------------------------------------------------- float ambiguity
+// A pixel's march takes discrete decisions: section comparisons, a bin border
+// snapping to a section end, mass and transmittance thresholds.  The FP64 run
+// marks a pixel whose decisions compare values closer than a float32 run can
+// resolve, because a float run may then take the other branch and sample
+// differently from there on.  The watch is installed per pixel (thread-local,
+// so OpenMP rows stay independent); with none installed the checks do nothing.
+struct TieWatch {
+    unsigned hit = 0;   // bit set of the decision kinds below that were near a tie
+};
+enum TieKind : unsigned {
+    kTieSection = 1,       // section list comparisons (put / combine)
+    kTieStep = 2,          // step sizes of overlapping sections
+    kTieBorder = 4,        // a border snapping to a section end
+    kTieMass = 8,          // mass on the ray against its thresholds
+    kTieInBins = 16,       // a Gaussian's mass in the bins against 1e-4
+    kTieTransmittance = 32 // transmittance against 1/255
+};
+inline thread_local TieWatch* tie_watch = nullptr;
+inline thread_local unsigned tie_ignore = 0;   // kinds not recorded (decisions shared with the float32 run)
+
+struct TieTolerance {
+    // positions along the ray: float error of a border after up to 2 x 127
+    // accumulated steps and of the 1D centre is below 1e-5 of max(|t|, 1)
+    static constexpr double position = 2e-5;
+    // step sizes 6 sd / 31 compared between sections (relative)
+    static constexpr double step = 2e-5;
+    // mass on the ray (weight x mass) against its thresholds (relative)
+    static constexpr double mass = 1e-4;
+    // transmittance against 1/255 (relative): the float cdf's absolute error
+    // (1.5e-7 per evaluation) and skipped bins under 1e-5 move ln T by ~1e-5
+    static constexpr double transmittance = 2e-4;
+    // absolute error of a float mass in bins per unit of mass on the ray
+    static constexpr double cdf_abs = 4e-7;
+};
+
+inline void watch_near(double a, double b, double rel, double floor, unsigned kind)
+{
+    if (!tie_watch || a == b || !std::isfinite(a) || !std::isfinite(b))
+        return;
+    const double scale = std::max({ std::abs(a), std::abs(b), floor });
+    if (std::abs(a - b) <= rel * scale)
+        tie_watch->hit |= kind & ~tie_ignore;
+}
+inline void watch_position(double a, double b, unsigned kind = kTieSection) { watch_near(a, b, TieTolerance::position, 1.0, kind); }
+inline void watch_step(double a, double b) { watch_near(a, b, TieTolerance::step, 0.0, kTieStep); }
+inline void watch_mass(double v, double threshold) { watch_near(v, threshold, TieTolerance::mass, 0.0, kTieMass); }
+inline void watch_transmittance(double v) { watch_near(v, 1.0 / 255.0, TieTolerance::transmittance, 0.0, kTieTransmittance); }
+inline void watch_abs(double v, double threshold, double tol)
+{
+    if (tie_watch && std::abs(v - threshold) <= tol)
+        tie_watch->hit |= kTieInBins;
+}
+
 // ---------------------------------------------------------------- sampling sections
 template <class S>
 struct Section {
@@ -39,6 +93,9 @@ struct Section {
 template <class S>
 inline void combine_sections(const Section<S>& a, const Section<S>& b, Section<S>* out)
 {
+    watch_position(a.start, b.start);
+    watch_position(b.end, a.end);
+    watch_step(a.dt, b.dt);
     if (a.start < b.start)
         out[0] = { a.start, std::min(a.end, b.start), a.dt };
     else
@@ -67,6 +124,8 @@ public:
 
     void put(Section<S> e)
     {
+        watch_position(e.end, start_);
+        watch_position(e.start, end_);
         if (e.end <= start_ || e.start >= end_)
             return;
         e.start = std::max(start_, e.start);
@@ -81,6 +140,8 @@ public:
             return;
         }
         unsigned stop = first_ending_after(e.end, first);
+        if (stop < size_)
+            watch_position(data_[stop].start, e.end);
         if (stop < size_ && data_[stop].start < e.end)
             ++stop;
 
@@ -93,6 +154,8 @@ public:
             if (ring_r >= ring_w)
                 break;
             const Section<S>& head = ring[ring_r % CAP];
+            watch_position(head.end, e.start);
+            watch_position(head.start, e.end);
             if (head.end <= e.start) {
                 data_[wr++] = ring[(ring_r++) % CAP];
                 continue;
@@ -140,9 +203,11 @@ public:
 private:
     unsigned first_ending_after(S v, unsigned from) const
     {
-        for (unsigned i = from; i < size_; ++i)
+        for (unsigned i = from; i < size_; ++i) {
+            watch_position(data_[i].end, v);
             if (data_[i].end > v)
                 return i;
+        }
         return size_;
     }
     unsigned size_ = 0;
@@ -162,6 +227,8 @@ inline void sample_borders(const SectionList<S, CAP>& sec, S* borders, unsigned 
     borders[0] = sec[0].start;
     unsigned cur = 0;
     for (unsigned i = 1; i < n; ++i) {
+        if (cur < sec.size() - 1)
+            watch_position(borders[i - 1] + sec[cur].dt, sec[cur].end, kTieBorder);
         const S v = std::min(sec[cur].end, borders[i - 1] + sec[cur].dt);
         borders[i] = v;
         if (cur < sec.size() - 1 && v == sec[cur].end)
@@ -268,11 +335,15 @@ inline void sample_gaussian(S mass, const S* rgb, const S* mu, const S* A, const
     const S mor = g.weight * mass;
     if (std::isnan(g.centre))
         return;
+    watch_mass(mor, 1.0f / 255.f);
+    watch_mass(mor, 1000.0);
     if (mor < S(1.0f / 255.f) || mor > S(1000))
         return;
+    watch_mass(sd, 0.0001);
     if (sd <= S(0.0001) || std::isnan(sd) || std::isnan(mor) || mor > S(100000))
         return;
     const S in_bins = mor * (cdf_inv_sd(g.centre, inv_sd, borders[kBins]) - cdf_inv_sd(g.centre, inv_sd, borders[0]));
+    watch_abs(in_bins, 0.0001f, 0.0001 * TieTolerance::mass + double(mor) * TieTolerance::cdf_abs);
     if (in_bins < S(0.0001f))
         return;
     S c0 = cdf_inv_sd(g.centre, inv_sd, borders[0]);
@@ -280,6 +351,8 @@ inline void sample_gaussian(S mass, const S* rgb, const S* mu, const S* A, const
         const S c1 = cdf_inv_sd(g.centre, inv_sd, borders[k + 1]);
         const S m = std::max(S(0), (c1 - c0) * mor);
         c0 = c1;
+        // a bin's skip below 1e-5 is not watched: flipping it moves the
+        // image by at most 1e-5 x colour x transmittance
         if (m < S(0.00001f))
             continue;
         bins[k][0] += rgb[0] * m;
@@ -347,6 +420,7 @@ inline void integrate_bins(S* C, S* T, const S (*bins)[4])
         for (int i = 0; i < 3; ++i)
             C[i] += bins[k][i] * *T;
         *T *= tk;
+        watch_transmittance(*T);
         if (*T < S(1. / 255.))
             break;
     }
@@ -542,9 +616,12 @@ inline void build_sections(const MarchGeometry<S>& g, const Binning& b, int tile
         const uint32_t id = b.point_list[k];
         const Gaussian1d<S> g1 = intersect_ray(sc.means + 3 * id, &g.inv_cov[6 * id], o, d);
         const S sd = std::sqrt(g1.var);
+        watch_position(sec->end(), g.depth[id]);
         if (sec->end() < g.depth[id])
             break;
         const S mor = g1.weight * g.mass[id];
+        watch_mass(mor, 1.0f / 255.f);
+        watch_mass(mor, 1000.0);
         if (mor <= S(1.0f / 255.f) || mor > S(1000))
             continue;
         if (g1.var <= 0)
@@ -556,14 +633,75 @@ inline void build_sections(const MarchGeometry<S>& g, const Binning& b, int tile
     }
 }
 
-// one pixel of the forward march, vol_marcher_forward.cu:340-499
+// Ray and bin placement taken from the float32 pipeline: the pixel's ray
+// direction, and the bin borders of each large step (pass 1 and
+// marching_steps::sample) from the float32 geometry and section arithmetic.
+// A double march that uses them samples exactly where a float32 march
+// samples, so the two differ only by the rounding of what they compute along
+// that ray on those bins.  Both placements are discontinuous in rounding:
+//  * the ray unprojects NDC z = 0 (vol_marcher_forward.cu:308-315), a point
+//    about 2 znear from the camera, and subtracts the camera position, so a
+//    float32 direction carries ~1e-5 relative error by construction, which
+//    moves the ray ~1e-4 at the Gaussians and their mass on it by ~1e-3;
+//  * the border walk lands on section ends by design (a section is 31 steps
+//    long), so a float32 and a double march that each place their own borders
+//    split such ties by rounding noise and can sample a large step with one
+//    bin more or less.
+struct FloatSampling {
+    const MarchGeometry<float>* g = nullptr;
+    const Scene<float>* sc = nullptr;
+    const Camera<float>* cam = nullptr;
+    const float* inv_proj = nullptr;
+};
+
+template <class S>
+inline unsigned large_step_borders(const MarchGeometry<S>& g, const Binning& b, const Scene<S>& sc, int tile, const S* o, const S* d,
+    const MarchOptions& opt, S start, S* borders)
+{
+    SectionList<S, MarchConfig::n_large_steps> sec(start);
+    build_sections(g, b, tile, o, d, sc, opt, &sec);
+    sample_borders(sec, borders, MarchConfig::n_small_steps);
+    return sec.size();
+}
+
+template <class S>
+inline void float_ray(const FloatSampling& fs, int px, int py, S* o, S* d)
+{
+    float df[3];
+    pixel_ray(*fs.cam, fs.inv_proj, px, py, df);
+    for (int i = 0; i < 3; ++i) {
+        o[i] = S(fs.cam->pos[i]);
+        d[i] = S(df[i]);
+    }
+}
+
+template <class S>
+inline unsigned large_step_borders(const FloatSampling& fs, const Binning& b, int tile, int px, int py, const MarchOptions& opt, S start,
+    S* borders)
+{
+    float d[3], fb[MarchConfig::n_small_steps];
+    pixel_ray(*fs.cam, fs.inv_proj, px, py, d);
+    // the section and border decisions are the float32 run's own: a float32
+    // march that computes them in the same order takes the same ones
+    tie_ignore = kTieSection | kTieStep | kTieBorder;
+    const unsigned n = large_step_borders(*fs.g, b, *fs.sc, tile, fs.cam->pos, d, opt, float(start), fb);
+    tie_ignore = 0;
+    for (unsigned i = 0; i < MarchConfig::n_small_steps; ++i)
+        borders[i] = S(fb[i]);
+    return n;
+}
+
+// one pixel of the forward march, vol_marcher_forward.cu:340-499; with `fs`
+// the bin borders come from the float32 pipeline (see FloatSampling)
 template <class S>
 inline void march_pixel(const MarchGeometry<S>& g, const Binning& b, const Scene<S>& sc, const Camera<S>& cam,
-    const S* inv_proj, int px, int py, const MarchOptions& opt, S* C, S* T_out, int* n_large)
+    const S* inv_proj, int px, int py, const MarchOptions& opt, S* C, S* T_out, int* n_large, const FloatSampling* fs = nullptr)
 {
     S o[3] = { cam.pos[0], cam.pos[1], cam.pos[2] };
     S d[3];
     pixel_ray(cam, inv_proj, px, py, d);
+    if (fs)
+        float_ray(*fs, px, py, o, d);
     const int tile = (py / b.tile_h) * b.grid_x + (px / b.tile_w);
     S T = 1;
     C[0] = C[1] = C[2] = 0;
@@ -573,9 +711,8 @@ inline void march_pixel(const MarchGeometry<S>& g, const Binning& b, const Scene
     std::vector<S> binsv(4 * kBins);
     S (*bins)[4] = reinterpret_cast<S (*)[4]>(binsv.data());
     while (true) {
-        SectionList<S, MarchConfig::n_large_steps> sec(start);
-        build_sections(g, b, tile, o, d, sc, opt, &sec);
-        sample_borders(sec, borders.data(), MarchConfig::n_small_steps);
+        const unsigned n_sections = fs ? large_step_borders(*fs, b, tile, px, py, opt, start, borders.data())
+                                       : large_step_borders(g, b, sc, tile, o, d, opt, start, borders.data());
         std::fill(binsv.begin(), binsv.end(), S(0));
         for (uint32_t k = b.ranges[2 * tile]; k < b.ranges[2 * tile + 1]; ++k) {
             const uint32_t id = b.point_list[k];
@@ -583,7 +720,7 @@ inline void march_pixel(const MarchGeometry<S>& g, const Binning& b, const Scene
         }
         integrate_bins(C, &T, bins);
         ++steps;
-        if (sec.size() == 0 || T < S(1.f / 255.f))
+        if (n_sections == 0 || T < S(1.f / 255.f))
             break;
         if (opt.max_large_steps > 0 && steps >= opt.max_large_steps)
             break;
@@ -602,15 +739,18 @@ struct MarchRenderGrads {
     std::vector<S> inv_cov; // [n,6]
 };
 
-// one pixel of the backward march, vol_marcher_backward.cu:126-318
+// one pixel of the backward march, vol_marcher_backward.cu:126-318 (`fs` as
+// in march_pixel)
 template <class S>
 inline void march_pixel_backward(const MarchGeometry<S>& g, const Binning& b, const Scene<S>& sc, const Camera<S>& cam,
     const S* inv_proj, int px, int py, const MarchOptions& opt, const S* fb_px, S T_final, const S* gC, const S* bg,
-    MarchRenderGrads<S>* gr)
+    MarchRenderGrads<S>* gr, const FloatSampling* fs = nullptr)
 {
     S o[3] = { cam.pos[0], cam.pos[1], cam.pos[2] };
     S d[3];
     pixel_ray(cam, inv_proj, px, py, d);
+    if (fs)
+        float_ray(*fs, px, py, o, d);
     const int tile = (py / b.tile_h) * b.grid_x + (px / b.tile_w);
     S C[3];
     for (int i = 0; i < 3; ++i)
@@ -624,9 +764,8 @@ inline void march_pixel_backward(const MarchGeometry<S>& g, const Binning& b, co
     S (*bins)[4] = reinterpret_cast<S (*)[4]>(binsv.data());
     S (*gbins)[4] = reinterpret_cast<S (*)[4]>(gbinsv.data());
     while (true) {
-        SectionList<S, MarchConfig::n_large_steps> sec(start);
-        build_sections(g, b, tile, o, d, sc, opt, &sec);
-        sample_borders(sec, borders.data(), MarchConfig::n_small_steps);
+        const unsigned n_sections = fs ? large_step_borders(*fs, b, tile, px, py, opt, start, borders.data())
+                                       : large_step_borders(g, b, sc, tile, o, d, opt, start, borders.data());
         std::fill(binsv.begin(), binsv.end(), S(0));
         for (uint32_t k = b.ranges[2 * tile]; k < b.ranges[2 * tile + 1]; ++k) {
             const uint32_t id = b.point_list[k];
@@ -646,7 +785,7 @@ inline void march_pixel_backward(const MarchGeometry<S>& g, const Binning& b, co
                 gr->inv_cov[6 * id + i] += gA[i];
         }
         ++steps;
-        if (sec.size() == 0 || T < S(1.f / 255.f))
+        if (n_sections == 0 || T < S(1.f / 255.f))
             break;
         if (opt.max_large_steps > 0 && steps >= opt.max_large_steps)
             break;

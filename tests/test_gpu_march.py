@@ -6,13 +6,19 @@ Bars:
     precision;
   * double GPU vs double oracle: image and gradients within 1e-9 (relative to
     the largest magnitude), i.e. the same algorithm up to libm rounding;
-  * float GPU vs double oracle (float binning): image within 1e-3 except on a
-    bounded fraction of pixels where float-vs-double sample placement differs;
+  * float GPU vs the FP64 oracle with the float32 placement (the float
+    pipeline's binning, ray direction and bin borders, everything evaluated on
+    them in FP64): image and transmittance max |gpu - fp64| <= 1e-4 over the
+    pixels whose mass or transmittance thresholds are not within float
+    rounding (the oracle flags them, <= 2 %); gradients with dL/dimg zeroed on
+    flagged pixels, per parameter tensor max |gpu - fp64| <= 1e-3 max |fp64|
+    and ||gpu - fp64|| <= 1e-3 ||fp64||.  Each precision placing its own ray
+    and borders is not comparable at that level: the reference unprojects NDC
+    z = 0, about 2 znear from the camera, so a float32 ray direction carries
+    ~1e-5 relative error, and its border walk lands on section ends by design;
   * double gradients against central finite differences of the GPU forward on
     fixed sample sections (the reference's gradient-check mode), 32 random
-    parameters, relative error <= 1e-4;
-  * float gradients: L2-relative <= 2e-3 against the float restatement and
-    <= 2e-2 against the double oracle.
+    parameters, relative error <= 1e-4.
 """
 import numpy as np
 import pytest
@@ -89,6 +95,10 @@ def test_forward_double_matches_double_oracle():
     assert np.abs(out["remaining"] - T).max() <= 1e-9
 
 
+IMG_TOL = 1e-4
+GRAD_TOL = 1e-3
+
+
 def test_forward_float_binning_and_image():
     torch = _torch()
     scene, cam = small_scene()
@@ -97,12 +107,28 @@ def test_forward_float_binning_and_image():
     ref32 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=False)
     check_binning(out, ref32)
     ref64 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=True)
-    fb, T, _ = ref64.render(bg)
+    fb, T, _, amb = ref64.render(bg, with_ambiguity=True, float_sampling=True)
+    ok = amb == 0
     err = np.abs(out["framebuffer"] - fb).max(axis=0)
-    frac = float((err > 1e-3).mean())
-    print(f"float vs double: median {np.median(err):.3g}, p99 {np.quantile(err, 0.99):.3g}, >1e-3: {frac:.4f}")
-    assert np.median(err) <= 1e-5
-    assert frac <= 0.02
+    own, _, _ = ref64.render(bg)
+    err_own = np.abs(out["framebuffer"] - own).max(axis=0)
+    print(f"float vs FP64 (float placement): flagged {(~ok).mean():.4f}, max (unflagged) {err[ok].max():.3g}, max (all) {err.max():.3g}; "
+          f"vs FP64 own placement: median {np.median(err_own):.3g}, max {err_own.max():.3g}")
+    assert (~ok).mean() <= 0.02
+    assert err[ok].max() <= IMG_TOL
+    assert np.abs(out["remaining"] - T)[ok].max() <= IMG_TOL
+
+
+def _check_grads(got_all, want_all, n, k, label):
+    for name in ("means", "scales", "rotations", "opacity", "sh"):
+        got = got_all[name].reshape(n, -1)
+        want = want_all[name].reshape(n, -1)[:, :got.shape[1]]
+        scale = np.abs(want).max()
+        assert scale > 0, name
+        max_rel = np.abs(got - want).max() / scale
+        l2_rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+        print(f"{label} {name}: max rel {max_rel:.3g}, l2 rel {l2_rel:.3g}")
+        assert max_rel <= GRAD_TOL and l2_rel <= GRAD_TOL, (label, name)
 
 
 def test_backward_double_matches_double_oracle():
@@ -166,28 +192,54 @@ def test_backward_against_finite_differences():
 
 
 def test_backward_float_matches_oracle():
-    """float GPU gradients against the float restatement (same decisions) and
-    the double oracle (float binning): relative L2 error per parameter tensor."""
+    """float GPU gradients against the FP64 oracle with the float32 placement,
+    dL/dimg zeroed on the pixels it flags."""
     torch = _torch()
     scene, cam = small_scene(n=1500, w=48, h=32)
     bg = np.array([0.2, 0.3, 0.4], np.float32)
     rng = np.random.default_rng(5)
-    ref32 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=False)
-    fb32, T32, _ = ref32.render(bg)
-    grad = (rng.normal(size=fb32.shape) / fb32.size).astype(np.float32)
-    out = run_gpu(scene, cam, bg, torch.float32, grad_fb=grad)
-    r32 = ref32.backward(bg, fb32, T32, grad)
     ref64 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=True)
-    fb64, T64, _ = ref64.render(bg)
-    r64 = ref64.backward(bg, fb64, T64, grad.astype(np.float64))
-    n, k = scene.n, scene.sh.shape[1]
-    for name in ("means", "scales", "rotations", "opacity", "sh"):
-        got = out["grads"][name].reshape(n, -1)
-        for ref, tol, label in ((r32, 2e-3, "f32 oracle"), (r64, 2e-2, "f64 oracle")):
-            want = ref[name].reshape(n, -1)[:, :got.shape[1]]
-            rel = np.linalg.norm(got - want) / np.linalg.norm(want)
-            print(f"{name} vs {label}: l2 rel {rel:.3g}")
-            assert rel <= tol, (name, label)
+    fb64, T64, _, amb = ref64.render(bg, with_ambiguity=True, float_sampling=True)
+    grad = (rng.normal(size=fb64.shape) / fb64.size).astype(np.float32)
+    grad[:, amb != 0] = 0
+    out = run_gpu(scene, cam, bg, torch.float32, grad_fb=grad)
+    r64 = ref64.backward(bg, fb64, T64, grad.astype(np.float64), float_sampling=True)
+    _check_grads(out["grads"], r64, scene.n, scene.sh.shape[1], "f32 vs f64")
+
+
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+def test_backward_with_fewer_sh_coefficients(dtype_name):
+    """A caller passing 4 SH coefficients at active degree 3: the forward pads
+    the rest with zeros and the backward's preprocess reads them as ones
+    (python_bindings/vol_marcher.cu:92, :177); the gradient holds 4 rows."""
+    torch = _torch()
+    dtype = getattr(torch, dtype_name)
+    scene, cam = small_scene(n=1200, w=48, h=32, seed=9)
+    scene.sh = np.ascontiguousarray(scene.sh[:, :4])
+    bg = np.array([0.2, 0.3, 0.4])
+    dbl = dtype_name == "float64"
+    ref = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=not dbl)
+    if dbl:
+        fb, T, _ = ref.render(bg)
+        amb = np.zeros(T.shape, np.uint8)
+    else:
+        fb, T, _, amb = ref.render(bg, with_ambiguity=True, float_sampling=True)
+    grad = np.random.default_rng(13).normal(size=fb.shape) / fb.size
+    grad[:, amb != 0] = 0
+    out = run_gpu(scene, cam, bg.astype(np.float32) if not dbl else bg, dtype, grad_fb=grad.astype(np.float32) if not dbl else grad)
+    assert out["grads"]["sh"].shape == (scene.n, 4, 3)
+    want = ref.backward(bg, fb, T, grad, float_sampling=not dbl, sh_stored=4)
+    if dbl:
+        for name in ("means", "scales", "rotations", "opacity", "sh"):
+            got = out["grads"][name].reshape(scene.n, -1)
+            w = want[name].reshape(scene.n, -1)[:, :got.shape[1]]
+            rel = np.abs(got - w).max() / np.abs(w).max()
+            print(f"{name}: max rel {rel:.3g}")
+            assert rel <= 1e-9, name
+        plain = ref.backward(bg, fb, T, grad)
+        assert np.abs(plain["means"] - want["means"]).max() > 1e-6 * np.abs(want["means"]).max()   # the padding is visible
+    else:
+        _check_grads(out["grads"], want, scene.n, 4, "f32 sh4")
 
 
 def test_volmarcher_autograd_and_large_step_cap():
