@@ -497,11 +497,11 @@ def raster_workload(lib, with_cpu: bool, steps: int = 3):
 
 def serving_workload(n_requests: int, rate_hz: float = 200.0):
     """Poisson arrivals at rate_hz over 8 resident scenes (50k..3M Gaussians,
-    log-uniform), cameras 256^2..1080p, 30 % forward+backward: continuous
+    log-spaced), cameras 256^2..1080p, 30 % forward+backward: continuous
     batching against a FIFO server that renders one request at a time through
-    the plain path.  Also both servers with every request present at t=0
-    (saturated, median of three runs), which measures sustainable
-    throughput."""
+    the plain path, at rate_hz, at 1000/s and 2000/s, and with every request
+    present at t=0 (saturated, median of three runs), which measures
+    sustainable throughput."""
     import copy
 
     import numpy as np
@@ -511,8 +511,8 @@ def serving_workload(n_requests: int, rate_hz: float = 200.0):
 
     resident = serving.make_resident_scenes(8, seed=5)
     trace = serving.make_trace(n_requests, rate_hz, 8, seed=5)
-    out = {"workload": f"{n_requests} requests, Poisson {rate_hz:g}/s, 8 resident scenes of 50k..3M Gaussians, 256^2..1080p, "
-                       "30% fwd+bwd", "scenes_gaussians": [sc.n for sc in resident]}
+    out = {"workload": f"{n_requests} requests, Poisson {rate_hz:g}/s, 8 resident scenes of 50k..3M Gaussians (log-spaced), "
+                       "256^2..1080p, 30% fwd+bwd", "scenes_gaussians": [sc.n for sc in resident]}
     budget = 48 << 30
     retries0 = torch.cuda.memory_stats().get("num_alloc_retries", 0)
 
@@ -530,6 +530,14 @@ def serving_workload(n_requests: int, rate_hz: float = 200.0):
     run(serving.FifoServer, copy.deepcopy(warm))
     out["continuous"] = run(serving.ContinuousBatchingServer, copy.deepcopy(trace), hbm_budget_bytes=budget)
     out["fifo_plain"] = run(serving.FifoServer, copy.deepcopy(trace))
+    # latency and throughput against the arrival rate: the same scenes and
+    # request mix at higher Poisson rates
+    sweep = {}
+    for rate in (1000.0, 2000.0):
+        tr = serving.make_trace(n_requests, rate, 8, seed=5)
+        sweep[f"{rate:g}/s"] = {"continuous": run(serving.ContinuousBatchingServer, copy.deepcopy(tr), hbm_budget_bytes=budget),
+                                "fifo_plain": run(serving.FifoServer, copy.deepcopy(tr))}
+    out["rate_sweep"] = sweep
     sat = copy.deepcopy(trace)
     for r in sat:
         r.arrival = 0.0

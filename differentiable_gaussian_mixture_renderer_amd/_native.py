@@ -113,6 +113,9 @@ class MarchOptions(ctypes.Structure):
         ("n_frozen", ctypes.c_int32),
         ("frozen", (ctypes.c_double * 3) * 8),
         ("bins_per_chunk", ctypes.c_int32),
+        ("render_mode", ctypes.c_int32),
+        ("debug_render_bin", ctypes.c_int32),
+        ("max_depth", ctypes.c_double),
     ]
 
 

@@ -154,18 +154,63 @@ void stage_set_bytes(int slot, double bytes)
         g_prof.records[slot].bytes = bytes;
 }
 
+// The pair-count read-back slots: one pinned host word and one event per
+// calling thread.  A thread takes a slot on its first count and hands it back
+// to a process-wide free list when it ends (no HIP call at thread exit), so
+// the pinned memory is bounded by the largest number of threads that have
+// counted at the same time, however many threads come and go.
+namespace {
+struct CountSlot {
+    uint32_t* host = nullptr;
+    hipEvent_t done = nullptr;
+};
+std::mutex g_slot_mutex;
+std::vector<CountSlot>* g_free_slots = new std::vector<CountSlot>(); // never destroyed: threads may end after static destructors
+struct ThreadSlot {
+    CountSlot slot;
+    ~ThreadSlot()
+    {
+        if (!slot.host)
+            return;
+        std::lock_guard<std::mutex> lk(g_slot_mutex);
+        g_free_slots->push_back(slot);
+    }
+};
+thread_local ThreadSlot t_slot;
+
+int acquire_count_slot(CountSlot* out)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_slot_mutex);
+        if (!g_free_slots->empty()) {
+            *out = g_free_slots->back();
+            g_free_slots->pop_back();
+            return DGMR_OK;
+        }
+    }
+    void* p = nullptr;
+    DGMR_HIP_CHECK(hipHostMalloc(&p, 256, hipHostMallocDefault));
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipHostFree(p);
+        set_error("hipEventCreateWithFlags failed for the pair count");
+        return DGMR_ERR_DEVICE;
+    }
+    out->host = static_cast<uint32_t*>(p);
+    out->done = e;
+    return DGMR_OK;
+}
+} // namespace
+
 int enqueue_count(const uint32_t* dev_count, hipStream_t st, PendingCount* pc)
 {
-    // one pinned word and one event per thread, kept for the life of the process
-    thread_local uint32_t* host = nullptr;
-    thread_local hipEvent_t done = nullptr;
-    if (!host) {
-        void* p = nullptr;
-        DGMR_HIP_CHECK(hipHostMalloc(&p, 256, hipHostMallocDefault));
-        host = static_cast<uint32_t*>(p);
+    if (!t_slot.slot.host) {
+        const int rc = acquire_count_slot(&t_slot.slot);
+        if (rc != DGMR_OK)
+            return rc;
     }
-    if (!done)
-        DGMR_HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    uint32_t* host = t_slot.slot.host;
+    hipEvent_t done = t_slot.slot.done;
     DGMR_HIP_CHECK(hipMemcpyAsync(host, dev_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     DGMR_HIP_CHECK(hipEventRecord(done, st));
     pc->device = dev_count;

@@ -27,11 +27,19 @@ from .. import _native as N
 DEFAULT_TILING = ((16, 16), 256)
 # march options: large-step cap (0 = unbounded, as the reference) and frozen
 # sample sections for gradient checks (DGMR_TORCH_GRAD_CHECK_CONST_SAMPLES)
-DEFAULT_OPTIONS = {"max_large_steps": 0, "frozen": None, "bins_per_chunk": 0}
+# and the reference's debug render modes (ForwardData::debug_render_mode,
+# debug_render_bin, max_depth; vol_marcher.h:44-55), forward only
+DEFAULT_OPTIONS = {"max_large_steps": 0, "frozen": None, "bins_per_chunk": 0, "render_mode": 0, "debug_render_bin": -1,
+                   "max_depth": 20.0}
+RENDER_MODES = {"full": 0, "bins": 1, "depth": 2}
 
 
-def _options(max_large_steps=None, frozen=None) -> N.MarchOptions:
+def _options(max_large_steps=None, frozen=None, render_mode=None, debug_render_bin=None, max_depth=None) -> N.MarchOptions:
     o = N.MarchOptions()
+    mode = DEFAULT_OPTIONS["render_mode"] if render_mode is None else render_mode
+    o.render_mode = int(RENDER_MODES.get(mode, mode) if isinstance(mode, str) else mode)
+    o.debug_render_bin = int(DEFAULT_OPTIONS["debug_render_bin"] if debug_render_bin is None else debug_render_bin)
+    o.max_depth = float(DEFAULT_OPTIONS["max_depth"] if max_depth is None else max_depth)
     o.max_large_steps = int(DEFAULT_OPTIONS["max_large_steps"] if max_large_steps is None else max_large_steps)
     fr = DEFAULT_OPTIONS["frozen"] if frozen is None else frozen
     fr = list(fr or [])
@@ -46,7 +54,8 @@ def _options(max_large_steps=None, frozen=None) -> N.MarchOptions:
 
 def _forward(sh_params, weights, centroids, cov_scales, cov_rotations, view_matrix, proj_matrix, cam_position, background,
              render_width: int, render_height: int, tan_fovx: float, tan_fovy: float, sh_degree: int,
-             tile=None, batch: int = 0, max_large_steps=None, frozen=None, scale_multiplier: float = 1.0):
+             tile=None, batch: int = 0, max_large_steps=None, frozen=None, scale_multiplier: float = 1.0, render_mode=None,
+             debug_render_bin=None, max_depth=None):
     tile = tile or DEFAULT_TILING[0]
     batch = batch or DEFAULT_TILING[1]
     sh_params, weights, centroids, cov_scales, cov_rotations = C.prepare_inputs(sh_params, weights, centroids, cov_scales, cov_rotations,
@@ -59,7 +68,7 @@ def _forward(sh_params, weights, centroids, cov_scales, cov_rotations, view_matr
     tiling = C.tiling_struct(tile, batch)
     cam = C.camera_struct(view_matrix, proj_matrix, cam_position, background, W, H, tan_fovx, tan_fovy)
     g = C.gaussians_struct(sh_params, weights, centroids, cov_scales, cov_rotations, sh_degree, scale_multiplier)
-    opt = _options(max_large_steps, frozen)
+    opt = _options(max_large_steps, frozen, render_mode, debug_render_bin, max_depth)
     gx, gy = (W + tiling.tile_w - 1) // tiling.tile_w, (H + tiling.tile_h - 1) // tiling.tile_h
     fo = dict(dtype=dt, device=dev)
     io = dict(dtype=torch.int32, device=dev)

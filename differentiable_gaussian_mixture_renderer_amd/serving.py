@@ -64,13 +64,22 @@ class ResidentScene:
         return self.params[2].shape[0]
 
 
+def resident_sizes(count: int, n_min: int = 50_000, n_max: int = 3_000_000) -> List[int]:
+    """Gaussian counts of `count` resident scenes spread log-uniformly over
+    [n_min, n_max]: the quantiles k / (count - 1) of the log-uniform
+    distribution, so the smallest scene has n_min Gaussians and the largest
+    n_max."""
+    if count == 1:
+        return [int(round(math.sqrt(n_min * n_max)))]
+    lo, hi = math.log(n_min), math.log(n_max)
+    return [int(round(math.exp(lo + k * (hi - lo) / (count - 1)))) for k in range(count)]
+
+
 def make_resident_scenes(count: int, seed: int, n_min: int = 50_000, n_max: int = 3_000_000, device="cuda:0") -> List[ResidentScene]:
-    """Scenes with Gaussian counts log-uniform in [n_min, n_max] (the 1M-Gaussian training-frame distributions)."""
-    st = scenes.Stream(seed, 11)
-    u = st.uniform(count)
+    """Scenes whose Gaussian counts span [n_min, n_max] log-uniformly
+    (resident_sizes), each with the 1M-Gaussian training-frame distributions."""
     out = []
-    for k in range(count):
-        n = int(round(math.exp(math.log(n_min) + u[k] * (math.log(n_max) - math.log(n_min)))))
+    for k, n in enumerate(resident_sizes(count, n_min, n_max)):
         sc = scenes.make_scene(n, seed=1000 * seed + k, extent=5.0, sh_degree=3)
         t = lambda a: torch.tensor(np.ascontiguousarray(a), device=device)
         out.append(ResidentScene([t(sc.sh), t(sc.opacity), t(sc.means), t(sc.scales), t(sc.rotations)], sc.sh_degree))
@@ -202,33 +211,38 @@ class ContinuousBatchingServer:
                 if not self._work and self._stop:
                     return
                 scene_id, batch, cost = self._work.popleft()
-            # gradient requests first, one gradient group each; forward-only views after them
-            batch = [r for r in batch if r.backward] + [r for r in batch if not r.backward]
-            n_bwd = sum(r.backward for r in batch)
-            try:
-                with torch.cuda.stream(stream):
-                    sc = self.resident[scene_id]
-                    if len(batch) == 1:
-                        # a lone request (low load): the one-camera path has less per-launch setup
-                        imgs, grads = self._single(sc, batch[0])
-                    else:
-                        imgs, grads = self._batched(splatter, sc, batch, n_bwd)
-                    stream.synchronize()
-            except BaseException:
-                with self._lock:
-                    self._in_flight -= cost  # the failed batch's admission is released
-                raise
-            t = self.now()
+            self._run_batch(stream, splatter, scene_id, batch, cost)
+
+    def _run_batch(self, stream, splatter, scene_id, batch, cost):
+        """Renders one admitted batch on `stream`, records its results and
+        latencies, and releases its admission cost (also when it fails)."""
+        # gradient requests first, one gradient group each; forward-only views after them
+        batch = [r for r in batch if r.backward] + [r for r in batch if not r.backward]
+        n_bwd = sum(r.backward for r in batch)
+        try:
+            with torch.cuda.stream(stream):
+                sc = self.resident[scene_id]
+                if len(batch) == 1:
+                    # a lone request (low load): the one-camera path has less per-launch setup
+                    imgs, grads = self._single(sc, batch[0])
+                else:
+                    imgs, grads = self._batched(splatter, sc, batch, n_bwd)
+                stream.synchronize()
+        except BaseException:
             with self._lock:
-                for r, img, g in zip(batch, imgs, grads):
-                    r.done_at = t
-                    r.image = img
-                    r.grads = g
-                    self.stats.latencies.append(t - r.arrival)
-                    self.stats.pixels += r.pixels
-                self.stats.batches += 1
-                self._in_flight -= cost
-                self._lock.notify_all()
+                self._in_flight -= cost  # the failed batch's admission is released
+            raise
+        t = self.now()
+        with self._lock:
+            for r, img, g in zip(batch, imgs, grads):
+                r.done_at = t
+                r.image = img
+                r.grads = g
+                self.stats.latencies.append(t - r.arrival)
+                self.stats.pixels += r.pixels
+            self.stats.batches += 1
+            self._in_flight -= cost
+            self._lock.notify_all()
 
     def _single(self, sc, r):
         from .python_bindings import splatter
@@ -255,9 +269,27 @@ class ContinuousBatchingServer:
     # -------------------------------------------------------------- scheduler
     def _schedule(self):
         """Admits the waiting requests of the scene with the oldest waiting
-        request, as many as fit the budget and the batch limit."""
-        if not self._waiting:
+        request, as many as fit the budget and the batch limit, and queues
+        them for the workers."""
+        taken = self._take_batch()
+        if taken is None:
             return False
+        self._work.append(taken)
+        self._lock.notify_all()
+        return True
+
+    def _one_batch_waiting(self) -> bool:
+        """Every waiting request belongs to one scene and fits one batch."""
+        if not self._waiting or len(self._waiting) > self.max_views:
+            return False
+        s0 = self._waiting[0].scene
+        return all(r.scene == s0 for r in self._waiting)
+
+    def _take_batch(self):
+        """(scene id, requests, admitted bytes) of the next batch, removed
+        from the waiting queue and counted in flight; None if nothing fits."""
+        if not self._waiting:
+            return None
         scene_id = self._waiting[0].scene
         picked, rest = [], deque()
         while self._waiting:
@@ -270,35 +302,56 @@ class ContinuousBatchingServer:
             rest.append(r)
         self._waiting = rest
         if not picked:
-            return False
+            return None
         cost = estimate_bytes(self.resident[scene_id], picked)
         self._in_flight += cost
         self.stats.max_in_flight_bytes = max(self.stats.max_in_flight_bytes, self._in_flight)
-        self._work.append((scene_id, picked, cost))
-        self._lock.notify_all()
-        return True
+        return scene_id, picked, cost
 
     def run(self, trace: List[Request]) -> ServerStats:
+        """Serves the trace: arrivals are admitted on their arrival time.  On
+        an idle server (nothing in flight) with one batch's worth waiting, the
+        batch runs on this thread at once, without the hand-off to a worker,
+        so a lone request at low load waits no longer than on the plain path;
+        otherwise batches go to the worker streams and overlap on the GPU.
+        Re-raises the first error a batch raised."""
         self._t0 = time.perf_counter()
         workers = [threading.Thread(target=self._worker, args=(k,), daemon=True) for k in range(self.n_workers)]
         for w in workers:
             w.start()
+        inline_stream = _worker_stream(self.device, -1)
+        inline_splatter = None
         pending = deque(sorted(trace, key=lambda r: r.arrival))
         total = len(pending)
         while True:
+            inline = None
             with self._lock:
                 if self._error is not None:
                     break
                 t = self.now()
                 while pending and pending[0].arrival <= t:
                     self._waiting.append(pending.popleft())
-                # keep at most one queued batch per idle worker: the rest keep waiting and join later batches
-                while len(self._work) < self.n_workers and self._schedule():
-                    pass
-                if not pending and not self._waiting and not self._work and len(self.stats.latencies) == total:
+                if self._in_flight == 0 and not self._work and self._one_batch_waiting():
+                    inline = self._take_batch()
+                else:
+                    # keep at most one queued batch per idle worker: the rest keep waiting and join later batches
+                    while len(self._work) < self.n_workers and self._schedule():
+                        pass
+                if inline is None:
+                    if not pending and not self._waiting and not self._work and len(self.stats.latencies) == total:
+                        break
+                    timeout = (pending[0].arrival - t) if pending else 0.005
+                    self._lock.wait(timeout=max(0.0002, min(timeout, 0.005)))
+            if inline is not None:
+                if inline_splatter is None and len(inline[1]) > 1:
+                    inline_splatter = BatchSplatter(tile=self.tile, batch=self.lds_batch, device=self.device)
+                try:
+                    self._run_batch(inline_stream, inline_splatter, *inline)
+                except BaseException as exc:  # noqa: BLE001 -- re-raised below, as a worker's error
+                    with self._lock:
+                        if self._error is None:
+                            self._error = exc
                     break
-                timeout = (pending[0].arrival - t) if pending else 0.005
-                self._lock.wait(timeout=max(0.0005, min(timeout, 0.005)))
         with self._lock:
             self._stop = True
             self._lock.notify_all()

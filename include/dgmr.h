@@ -8,7 +8,10 @@
  * outputs, the per-frame cache and scratch come from the caller.  All work is
  * enqueued on the given stream; entry points that must know the number of
  * (tile, Gaussian) pairs read one 4-byte counter back from the device (the
- * reference does the same with .item(), splat_forward.cu:141).
+ * reference does the same with .item(), splat_forward.cu:141) into a pinned
+ * host word and an event that the calling thread takes on its first such
+ * call (256 bytes of pinned host memory) and returns to a free list when it
+ * ends, for the next thread to reuse.
  *
  * Layouts follow the reference's torch tensors (python_bindings/vol_marcher.cu):
  *   means [n,3], scales [n,3], rotations [n,4] (w,x,y,z), opacity [n],
@@ -155,6 +158,16 @@ typedef struct dgmr_march_options {
                                  (DGMR_TORCH_GRAD_CHECK_CONST_SAMPLES, vol_marcher.h:25) */
     double frozen[8][3];      /* start, end, step of each frozen section */
     int32_t bins_per_chunk;   /* float path: 16 or 32 bins kept in registers per pass (0 = 32); results do not depend on it */
+    /* ForwardData::debug_render_mode, debug_render_bin, max_depth
+     * (vol_marcher.h:44-55; the blends of vol_marcher_forward.cu:443-484),
+     * forward only (the backward computes the full mode's gradients):
+     * 0 full; 1 bins: one large step, colour (r m + g m + b m + m) * max_depth
+     * of bin debug_render_bin (-1, as unsigned, selects the last), T = 0;
+     * 2 depth: T multiplies exp(-m) over every bin, colour = the section
+     * list's end() / max_depth (+inf unless the list filled up). */
+    int32_t render_mode;
+    int32_t debug_render_bin;
+    double max_depth;         /* <= 0: 20, the reference's default */
 } dgmr_march_options;
 
 /* Marcher forward: replaces dgmr::vol_marcher::forward<T>

@@ -351,6 +351,9 @@ public:
             c.inverse_filtered_cov3d.data(), c.filtered_masses.data(), c.tiles_touched.as<uint32_t>() };
         dgmr_march_options opt {};
         opt.max_large_steps = max_large_steps;
+        opt.render_mode = int32_t(d.debug_render_mode);
+        opt.debug_render_bin = d.debug_render_bin;
+        opt.max_depth = double(d.max_depth);
         dgmr_stats stats {};
         int64_t cap = capacity_ > 0 ? capacity_ : 4 * nn + 4096;
         for (int attempt = 0;; ++attempt) {

@@ -45,7 +45,7 @@ def lib() -> ctypes.CDLL:
         L.orc_splat_backward.argtypes = [vp, vp, i32, i32, i32, i32] + [vp] * 12
         L.orc_splat_tile_work.argtypes = [vp, vp, vp, vp]
         L.orc_march_get_geometry.argtypes = [vp] * 9
-        L.orc_march_render.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, vp]
+        L.orc_march_render.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, i32, i32, f64, vp, vp, vp, vp]
         L.orc_march_backward.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, i32, vp, vp, vp] + [vp] * 9
         L.orc_time_splat_fwd_bwd.restype = f64
         L.orc_time_splat_fwd_bwd.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32]
@@ -211,13 +211,17 @@ class MarchOracle(_Handle):
         arr = np.ascontiguousarray(frozen, np.float64).reshape(-1, 3)
         return arr.shape[0], arr
 
-    def render(self, bg, crop=None, max_large_steps=0, frozen=None, with_ambiguity=False, float_sampling=False):
+    def render(self, bg, crop=None, max_large_steps=0, frozen=None, with_ambiguity=False, float_sampling=False, render_mode=0,
+               debug_render_bin=-1, max_depth=20.0):
         """(fb [3,H,W], T [H,W], large steps per pixel [H,W]) and, with
         with_ambiguity, a [H,W] uint8 mask of pixels whose discrete decisions
         lie within float32 rounding (meaningful for dbl=True): bit 0 section
         comparisons, 1 step sizes, 2 border snaps, 3 mass thresholds, 4 mass
         in the bins, 5 transmittance.  float_sampling (dbl=True): the bin
-        borders come from the float32 pipeline, everything on them is FP64."""
+        borders come from the float32 pipeline, everything on them is FP64.
+        render_mode 1 (Bins) / 2 (Depth): the reference's debug modes
+        (vol_marcher_forward.cu:443-484) with debug_render_bin and max_depth;
+        Depth needs whole tiles in the crop."""
         x0, y0, x1, y1 = crop or (0, 0, self.W, self.H)
         fb = np.zeros((3, self.H, self.W), self.dt)
         T = np.ones((self.H, self.W), self.dt)
@@ -225,7 +229,8 @@ class MarchOracle(_Handle):
         amb = np.zeros((self.H, self.W), np.uint8)
         nf, fr = self._frozen(frozen)
         bgc = _c(bg, self.dt)
-        lib().orc_march_render(self.h, P(bgc), x0, y0, x1, y1, max_large_steps, nf, P(fr), int(float_sampling), P(fb), P(T), P(nl),
+        lib().orc_march_render(self.h, P(bgc), x0, y0, x1, y1, max_large_steps, nf, P(fr), int(float_sampling), int(render_mode),
+                               int(debug_render_bin), float(max_depth), P(fb), P(T), P(nl),
                                P(amb) if with_ambiguity else None)
         return (fb, T, nl, amb) if with_ambiguity else (fb, T, nl)
 

@@ -342,13 +342,17 @@ void orc_march_get_geometry(void* p, int32_t* rect, uint32_t* tiles, void* xy, v
 }
 
 void orc_march_render(void* p, const void* bg, int x0, int y0, int x1, int y1, int max_large_steps, int n_frozen,
-    const double* frozen, int float_sampling, void* fb, void* T_final, int32_t* n_large, uint8_t* ambiguous)
+    const double* frozen, int float_sampling, int render_mode, int debug_bin, double max_depth, void* fb, void* T_final, int32_t* n_large,
+    uint8_t* ambiguous)
 {
     auto* h = static_cast<MarchHandle*>(p);
     MarchOptions opt;
     opt.max_large_steps = max_large_steps;
     opt.n_frozen = n_frozen;
     opt.frozen = frozen;
+    opt.mode = static_cast<RenderMode>(render_mode);
+    opt.debug_bin = debug_bin;
+    opt.max_depth = max_depth;
     float ipf[16];
     for (int i = 0; i < 16; ++i)
         ipf[i] = float(h->inv_proj[i]);
@@ -365,6 +369,7 @@ void orc_march_render(void* p, const void* bg, int x0, int y0, int x1, int y1, i
         const S* bgs = static_cast<const S*>(bg);
         S* out = static_cast<S*>(fb);
         S* Tout = static_cast<S*>(T_final);
+        std::vector<int32_t> steps(plane, 0);
 #pragma omp parallel for schedule(dynamic, 1)
         for (int py = y0; py < y1; ++py)
             for (int px = x0; px < x1; ++px) {
@@ -380,9 +385,37 @@ void orc_march_render(void* p, const void* bg, int x0, int y0, int x1, int y1, i
                 for (int c = 0; c < 3; ++c)
                     out[c * plane + q] = C[c] + T * bgs[c];
                 Tout[q] = T;
+                steps[q] = nl;
                 if (n_large)
                     n_large[q] = nl;
             }
+        if (opt.mode == RenderMode::Depth) {
+            // A pixel that has stopped keeps taking part in its tile's large
+            // steps (vol_marcher_forward.cu:340-499 loops per block): pass 1
+            // skips it, so its section list is empty, its end() +inf, and
+            // the depth colour becomes +inf / max_depth; its bins stay
+            // empty, so T is unchanged.  Tiles are whole in the crop only
+            // when the crop is tile-aligned; steps outside the crop are not
+            // known, so the rule is applied over the crop's part of a tile.
+            const int tw = h->bins.tile_w, th = h->bins.tile_h;
+            for (int ty = y0 / th; ty * th < y1; ++ty)
+                for (int tx = x0 / tw; tx * tw < x1; ++tx) {
+                    const int ya = std::max(y0, ty * th), yb = std::min(y1, (ty + 1) * th);
+                    const int xa = std::max(x0, tx * tw), xb = std::min(x1, (tx + 1) * tw);
+                    int most = 0;
+                    for (int py = ya; py < yb; ++py)
+                        for (int px = xa; px < xb; ++px)
+                            most = std::max(most, steps[size_t(py) * W + px]);
+                    const S far = std::numeric_limits<S>::infinity() / S(opt.max_depth);
+                    for (int py = ya; py < yb; ++py)
+                        for (int px = xa; px < xb; ++px) {
+                            const size_t q = size_t(py) * W + px;
+                            if (steps[q] < most)
+                                for (int c = 0; c < 3; ++c)
+                                    out[c * plane + q] = far + Tout[q] * bgs[c];
+                        }
+                }
+        }
     };
     if (h->dbl)
         go(h->sd, h->gd, static_cast<double*>(nullptr));

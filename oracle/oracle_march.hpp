@@ -596,10 +596,16 @@ inline void pixel_ray(const Camera<S>& cam, const S* inv_proj, int px, int py, S
         dir[i] = p[i] * inv_len;
 }
 
+// ForwardData::RenderMode (vol_marcher.h:48-53)
+enum class RenderMode { Full = 0, Bins = 1, Depth = 2 };
+
 struct MarchOptions {
     int max_large_steps = 0;              // 0: unbounded, as the reference
     int n_frozen = 0;                     // >0: fixed sample sections (gradient-check mode)
     const double* frozen = nullptr;       // [n_frozen][3] start, end, dt
+    RenderMode mode = RenderMode::Full;   // debug_render_mode (forward only)
+    int debug_bin = -1;                   // debug_render_bin; -1 (as unsigned) selects the last bin
+    double max_depth = 20.0;              // ForwardData::max_depth
 };
 
 // pass 1 of one large step: build the sample sections from the tile's Gaussians
@@ -656,11 +662,13 @@ struct FloatSampling {
 
 template <class S>
 inline unsigned large_step_borders(const MarchGeometry<S>& g, const Binning& b, const Scene<S>& sc, int tile, const S* o, const S* d,
-    const MarchOptions& opt, S start, S* borders)
+    const MarchOptions& opt, S start, S* borders, S* end = nullptr)
 {
     SectionList<S, MarchConfig::n_large_steps> sec(start);
     build_sections(g, b, tile, o, d, sc, opt, &sec);
     sample_borders(sec, borders, MarchConfig::n_small_steps);
+    if (end)
+        *end = sec.end();
     return sec.size();
 }
 
@@ -677,17 +685,20 @@ inline void float_ray(const FloatSampling& fs, int px, int py, S* o, S* d)
 
 template <class S>
 inline unsigned large_step_borders(const FloatSampling& fs, const Binning& b, int tile, int px, int py, const MarchOptions& opt, S start,
-    S* borders)
+    S* borders, S* end = nullptr)
 {
     float d[3], fb[MarchConfig::n_small_steps];
     pixel_ray(*fs.cam, fs.inv_proj, px, py, d);
     // the section and border decisions are the float32 run's own: a float32
     // march that computes them in the same order takes the same ones
     tie_ignore = kTieSection | kTieStep | kTieBorder;
-    const unsigned n = large_step_borders(*fs.g, b, *fs.sc, tile, fs.cam->pos, d, opt, float(start), fb);
+    float fend = 0;
+    const unsigned n = large_step_borders(*fs.g, b, *fs.sc, tile, fs.cam->pos, d, opt, float(start), fb, &fend);
     tie_ignore = 0;
     for (unsigned i = 0; i < MarchConfig::n_small_steps; ++i)
         borders[i] = S(fb[i]);
+    if (end)
+        *end = S(fend);
     return n;
 }
 
@@ -706,19 +717,51 @@ inline void march_pixel(const MarchGeometry<S>& g, const Binning& b, const Scene
     S T = 1;
     C[0] = C[1] = C[2] = 0;
     S start = 0;
+    S distance = 0;
     int steps = 0;
     std::vector<S> borders(MarchConfig::n_small_steps);
     std::vector<S> binsv(4 * kBins);
     S (*bins)[4] = reinterpret_cast<S (*)[4]>(binsv.data());
     while (true) {
-        const unsigned n_sections = fs ? large_step_borders(*fs, b, tile, px, py, opt, start, borders.data())
-                                       : large_step_borders(g, b, sc, tile, o, d, opt, start, borders.data());
+        S section_end = 0;
+        const unsigned n_sections = fs ? large_step_borders(*fs, b, tile, px, py, opt, start, borders.data(), &section_end)
+                                       : large_step_borders(g, b, sc, tile, o, d, opt, start, borders.data(), &section_end);
         std::fill(binsv.begin(), binsv.end(), S(0));
         for (uint32_t k = b.ranges[2 * tile]; k < b.ranges[2 * tile + 1]; ++k) {
             const uint32_t id = b.point_list[k];
             sample_gaussian(g.mass[id], &g.rgb[3 * id], sc.means + 3 * id, &g.inv_cov[6 * id], o, d, borders.data(), bins);
         }
-        integrate_bins(C, &T, bins);
+        // the blend of vol_marcher_forward.cu:443-484 by debug_render_mode
+        if (opt.mode == RenderMode::Bins) {
+            const unsigned bin = std::min(unsigned(opt.debug_bin), kBins - 1);
+            const S mass = ((bins[bin][0] + bins[bin][1]) + bins[bin][2]) + bins[bin][3];
+            const S v = mass * S(opt.max_depth);
+            C[0] = C[1] = C[2] = v;
+            if (mass == 0) {
+                C[0] = 0;
+                C[1] = 1;
+                C[2] = 0;
+            }
+            if (std::isnan(mass)) {
+                C[0] = 1;
+                C[1] = 0;
+                C[2] = S(0.5);
+            }
+            if (mass < 0) {
+                C[0] = 1;
+                C[1] = 0;
+                C[2] = 0;
+            }
+            T = 0;
+        } else if (opt.mode == RenderMode::Depth) {
+            for (unsigned k = 0; k < kBins; ++k)
+                T *= std::exp(-bins[k][3]);
+            // the section list's end: +inf unless the list filled up
+            distance = std::max(section_end, distance);
+            C[0] = C[1] = C[2] = distance / S(opt.max_depth);
+        } else {
+            integrate_bins(C, &T, bins);
+        }
         ++steps;
         if (n_sections == 0 || T < S(1.f / 255.f))
             break;

@@ -324,3 +324,34 @@ def test_compacted_binning_equals_uncompacted_subset():
     assert torch.equal(full[10], sub[10])                                    # tile ranges
     assert full[11].numel() == sub[11].numel() > 0
     np.testing.assert_array_equal(full[11].numpy(), keep[sub[11].numpy()].astype(np.int32))
+
+
+@pytest.mark.parametrize("mode,bin_", [(1, 0), (1, 40), (1, -1), (2, -1)])
+def test_debug_render_modes(mode, bin_):
+    """The reference's debug modes (vol_marcher_forward.cu:443-484): Bins shows
+    one bin's (r m + g m + b m + m) * max_depth (green when empty) with T = 0;
+    Depth multiplies exp(-m) over every bin and shows end() / max_depth (+inf
+    until the section list is full, and for pixels whose tile marches on after
+    they stopped).  Double GPU against the double oracle: equal infinities,
+    finite values within 1e-9 relative."""
+    torch = _torch()
+    from differentiable_gaussian_mixture_renderer_amd.python_bindings import vol_marcher as vm
+    scene, cam = small_scene()
+    bg = np.array([0.2, 0.3, 0.4])
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev, dtype=torch.float64)
+    args = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations), t(cam.world_view_transform),
+            t(cam.full_proj_transform), t(cam.camera_center), t(bg), cam.width, cam.height, cam.tan_fovx, cam.tan_fovy, scene.sh_degree]
+    print(f"[gpu march] debug mode {mode} bin {bin_}", flush=True)
+    cache = vm.cpp_binding.forward(*args, render_mode=mode, debug_render_bin=bin_, max_depth=15.0)
+    fb, T = cache[0].cpu().numpy(), cache[12].cpu().numpy()
+    ref = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=False)
+    fb64, T64, _ = ref.render(bg, render_mode=mode, debug_render_bin=bin_, max_depth=15.0)
+    assert np.array_equal(np.isinf(fb), np.isinf(fb64))
+    fin = np.isfinite(fb64)
+    scale = max(1.0, float(np.abs(fb64[fin]).max())) if fin.any() else 1.0
+    assert np.abs(fb[fin] - fb64[fin]).max(initial=0.0) <= 1e-9 * scale
+    assert np.abs(T - T64).max() <= 1e-9
+    if mode == 1:
+        assert (T == 0).all()
+        assert fin.all() and (fb64 != 0).any()

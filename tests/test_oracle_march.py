@@ -90,3 +90,46 @@ def test_backward_pads_missing_sh_coefficients_with_ones():
         changed += bool(np.any(gdir != 0))
     assert changed > 10
     np.testing.assert_allclose(quirk["means"], want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+
+
+def test_debug_render_modes_on_one_gaussian():
+    """The reference's debug modes (vol_marcher_forward.cu:443-484) on one
+    Gaussian seen through fixed sample sections [4, 8] (126 bins and one of
+    zero width): Bins mode shows, per bin, (r m + g m + b m + m) * max_depth
+    with T = 0, and green for an empty bin; Depth mode multiplies exp(-m) over
+    every bin and colours the pixel with end() / max_depth, +inf while the
+    section list is not full.  So the Bins-mode values of all bins add up to
+    (1 + r + g + b) * max_depth * (-ln T_depth), which ties both modes to the
+    same bins."""
+    from differentiable_gaussian_mixture_renderer_amd import scenes
+    scene = scenes.make_scene(1, seed=1, extent=1.0, sh_mode="rgb")
+    scene.means[:] = 0.0
+    scene.scales[:] = 0.3
+    scene.rotations[:] = [1.0, 0.0, 0.0, 0.0]
+    scene.opacity[:] = 0.8
+    cam = scenes.look_at_camera([0.0, 0.0, -6.0], [0.0, 0.0, 0.0], 33, 33, 0.3, 0.3)
+    ref = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=False)
+    frozen = [(4.0, 8.0, 4.0 / 126)]
+    bg = np.array([0.1, 0.2, 0.3])
+    px = (16, 16)
+    total, filled = 0.0, 0
+    for k in range(127):
+        fb, T, _ = ref.render(bg, frozen=frozen, render_mode=1, debug_render_bin=k, max_depth=20.0)
+        c = fb[:, px[1], px[0]]
+        assert T[px[1], px[0]] == 0.0
+        if k == 126 or np.array_equal(c, [0.0, 1.0, 0.0]):
+            continue   # green: an empty bin (the zero-width last one, and tail bins under the 1e-5 cut)
+        assert c[0] == c[1] == c[2] and c[0] > 0
+        total += c[0]
+        filled += 1
+    assert filled >= 60
+    fb_126, _, _ = ref.render(bg, frozen=frozen, render_mode=1, debug_render_bin=126, max_depth=20.0)
+    np.testing.assert_array_equal(fb_126[:, px[1], px[0]], [0.0, 1.0, 0.0])   # the zero-width last bin
+    fb_last, _, _ = ref.render(bg, frozen=frozen, render_mode=1, debug_render_bin=-1, max_depth=20.0)
+    np.testing.assert_array_equal(fb_last[:, px[1], px[0]], [0.0, 1.0, 0.0])   # -1 selects the last bin
+    fb_d, T_d, _ = ref.render(bg, frozen=frozen, render_mode=2, max_depth=20.0)
+    assert np.isinf(fb_d[:, px[1], px[0]]).all()
+    rgb = ref.geometry()["rgb"][0]
+    np.testing.assert_allclose(total, (1.0 + rgb.sum()) * 20.0 * -np.log(T_d[px[1], px[0]]), rtol=1e-12)
+    fb_full, T_full, _ = ref.render(bg, frozen=frozen)
+    assert T_full[px[1], px[0]] >= T_d[px[1], px[0]]   # Full stops at 1/255, Depth multiplies every bin

@@ -73,3 +73,42 @@ def test_all_batches_complete_without_failures(host_only, monkeypatch):
     assert len(stats.latencies) == len(trace)
     assert srv._in_flight == 0
     assert np.all(np.array(stats.latencies) >= 0)
+
+
+def test_idle_server_runs_a_lone_request_without_a_hand_off(host_only, monkeypatch):
+    """Spaced arrivals (each finds the server idle) run on the scheduling
+    thread itself; a burst over several scenes goes to the worker threads."""
+    threads = []
+
+    def single(self, sc, r):
+        threads.append(threading.current_thread())
+        return [None], [None]
+
+    def batched(sp, sc, batch, n_bwd):
+        threads.append(threading.current_thread())
+        return ([None] * len(batch),) * 2
+
+    monkeypatch.setattr(serving.ContinuousBatchingServer, "_single", single)
+    monkeypatch.setattr(serving.ContinuousBatchingServer, "_batched", staticmethod(batched))
+    resident = [serving.ResidentScene([torch.zeros(10, 1, 3), torch.zeros(10, 1), torch.zeros(10, 3), torch.zeros(10, 3),
+                                       torch.zeros(10, 4)], 0) for _ in range(3)]
+    spaced = serving.make_trace(6, rate_hz=50.0, n_scenes=3, seed=2, min_res=(16, 16), max_res=(32, 32))
+    srv = serving.ContinuousBatchingServer(resident, hbm_budget_bytes=1 << 30, workers=2, device="cpu")
+    srv.run(spaced)
+    assert threads and all(t is threading.current_thread() for t in threads)
+    threads.clear()
+    burst = serving.make_trace(24, rate_hz=50.0, n_scenes=3, seed=3, min_res=(16, 16), max_res=(32, 32))
+    for r in burst:
+        r.arrival = 0.0
+    srv = serving.ContinuousBatchingServer(resident, hbm_budget_bytes=1 << 30, workers=2, device="cpu")
+    stats = srv.run(burst)
+    assert len(stats.latencies) == len(burst)
+    assert any(t is not threading.current_thread() for t in threads)
+
+
+def test_resident_sizes_span_the_range():
+    sizes = serving.resident_sizes(8)
+    assert sizes[0] == 50_000 and sizes[-1] == 3_000_000
+    assert all(a < b for a, b in zip(sizes, sizes[1:]))
+    ratios = [b / a for a, b in zip(sizes, sizes[1:])]
+    assert max(ratios) / min(ratios) < 1.001   # evenly spaced in log
