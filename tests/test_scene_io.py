@@ -49,3 +49,54 @@ def test_cameras_json_matches_look_at():
     np.testing.assert_allclose(cam.world_view_transform, ref.world_view_transform, atol=2e-6)
     np.testing.assert_allclose(cam.full_proj_transform, ref.full_proj_transform, atol=2e-6)
     assert abs(cam.tan_fovx - tx) < 1e-12 and abs(cam.tan_fovy - ty) < 1e-12
+
+
+def _fixture_values():
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "make_inria_ply.py")
+    spec = importlib.util.spec_from_file_location("make_inria_ply", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.values(np.random.default_rng(1234)), os.path.dirname(path)
+
+
+def test_inria_fixture_loads_in_the_trainer_layout():
+    """tests/golden/inria_scene.ply is written byte by byte in the trainer's
+    property order (make_inria_ply.py, not save_ply): every loaded field
+    equals the recipe's value, f_rest read channel-major."""
+    import os
+    v, d = _fixture_values()
+    ply = scene_io.load_ply(os.path.join(d, "inria_scene.ply"))
+    assert ply.n == 300 and ply.sh_degree == 3 and ply.sh.shape == (300, 16, 3)
+    f = lambda a: np.asarray(a, np.float32)
+    np.testing.assert_array_equal(ply.means, np.stack([f(v["x"]), f(v["y"]), f(v["z"])], 1))
+    for c in range(3):
+        np.testing.assert_array_equal(ply.sh[:, 0, c], f(v[f"f_dc_{c}"]))
+        for k in range(1, 16):
+            np.testing.assert_array_equal(ply.sh[:, k, c], f(v[f"f_rest_{15 * c + k - 1}"]))
+    np.testing.assert_array_equal(ply.opacity_logits[:, 0], f(v["opacity"]))
+    np.testing.assert_array_equal(ply.log_scales, np.stack([f(v[f"scale_{i}"]) for i in range(3)], 1))
+    np.testing.assert_array_equal(ply.rotations, np.stack([f(v[f"rot_{i}"]) for i in range(4)], 1))
+    sh, op, mu, s, q = ply.activated()
+    np.testing.assert_allclose(np.linalg.norm(q, axis=1), 1.0, rtol=1e-6)
+    np.testing.assert_allclose(op[:, 0], 1.0 / (1.0 + np.exp(-f(v["opacity"]).astype(np.float64))), rtol=1e-6)
+
+
+def test_inria_cameras_fixture():
+    """cameras.json entries look at the origin: the world-to-camera transform
+    puts the origin on the +z axis at the camera's distance, and the
+    projection maps it to the image centre (NDC 0, 0)."""
+    import os
+    _, d = _fixture_values()
+    cams = scene_io.load_cameras_json(os.path.join(d, "inria_cameras.json"))
+    assert len(cams) == 2
+    for cam in cams:
+        V = cam.world_view_transform.T.astype(np.float64)
+        P = cam.full_proj_transform.T.astype(np.float64)
+        o = V @ np.array([0.0, 0.0, 0.0, 1.0])
+        np.testing.assert_allclose(o[:2], 0.0, atol=1e-5)
+        assert abs(o[2] - np.linalg.norm(cam.camera_center)) < 1e-5
+        clip = P @ np.array([0.0, 0.0, 0.0, 1.0])
+        np.testing.assert_allclose(clip[:2] / clip[3], 0.0, atol=1e-6)
+        assert cam.width in (160, 128)
