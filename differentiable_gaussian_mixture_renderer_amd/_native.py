@@ -180,6 +180,8 @@ SIGNATURES = {
     "dgmr_sort_workspace_size": (ctypes.c_int, [ctypes.c_int64, _P(ctypes.c_size_t)]),
     "dgmr_sort_pairs": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, c_void_p,
                                        ctypes.c_size_t, c_void_p]),
+    "dgmr_sort_pairs_noalloc": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                               c_void_p]),
     "dgmr_selftest_wave_sum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
 

@@ -11,7 +11,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["abi_common.hip", "primitives.hip", "splat.hip", "march.hip", "batch.hip", "optim.hip", "raster.hip", "loss.hip"]
+SOURCES = ["abi_common.hip", "primitives.hip", "splat.hip", "march.hip", "batch.hip", "optim.hip", "raster.hip", "loss.hip", "noalloc.hip"]
 OUT = os.path.join(os.path.dirname(HERE), "_lib", "libdgmr_hip.so")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -5,7 +5,8 @@
  *
  * Every pointer in a descriptor is a device pointer unless stated otherwise;
  * camera matrices are host values.  No entry point allocates device memory:
- * outputs, the per-frame cache and scratch come from the caller.  All work is
+ * outputs, the per-frame cache and scratch come from the caller (the scratch
+ * of dgmr_sort_pairs_noalloc is a library-owned array).  All work is
  * enqueued on the given stream; entry points that must know the number of
  * (tile, Gaussian) pairs read one 4-byte counter back from the device (the
  * reference does the same with .item(), splat_forward.cu:141) into a pinned
@@ -323,6 +324,21 @@ int dgmr_l1_loss_backward_many(const dgmr_l1_item* items, int32_t n_items, dgmr_
 int dgmr_sort_workspace_size(int64_t n, size_t* bytes);
 int dgmr_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, int64_t n, int32_t begin_bit, int32_t end_bit,
     void* ws, size_t ws_bytes, dgmr_stream stream);
+
+/* No counterpart in the reference: dgmr_sort_pairs for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same sort, with the scratch that
+ * dgmr_sort_pairs takes as ws kept in a library-owned device array of
+ * 16 MiB + 64 KiB per GPU (allocated with the library's code object and held
+ * until the process ends); the caller still passes the keys, values and the
+ * ping-pong halves.  n above DGMR_SORT_NOALLOC_MAX_ITEMS (2^25) is rejected
+ * with DGMR_ERR_ARGUMENT before any device work.  The entry point calls no
+ * allocating routine.  Every call on one GPU uses that GPU's one array:
+ * calls in stream order are safe, calls that overlap on one GPU (other
+ * streams, other threads) race. */
+#define DGMR_SORT_NOALLOC_MAX_ITEMS 33554432
+int dgmr_sort_pairs_noalloc(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, int64_t n, int32_t begin_bit,
+    int32_t end_bit, dgmr_stream stream);
 
 /* ------------------------------------------------------------ self-test
  * Runs the blend backward's wave reductions on one wave: in holds 9 x 64
