@@ -5,6 +5,7 @@
 
 #include "device_grad.h"
 #include "device_math.h"
+#include "tile_reach.h"
 
 namespace dgmr {
 
@@ -370,6 +371,12 @@ __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr int kBlendWaves = 4; // tile waves per workgroup
 
+// conic and opacity co = (a, b, c, o) at screen centre xy (tile_reach.h)
+__device__ __forceinline__ bool reaches_box(float2 xy, float4 co, float bx0, float bx1, float by0, float by1)
+{
+    return reaches_box(xy.x, xy.y, co.x, co.y, co.z, co.w, bx0, bx1, by0, by1);
+}
+
 template <int B>
 struct BlendSmem {
     float4 geo[B];  // x, y, A, B
@@ -418,19 +425,53 @@ struct BlendFetch {
         }
     }
 
-    // slot s of this lane into LDS, conic pre-scaled
-    __device__ __forceinline__ void stage(float4* geo, float4* col, float* blue, int s) const
+    // slot s of this lane into LDS slot i, conic pre-scaled
+    __device__ __forceinline__ void stage_at(float4* geo, float4* col, float* blue, int s, int i) const
     {
-        const int i = int(lane_id()) + 64 * s;
         geo[i] = make_float4(xy[s].x, xy[s].y, (-0.5f * kLog2e) * co[s].x, -kLog2e * co[s].y);
         col[i] = make_float4((-0.5f * kLog2e) * co[s].z, co[s].w, r[s], g[s]);
         blue[i] = b[s];
+    }
+
+    // Stages, in list order, only the batch's Gaussians whose alpha can reach
+    // 1/255 at a pixel of the tile box (reaches_box); the others would blend
+    // nowhere in the tile, so skipping them changes no value.  gid (may be
+    // null) receives the staged Gaussians' ids.  Returns the staged count.
+    __device__ __forceinline__ int stage_reaching(float4* geo, float4* col, float* blue, uint32_t* gid, uint32_t base, uint32_t last,
+        float bx0, float bx1, float by0, float by1) const
+    {
+        const uint64_t below = (uint64_t(1) << lane_id()) - 1;
+        int cnt = 0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const bool keep = base + lane_id() + 64u * s < last && reaches_box(xy[s], co[s], bx0, bx1, by0, by1);
+            const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
+            if (keep) {
+                const int i = cnt + __builtin_popcountll(m & below);
+                stage_at(geo, col, blue, s, i);
+                if (gid)
+                    gid[i] = id[s];
+            }
+            cnt += __builtin_popcountll(m);
+        }
+        return cnt;
     }
 };
 
 // LDS of one wave is private to it: LDS operations of a wave execute in
 // order, so a compiler barrier orders its staging writes before its reads
 __device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// true when the predicate holds on some lane of the wave: the ballot of the
+// condition itself, so the lane mask is used as it is (a ballot of an int
+// re-materialises the mask through a VGPR first)
+__device__ __forceinline__ bool wave_any(bool pred) { return __builtin_amdgcn_ballot_w64(pred) != 0; }
+
+// lane masks of float comparisons (LLVM predicates), combined with scalar
+// instructions where a wave-uniform decision needs them
+enum : int { kCmpOGT = 2, kCmpOGE = 3, kCmpOLT = 4, kCmpUGE = 11 };
+template <int PRED>
+__device__ __forceinline__ uint64_t lanes_where(float a, float b) { return __builtin_amdgcn_fcmpf(a, b, PRED); }
 
 // per-Gaussian terms shared by a lane's pixels (dx is common to them)
 struct LaneTerms {
@@ -461,13 +502,14 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
     const int px = px0 + lane % TW;
     const float pxf = float(px);
     const float pyf0 = float(py0 + lane / TW);
+    const float bx0 = float(px0), bx1 = float(px0 + TW - 1), by0 = float(py0), by1 = float(py0 + TH - 1);
+    // a pixel that has stopped carries its transmittance negated (a live
+    // pixel's T is at least 1e-4), so liveness costs no separate register
     float T[NP], C0[NP], C1[NP], C2[NP];
-    bool live[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int py = py0 + lane / TW + RS * k;
-        live[k] = px < width && py < height;
-        T[k] = 1.f;
+        T[k] = (px < width && py < height) ? 1.f : -1.f;
         C0[k] = C1[k] = C2[k] = 0.f;
     }
     if (first < last) {
@@ -479,18 +521,15 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
             bool any = false;
 #pragma unroll
             for (int k = 0; k < NP; ++k)
-                any |= live[k];
-            if (__ballot(any) == 0)
+                any |= T[k] > 0.f;
+            if (!wave_any(any))
                 break;
             wave_lds_sync(); // the previous batch has been read
-#pragma unroll
-            for (int s = 0; s < BlendFetch<B>::NS; ++s)
-                f.stage(sm.geo, sm.col, sm.blue, s);
+            const int cnt = f.stage_reaching(sm.geo, sm.col, sm.blue, nullptr, base, last, bx0, bx1, by0, by1);
             wave_lds_sync();
             // unconditional (clamped) so the loaded registers need no merge after a branch
             f.load_data(xy, conic_opacity, rgb);
             f.load_ids(base + 2 * B, last, point_list);
-            const int cnt = int(min<uint32_t>(B, last - base));
             for (int j = 0; j < cnt; ++j) {
                 const float4 g = sm.geo[j];
                 const float4 c = sm.col[j];
@@ -502,7 +541,7 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
                 for (int k = 0; k < NP; ++k) {
                     const float alpha = fminf(0.99f, c.y * gauss_at(lt, c.x, pyf0 + float(RS * k) - g.y));
                     const float test_T = T[k] * (1.f - alpha);
-                    const bool blend = live[k] && alpha >= 1.0f / 255.0f;
+                    const bool blend = T[k] > 0.f && alpha >= 1.0f / 255.0f;
                     const bool stop = blend && test_T < 0.0001f;
                     const bool use = blend && !stop;
                     // one select on the weight: fma(c, 0, C) is C exactly (colours are finite, >= 0)
@@ -510,11 +549,10 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
                     C0[k] = fmaf(c.z, wgt, C0[k]);
                     C1[k] = fmaf(c.w, wgt, C1[k]);
                     C2[k] = fmaf(blue, wgt, C2[k]);
-                    T[k] = use ? test_T : T[k];
-                    live[k] = live[k] && !stop;
-                    still |= live[k];
+                    T[k] = use ? test_T : (stop ? -T[k] : T[k]);
+                    still |= T[k] > 0.f;
                 }
-                if (__ballot(still) == 0)
+                if (!wave_any(still))
                     break;
             }
         }
@@ -525,10 +563,11 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
         const int py = py0 + lane / TW + RS * k;
         if (px < width && py < height) {
             const int64_t p = int64_t(py) * width + px;
-            framebuffer[p] = fmaf(T[k], bg0, C0[k]);
-            framebuffer[plane + p] = fmaf(T[k], bg1, C1[k]);
-            framebuffer[2 * plane + p] = fmaf(T[k], bg2, C2[k]);
-            final_T[p] = T[k];
+            const float Tk = fabsf(T[k]);
+            framebuffer[p] = fmaf(Tk, bg0, C0[k]);
+            framebuffer[plane + p] = fmaf(Tk, bg1, C1[k]);
+            framebuffer[2 * plane + p] = fmaf(Tk, bg2, C2[k]);
+            final_T[p] = Tk;
         }
     }
 }
@@ -579,12 +618,14 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
     const int px = px0 + lane % TW;
     const float pxf = float(px);
     const float pyf0 = float(py0 + lane / TW);
+    const float bx0 = float(px0), bx1 = float(px0 + TW - 1), by0 = float(py0), by1 = float(py0 + TH - 1);
     // D = <dL/dC, S> with S the framebuffer minus what has been composited so
     // far, i.e. the remaining colour plus T_final * background: D carries the
     // remaining-colour term and dL/dT_final * T_final of the alpha adjoint in
     // one scalar per pixel
+    // T[k] = 0 marks a pixel that takes no further part (outside the image,
+    // no gradient, or stopped); a live pixel's T never falls below 1e-4
     float gC0[NP], gC1[NP], gC2[NP], D[NP], T[NP];
-    bool live[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int py = py0 + lane / TW + RS * k;
@@ -606,8 +647,7 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             }
             D[k] = gC0[k] * f0 + gC1[k] * f1 + gC2[k] * f2;
         }
-        T[k] = 1.f;
-        live[k] = inside && !(gC0[k] == 0.f && gC1[k] == 0.f && gC2[k] == 0.f);
+        T[k] = (inside && !(gC0[k] == 0.f && gC1[k] == 0.f && gC2[k] == 0.f)) ? 1.f : 0.f;
     }
     if (first >= last)
         return;
@@ -638,8 +678,8 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
         bool any = false;
 #pragma unroll
         for (int k = 0; k < NP; ++k)
-            any |= live[k];
-        if (__ballot(any) == 0)
+            any |= T[k] > 0.f;
+        if (!wave_any(any))
             break;
         wave_lds_sync(); // the previous batch has been read
         flush();
@@ -647,17 +687,15 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int i = lane + 64 * s;
-            sm.gid[i] = f.id[s];
-            f.stage(sm.geo, sm.col, sm.blue, s);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 sm.acc[q * B + i] = 0.f; // 9 B consecutive words over the lanes and steps
         }
+        cnt = f.stage_reaching(sm.geo, sm.col, sm.blue, sm.gid, base, last, bx0, bx1, by0, by1);
         wave_lds_sync();
         // unconditional (clamped) so the loaded registers need no merge after a branch
         f.load_data(xy, conic_opacity, rgb);
         f.load_ids(base + 2 * B, last, point_list);
-        cnt = int(min<uint32_t>(B, last - base));
         for (int j = 0; j < cnt; ++j) {
             const float4 g = sm.geo[j];
             const float4 c = sm.col[j];
@@ -665,25 +703,27 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
             const LaneTerms lt(pxf, g);
             // cheap part for every pixel: the blend decisions
             float Gk[NP], alphak[NP];
-            bool use[NP];
-            bool any_use = false, still = false;
+            bool use[NP], stop[NP];
+            uint64_t use_lanes[NP]; // use[k] as a lane mask, for the wave-uniform tests
+            uint64_t any_use = 0;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
                 Gk[k] = gauss_at(lt, c.x, pyf0 + float(RS * k) - g.y);
                 alphak[k] = fminf(0.99f, c.y * Gk[k]);
-                const bool blend = live[k] && alphak[k] >= 1.0f / 255.0f;
-                const bool stop = blend && T[k] * (1.f - alphak[k]) < 0.0001f;
-                use[k] = blend && !stop;
-                any_use |= use[k];
-                live[k] = live[k] && !stop;
-                still |= live[k];
+                const float test_T = T[k] * (1.f - alphak[k]);
+                const bool blend = T[k] > 0.f && alphak[k] >= 1.0f / 255.0f;
+                stop[k] = blend && test_T < 0.0001f;
+                use[k] = blend && !stop[k];
+                use_lanes[k] = lanes_where<kCmpOGT>(T[k], 0.f) & lanes_where<kCmpOGE>(alphak[k], 1.0f / 255.0f)
+                    & lanes_where<kCmpUGE>(test_T, 0.0001f);
+                any_use |= use_lanes[k];
             }
-            if (__ballot(any_use) != 0) {
+            if (any_use != 0) {
                 float m0 = 0.f, m1 = 0.f, m2 = 0.f, so = 0.f, sr = 0.f, sg = 0.f, sbl = 0.f;
                 // selects keep the sums exact for pixels that do not blend this Gaussian
 #pragma unroll
                 for (int k = 0; k < NP; ++k) {
-                    if (__ballot(use[k]) == 0)
+                    if (use_lanes[k] == 0)
                         continue; // no pixel of this row group blends the Gaussian
                     // the same operations as the cheap part, so the same values
                     const float dy = pyf0 + float(RS * k) - g.y;
@@ -721,7 +761,14 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                 if (slot8 >= 0)
                     sm.acc[9 * j + slot8] = slot8 == 8 ? t9 : t8;
             }
-            if (__ballot(still) == 0)
+            // a pixel stops before the Gaussian that would take T under 1e-4
+            bool still = false;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                T[k] = stop[k] ? 0.f : T[k];
+                still |= T[k] > 0.f;
+            }
+            if (!wave_any(still))
                 break;
         }
     }
