@@ -158,7 +158,7 @@ typedef struct dgmr_march_options {
     int32_t n_frozen;         /* >0: fixed sample sections for gradient checks
                                  (DGMR_TORCH_GRAD_CHECK_CONST_SAMPLES, vol_marcher.h:25) */
     double frozen[8][3];      /* start, end, step of each frozen section */
-    int32_t bins_per_chunk;   /* float path: 16 or 32 bins kept in registers per pass (0 = 32); results do not depend on it */
+    int32_t bins_per_chunk;   /* float path: 16 or 32 bins kept in registers per pass (0: 32 forward, 16 backward); results do not depend on it */
     /* ForwardData::debug_render_mode, debug_render_bin, max_depth
      * (vol_marcher.h:44-55; the blends of vol_marcher_forward.cu:443-484),
      * forward only (the backward computes the full mode's gradients):

@@ -161,8 +161,14 @@ def test_marcher_frame_finite_differences():
     sections (the reference's gradient-check mode, vol_marcher.h:25): the
     backward against central differences of its own forward, 32 parameters of
     Gaussians seen by a band of 16 rows (dL/dimg random there, zero
-    elsewhere); h = 1e-5 max(1, |theta|); |gpu - fd| <= 1e-4 of the largest
-    |fd| drawn."""
+    elsewhere); h = 1e-6 max(1, |theta|); |gpu - fd| <= 1e-4 of the largest
+    |fd| drawn.  The forward keeps discrete thresholds (a bin under 1e-5 is
+    dropped, a Gaussian under 1/255 mass on the ray is skipped), and at this
+    size some parameter's step can move one of them: a parameter whose
+    quotients at h and h/2 disagree by more than 1e-5 relative (plus 1e-6
+    absolute, well above the quotients' round-off of ~1e-8) had a decision
+    inside its step, and another parameter is drawn instead (at most 8
+    redraws, reported)."""
     torch = _torch()
     from differentiable_gaussian_mixture_renderer_amd import scenes
     from differentiable_gaussian_mixture_renderer_amd.python_bindings import vol_marcher as vm
@@ -188,27 +194,36 @@ def test_marcher_frame_finite_differences():
     g = dict(zip(names, vm.cpp_binding.backward(*[base[k] for k in names], *cam_args, grad_t, cache, frozen=frozen)))
     live = np.nonzero(g["means"].abs().sum(dim=1).cpu().numpy() > 0)[0]
     assert live.size > 100
-    rows = []
-    for r in range(32):
-        name = names[r % 5]
-        i = int(live[rng.integers(live.size)])
-        flat = base[name][i].reshape(-1)
-        j = int(rng.integers(3 if name == "sh" else flat.numel()))
-        v = float(flat[j])
-        h = 1e-5 * max(1.0, abs(v))
+    def quotient(name, i, j, v, h):
         vals = []
         for sgn in (1, -1):
             p = dict(base)
             p[name] = base[name].clone()
             p[name][i].reshape(-1)[j] = v + sgn * h
             vals.append(float((fwd(p)[0] * grad_t).sum()))
-        fd = (vals[0] - vals[1]) / (2 * h)
+        return (vals[0] - vals[1]) / (2 * h)
+
+    rows, redraws = [], 0
+    r = 0
+    while len(rows) < 32:
+        name = names[r % 5]
+        r += 1
+        i = int(live[rng.integers(live.size)])
+        flat = base[name][i].reshape(-1)
+        j = int(rng.integers(3 if name == "sh" else flat.numel()))
+        v = float(flat[j])
+        h = 1e-6 * max(1.0, abs(v))
+        fd, fd2 = quotient(name, i, j, v, h), quotient(name, i, j, v, h / 2)
         an = float(g[name][i].reshape(-1)[j])
+        print(f"  {name}[{i}][{j}]: fd {fd:.8g} (half step {fd2:.8g}) analytic {an:.8g}", flush=True)
+        if abs(fd - fd2) > 1e-5 * max(abs(fd), abs(fd2)) + 1e-6:
+            redraws += 1
+            assert redraws <= 8, "too many parameters with a decision inside their step"
+            continue
         rows.append((name, i, j, fd, an))
-        print(f"  {name}[{i}][{j}]: fd {fd:.8g} analytic {an:.8g}", flush=True)
     scale = max(abs(fd) for *_, fd, _ in rows)
     worst = max(abs(fd - an) for *_, fd, an in rows) / scale
-    print(f"worst |gpu - fd| / max |fd| = {worst:.3g}", flush=True)
+    print(f"worst |gpu - fd| / max |fd| = {worst:.3g} ({redraws} parameters redrawn)", flush=True)
     assert worst <= 1e-4
 
 
