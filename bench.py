@@ -239,13 +239,24 @@ BLEND_FLOP_PER_EVAL = {"splat.blend_forward": 13, "splat.blend_backward": 37}
 VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 
 
+# chip VALU issue capacity: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 FP32
+# instruction per SIMD every 4 cycles (16 lanes; packed FP32 is not used)
+VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
+
+
 def blend_roofline(name, avg_ms, bytes_per_launch, achieved_gbs, traffic, cpu):
     """Roofline of the dominant stage against both ceilings: algorithmic
     HBM bytes per launch / duration against 8 TB/s, and algorithmic FLOP per
     launch (the oracle's count of (pixel, pair) evaluations of the frame times
-    the per-evaluation FLOP) / duration against the FP32 vector peak.  `bound`
-    names the ceiling the kernel is closer to; `traffic` is the measured HBM
-    bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)."""
+    the per-evaluation FLOP) / duration against the FP32 vector peak.
+    `traffic` is the measured HBM bytes per launch (rocprofv3 FETCH_SIZE +
+    WRITE_SIZE, scaled; profiles/traffic_splat_1M_1080p.json) and
+    `valu.issue_utilisation` the measured VALU instructions per launch
+    (rocprofv3 SQ_INSTS_VALU; profiles/sq_splat_1M_1080p.json) over the
+    chip's issue capacity in the launch's duration.  `bound` names the
+    resource the measurements show closer to saturation: the measured HBM
+    traffic fraction against the VALU issue utilisation (the algorithmic
+    fractions when no measurement is at hand)."""
     hbm = {"achieved_gbs": round(achieved_gbs, 2), "peak_gbs": HBM_PEAK_GBS, "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
            "algorithmic_bytes_per_launch": bytes_per_launch}
     if traffic:
@@ -258,7 +269,17 @@ def blend_roofline(name, avg_ms, bytes_per_launch, achieved_gbs, traffic, cpu):
         valu = {"achieved_tflops": round(tflops, 3), "peak_tflops": VALU_PEAK_TFLOPS, "frac": round(tflops / VALU_PEAK_TFLOPS, 4),
                 "evaluations_per_launch": work["evaluations"], "flop_per_evaluation": BLEND_FLOP_PER_EVAL[name],
                 "tile_evaluations_per_launch": work["tile_evaluations"]}
-    if valu is not None and valu["frac"] > hbm["frac"]:
+    sq_path = os.path.join(ROOT, "profiles", "sq_splat_1M_1080p.json")
+    if os.path.exists(sq_path):
+        with open(sq_path) as f:
+            sq = json.load(f).get(name)
+        if sq:
+            valu = valu or {}
+            valu["measured_valu_insts_per_launch"] = sq["sq_insts_valu"]
+            valu["issue_utilisation"] = round(sq["sq_insts_valu"] / (avg_ms * 1e-3) / VALU_WAVE_INSTS_PER_S, 4)
+    hbm_level = hbm.get("measured_traffic_frac", hbm["frac"])
+    valu_level = (valu or {}).get("issue_utilisation", (valu or {}).get("frac", 0.0))
+    if valu and "frac" in valu and valu_level > hbm_level:
         head = {"bound": "valu", "achieved": valu["achieved_tflops"], "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": valu["frac"]}
     else:
         head = {"bound": "hbm", "achieved": hbm["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm["frac"]}
