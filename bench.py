@@ -449,8 +449,12 @@ def marcher_workload(lib, with_cpu: bool, steps: int = 3, max_large_steps: int =
         stages = {k: round(v[0] / max(v[1], 1), 3) for k, v in stage_times(lib).items() if k.startswith("march")}
     finally:
         vol_marcher.DEFAULT_OPTIONS.update(saved)
+    replay = vol_marcher.make_replay(W, H, torch.float32, dev, steps=max_large_steps)
     out = {"workload": f"volumetric marcher fwd+bwd, {scene.n} Gaussians SH3, {W}x{H} rays, <= {max_large_steps * 127} bins per ray, L1",
-           "ms_per_step": round(ms, 3), "mrays_per_s": round(W * H / ms / 1e3, 2), "stages_ms": stages}
+           "ms_per_step": round(ms, 3), "mrays_per_s": round(W * H / ms / 1e3, 2), "stages_ms": stages,
+           "replay_store": None if replay is None else {"large_steps": replay.steps, "bytes": replay.buffer.numel(),
+                                                         "note": "the forward keeps each pixel's bins for the backward"}}
+    del replay
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O

@@ -116,6 +116,9 @@ class MarchOptions(ctypes.Structure):
         ("render_mode", ctypes.c_int32),
         ("debug_render_bin", ctypes.c_int32),
         ("max_depth", ctypes.c_double),
+        ("replay", c_void_p),
+        ("replay_bytes", ctypes.c_size_t),
+        ("replay_steps", ctypes.c_int32),
     ]
 
 
@@ -155,6 +158,8 @@ SIGNATURES = {
                                            c_void_p]),
     "dgmr_march_forward": (ctypes.c_int, [_P(Gaussians), _P(Camera), _P(Tiling), _P(MarchOptions), _P(MarchGeometry), _P(Binning),
                                           c_void_p, c_void_p, c_void_p, ctypes.c_size_t, _P(Stats), c_void_p]),
+    "dgmr_march_replay_size": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _P(Tiling), ctypes.c_int32, ctypes.c_int32,
+                                              _P(ctypes.c_size_t)]),
     "dgmr_march_backward": (ctypes.c_int, [_P(Gaussians), _P(Camera), _P(Tiling), _P(MarchOptions), _P(MarchGeometry), _P(Binning),
                                            ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p]),

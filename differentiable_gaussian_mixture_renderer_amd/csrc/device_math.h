@@ -47,6 +47,9 @@ template <class S>
 __device__ __forceinline__ S dmin(S a, S b) { return b < a ? b : a; }
 template <class S>
 __device__ __forceinline__ S dmax(S a, S b) { return a < b ? b : a; }
+// a b + c rounded once (the library is built without implicit contraction)
+__device__ __forceinline__ float dfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double dfma(double a, double b, double c) { return fma(a, b, c); }
 
 template <class S>
 __device__ __forceinline__ void transform_point(const S* m, const S* p, S* out, int n_out)

@@ -261,6 +261,32 @@ __device__ __forceinline__ float cdf_inv_sd<float>(float centre, float inv_sd, f
     return 0.5f + 0.5f * copysignf(erf_abs, v);
 }
 
+// The cdf and the density at x of the normal (centre, 1 / inv_sd), for the
+// adjoint of a bin's mass (d cdf / d centre = -inv_sd pdf, d cdf / d inv_sd =
+// (x - centre) pdf).  Double: erf and exp.  Float: the cdf above, whose
+// e^{-v^2} with v = z / sqrt 2 is the density's e^{-z^2 / 2}, so one
+// exponential serves both.
+template <class S>
+__device__ __forceinline__ void cdf_pdf_inv_sd(S centre, S inv_sd, S x, S* cdf, S* pdf)
+{
+    const S z = (x - centre) * inv_sd;
+    *cdf = S(0.5) * (S(1) + derf(z * S(0.70710678118654752440)));
+    *pdf = dexp(S(-0.5) * z * z) * S(0.39894228040143267794);
+}
+
+template <>
+__device__ __forceinline__ void cdf_pdf_inv_sd<float>(float centre, float inv_sd, float x, float* cdf, float* pdf)
+{
+    const float v = (x - centre) * inv_sd * 0.70710678118654752440f;
+    const float a = fabsf(v);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.f));
+    const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f), 0.254829592f);
+    const float e = __builtin_amdgcn_exp2f(-a * a * 1.4426950408889634f);
+    const float erf_abs = fmaf(-poly, e, 1.f);
+    *cdf = 0.5f + 0.5f * copysignf(erf_abs, v);
+    *pdf = e * 0.39894228040143267794f;
+}
+
 // pixel ray, vol_marcher_forward.cu:308-315
 template <class S>
 __device__ __forceinline__ void pixel_ray(const DevCamera<S>& cam, int px, int py, S* dir)

@@ -30,8 +30,62 @@ DEFAULT_TILING = ((16, 16), 256)
 # and the reference's debug render modes (ForwardData::debug_render_mode,
 # debug_render_bin, max_depth; vol_marcher.h:44-55), forward only
 DEFAULT_OPTIONS = {"max_large_steps": 0, "frozen": None, "bins_per_chunk": 0, "render_mode": 0, "debug_render_bin": -1,
-                   "max_depth": 20.0}
+                   "max_depth": 20.0, "replay_steps": 0, "replay_max_bytes": 16 << 30}
 RENDER_MODES = {"full": 0, "bins": 1, "depth": 2}
+
+
+class MarchReplay:
+    """Device memory in which the forward keeps each pixel's bins and bin
+    borders for the first `steps` large steps (dgmr_march_options.replay),
+    so that the backward reads them back instead of recomputing them.  Tiles
+    whose rays took more large steps are recomputed; results are the same
+    either way.  No counterpart in the reference, whose backward recomputes
+    every bin (vol_marcher_backward.cu:126-318)."""
+
+    def __init__(self, buffer: torch.Tensor, steps: int):
+        self.buffer = buffer
+        self.steps = int(steps)
+
+
+def replay_bytes(width: int, height: int, tile, batch: int, steps: int, dtype) -> int:
+    lib = N.load()
+    out = ctypes.c_size_t(0)
+    N.check(lib.dgmr_march_replay_size(int(width), int(height), ctypes.byref(C.tiling_struct(tile, batch)), int(steps),
+                                       N.DGMR_F64 if dtype == torch.float64 else N.DGMR_F32, ctypes.byref(out)), "dgmr_march_replay_size")
+    return int(out.value)
+
+
+def make_replay(width: int, height: int, dtype, device, tile=None, batch: int = 0, steps=None, max_bytes=None):
+    """A replay store for one frame, or None when none fits: `steps` large
+    steps (default: the large-step cap when set, else 4, at most 4), fewer
+    while the store would exceed max_bytes or a quarter of the free device
+    memory."""
+    tile = tile or DEFAULT_TILING[0]
+    batch = batch or DEFAULT_TILING[1]
+    if steps is None:
+        steps = int(DEFAULT_OPTIONS.get("replay_steps", 0))
+    if steps <= 0:
+        cap = int(DEFAULT_OPTIONS["max_large_steps"])
+        steps = min(cap, 4) if cap > 0 else 4
+    limit = int(DEFAULT_OPTIONS.get("replay_max_bytes", 0) if max_bytes is None else max_bytes)
+    free, _total = torch.cuda.mem_get_info(device)
+    limit = min(limit, free // 4)
+    while steps >= 1:
+        nbytes = replay_bytes(width, height, tile, batch, steps, dtype)
+        if nbytes <= limit:
+            return MarchReplay(torch.empty((nbytes,), dtype=torch.uint8, device=device), steps)
+        steps -= 1
+    return None
+
+
+def _replay_options(o: N.MarchOptions, replay) -> None:
+    if replay is None:
+        return
+    if replay.buffer.data_ptr() % 256:
+        raise ValueError("replay store must be 256-byte aligned")
+    o.replay = replay.buffer.data_ptr()
+    o.replay_bytes = replay.buffer.numel()
+    o.replay_steps = replay.steps
 
 
 def _options(max_large_steps=None, frozen=None, render_mode=None, debug_render_bin=None, max_depth=None) -> N.MarchOptions:
@@ -55,7 +109,7 @@ def _options(max_large_steps=None, frozen=None, render_mode=None, debug_render_b
 def _forward(sh_params, weights, centroids, cov_scales, cov_rotations, view_matrix, proj_matrix, cam_position, background,
              render_width: int, render_height: int, tan_fovx: float, tan_fovy: float, sh_degree: int,
              tile=None, batch: int = 0, max_large_steps=None, frozen=None, scale_multiplier: float = 1.0, render_mode=None,
-             debug_render_bin=None, max_depth=None):
+             debug_render_bin=None, max_depth=None, replay=None):
     tile = tile or DEFAULT_TILING[0]
     batch = batch or DEFAULT_TILING[1]
     sh_params, weights, centroids, cov_scales, cov_rotations = C.prepare_inputs(sh_params, weights, centroids, cov_scales, cov_rotations,
@@ -69,6 +123,7 @@ def _forward(sh_params, weights, centroids, cov_scales, cov_rotations, view_matr
     cam = C.camera_struct(view_matrix, proj_matrix, cam_position, background, W, H, tan_fovx, tan_fovy)
     g = C.gaussians_struct(sh_params, weights, centroids, cov_scales, cov_rotations, sh_degree, scale_multiplier)
     opt = _options(max_large_steps, frozen, render_mode, debug_render_bin, max_depth)
+    _replay_options(opt, replay)
     gx, gy = (W + tiling.tile_w - 1) // tiling.tile_w, (H + tiling.tile_h - 1) // tiling.tile_h
     fo = dict(dtype=dt, device=dev)
     io = dict(dtype=torch.int32, device=dev)
@@ -112,7 +167,7 @@ def _forward(sh_params, weights, centroids, cov_scales, cov_rotations, view_matr
 
 def _backward(sh_params, weights, centroids, cov_scales, cov_rotations, view_matrix, proj_matrix, cam_position, background,
               render_width: int, render_height: int, tan_fovx: float, tan_fovy: float, sh_degree: int, grad_framebuffer, cache_vector,
-              tile=None, batch: int = 0, max_large_steps=None, frozen=None, scale_multiplier: float = 1.0):
+              tile=None, batch: int = 0, max_large_steps=None, frozen=None, scale_multiplier: float = 1.0, replay=None):
     tile = tile or DEFAULT_TILING[0]
     batch = batch or DEFAULT_TILING[1]
     sh_params, weights, centroids, cov_scales, cov_rotations = C.prepare_inputs(sh_params, weights, centroids, cov_scales, cov_rotations,
@@ -127,6 +182,7 @@ def _backward(sh_params, weights, centroids, cov_scales, cov_rotations, view_mat
     cam = C.camera_struct(view_matrix, proj_matrix, cam_position, background, W, H, tan_fovx, tan_fovy)
     g = C.gaussians_struct(sh_params, weights, centroids, cov_scales, cov_rotations, sh_degree, scale_multiplier)
     opt = _options(max_large_steps, frozen)
+    _replay_options(opt, replay)
     grad_framebuffer = grad_framebuffer.to(dt).contiguous()
     fo = dict(dtype=dt, device=dev)
     with torch.cuda.device(dev):
@@ -157,8 +213,12 @@ class VolMarcher(torch.autograd.Function):
     def forward(ctx, shs: torch.Tensor, opacity: torch.Tensor, means3D: torch.Tensor, scales: torch.Tensor, rotations: torch.Tensor,
                 world_view_transform: torch.Tensor, full_proj_transform: torch.Tensor, camera_center: torch.Tensor,
                 bg_color: torch.Tensor, image_width: int, image_height: int, tanfovx: float, tanfovy: float, active_sh_degree: int):
+        # the replay store lives beside the reference's cache, for this call's backward only
+        ctx.replay = None
+        if any(ctx.needs_input_grad[:5]) and int(DEFAULT_OPTIONS["render_mode"]) == 0:
+            ctx.replay = make_replay(int(image_width), int(image_height), means3D.dtype, means3D.device)
         cache = cpp_binding.forward(shs, opacity, means3D, scales, rotations, world_view_transform, full_proj_transform, camera_center,
-                                    bg_color, image_width, image_height, tanfovx, tanfovy, active_sh_degree)
+                                    bg_color, image_width, image_height, tanfovx, tanfovy, active_sh_degree, replay=ctx.replay)
         ctx.save_for_backward(shs, opacity, means3D, scales, rotations, world_view_transform, full_proj_transform, camera_center,
                               bg_color, torch.tensor(image_width), torch.tensor(image_height), torch.tensor(tanfovx), torch.tensor(tanfovy),
                               torch.tensor(active_sh_degree), *cache)
@@ -172,7 +232,9 @@ class VolMarcher(torch.autograd.Function):
             image_height, tanfovx, tanfovy, active_sh_degree, *cache = ctx.saved_tensors
         grad_sh, grad_weights, grad_positions, grad_scales, grad_rotations = cpp_binding.backward(
             shs, opacity, means3D, scales, rotations, world_view_transform, full_proj_transform, camera_center, bg_color,
-            image_width.item(), image_height.item(), tanfovx.item(), tanfovy.item(), active_sh_degree.item(), grad_output, cache)
+            image_width.item(), image_height.item(), tanfovx.item(), tanfovy.item(), active_sh_degree.item(), grad_output, cache,
+            replay=ctx.replay)
+        ctx.replay = None
         return grad_sh, grad_weights, grad_positions, grad_scales, grad_rotations, None, None, None, None, None, None, None, None, None
 
 

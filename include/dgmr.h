@@ -158,7 +158,7 @@ typedef struct dgmr_march_options {
     int32_t n_frozen;         /* >0: fixed sample sections for gradient checks
                                  (DGMR_TORCH_GRAD_CHECK_CONST_SAMPLES, vol_marcher.h:25) */
     double frozen[8][3];      /* start, end, step of each frozen section */
-    int32_t bins_per_chunk;   /* float path: 16 or 32 bins kept in registers per pass (0: 32 forward, 16 backward); results do not depend on it */
+    int32_t bins_per_chunk;   /* float path: 16 or 32 bins kept in registers per pass (0: 32); results do not depend on it */
     /* ForwardData::debug_render_mode, debug_render_bin, max_depth
      * (vol_marcher.h:44-55; the blends of vol_marcher_forward.cu:443-484),
      * forward only (the backward computes the full mode's gradients):
@@ -169,7 +169,24 @@ typedef struct dgmr_march_options {
     int32_t render_mode;
     int32_t debug_render_bin;
     double max_depth;         /* <= 0: 20, the reference's default */
+    /* Replay store (no counterpart in the reference, whose backward
+     * recomputes every bin): device memory of dgmr_march_replay_size bytes,
+     * 256-byte aligned, or null.  The full-mode forward keeps in it, for the
+     * first replay_steps large steps of every tile, each pixel's bins and
+     * bin borders; a backward given the same store (and the same scene,
+     * camera, tiling and options, with nothing else written to it between
+     * the two calls) reads them back instead of recomputing them, for the
+     * tiles whose rays took at most replay_steps large steps, and recomputes
+     * the rest.  Results are the same either way. */
+    void* replay;
+    size_t replay_bytes;
+    int32_t replay_steps;
 } dgmr_march_options;
+
+/* Bytes of a replay store keeping replay_steps large steps per tile
+ * (dtype: dgmr_dtype of the scene).  About 2.5 KB (float) per pixel and
+ * kept step. */
+int dgmr_march_replay_size(int32_t width, int32_t height, const dgmr_tiling* tiling, int32_t replay_steps, int32_t dtype, size_t* bytes);
 
 /* Marcher forward: replaces dgmr::vol_marcher::forward<T>
  * (vol_marcher_forward.cu:42-43).  framebuffer [3,H,W], remaining

@@ -45,7 +45,10 @@ def small_scene(n=3000, extent=1.5, w=64, h=48, seed=3):
     return scene, cam
 
 
-def run_gpu(scene, cam, bg, dtype, grad_fb=None, max_large_steps=0, frozen=None, tile=(16, 16)):
+def run_gpu(scene, cam, bg, dtype, grad_fb=None, max_large_steps=0, frozen=None, tile=(16, 16), replay_steps=0, render_mode=None):
+    """Forward (and backward with grad_fb) through the reference binding;
+    replay_steps > 0 gives both calls a replay store keeping that many large
+    steps, whose per-tile step counts come back as out["tile_steps"]."""
     torch = _torch()
     from differentiable_gaussian_mixture_renderer_amd.python_bindings import vol_marcher as vm
     dev = torch.device("cuda:0")
@@ -53,14 +56,23 @@ def run_gpu(scene, cam, bg, dtype, grad_fb=None, max_large_steps=0, frozen=None,
     args = [t(scene.sh), t(scene.opacity), t(scene.means), t(scene.scales), t(scene.rotations)]
     cam_args = [t(cam.world_view_transform), t(cam.full_proj_transform), t(cam.camera_center), t(bg), cam.width, cam.height,
                 cam.tan_fovx, cam.tan_fovy, scene.sh_degree]
-    print(f"[gpu march] n={scene.n} {cam.width}x{cam.height} {dtype} frozen={frozen is not None}", flush=True)
-    cache = vm.cpp_binding.forward(*args, *cam_args, tile=tile, max_large_steps=max_large_steps, frozen=frozen)
+    print(f"[gpu march] n={scene.n} {cam.width}x{cam.height} {dtype} frozen={frozen is not None} replay={replay_steps}", flush=True)
+    replay = None
+    if replay_steps:
+        replay = vm.make_replay(cam.width, cam.height, dtype, dev, tile=tile, steps=replay_steps, max_bytes=1 << 40)
+        assert replay is not None and replay.steps == replay_steps
+    cache = vm.cpp_binding.forward(*args, *cam_args, tile=tile, max_large_steps=max_large_steps, frozen=frozen, replay=replay,
+                                   render_mode=render_mode)
     names = ("framebuffer", "rects", "rgb", "clamped", "depths", "xy", "inv_cov", "masses", "tiles", "offsets", "ranges",
              "point_list", "remaining")
     out = {k: v.cpu().numpy() for k, v in zip(names, cache)}
     if grad_fb is not None:
-        grads = vm.cpp_binding.backward(*args, *cam_args, t(grad_fb), cache, tile=tile, max_large_steps=max_large_steps, frozen=frozen)
+        grads = vm.cpp_binding.backward(*args, *cam_args, t(grad_fb), cache, tile=tile, max_large_steps=max_large_steps, frozen=frozen,
+                                        replay=replay)
         out["grads"] = dict(zip(("sh", "opacity", "means", "scales", "rotations"), [g.cpu().numpy() for g in grads]))
+    if replay is not None:
+        n_tiles = out["ranges"].shape[0]
+        out["tile_steps"] = replay.buffer[:4 * n_tiles].cpu().numpy().view(np.uint32).copy()
     torch.cuda.synchronize()
     return out
 
@@ -254,6 +266,12 @@ def test_volmarcher_autograd_and_large_step_cap():
                    t(np.zeros(3, np.float32)), cam.width, cam.height, cam.tan_fovx, cam.tan_fovy, 3)
     img.sum().backward()
     assert all(p.grad is not None and torch.isfinite(p.grad).all() for p in params)
+    # autograd keeps a replay store between its forward and backward; the
+    # gradients are the recomputing backward's
+    plain = run_gpu(scene, cam, np.zeros(3, np.float32), torch.float32, grad_fb=np.ones((3, cam.height, cam.width), np.float32))
+    for p, name in zip(params, ("sh", "opacity", "means", "scales", "rotations")):
+        want = plain["grads"][name].reshape(p.shape)
+        assert np.abs(p.grad.cpu().numpy() - want).max() <= 2e-5 * np.abs(want).max(), name
     # a cap of 2 large steps (<= 254 bins per ray) matches the oracle under the same cap
     bg = np.zeros(3, np.float32)
     out = run_gpu(scene, cam, bg, torch.float64, max_large_steps=2)
@@ -355,3 +373,58 @@ def test_debug_render_modes(mode, bin_):
     if mode == 1:
         assert (T == 0).all()
         assert fin.all() and (fb64 != 0).any()
+
+
+@pytest.mark.parametrize("dtype_name,tile", [("float64", (16, 16)), ("float32", (16, 16)), ("float32", (8, 8))])
+def test_replay_store_matches_recomputation(dtype_name, tile):
+    """With a replay store the forward keeps each pixel's bins and borders and
+    the backward reads them back for the tiles whose block took at most
+    replay_steps large steps, recomputing the others.  Keeping the fewest
+    steps any tile took (a mix of both kinds of tile) and the most (every
+    tile read back) give the recomputing backward's gradients: within float64 rounding in double, and
+    within the float atomics' summation order in float; the forward's image
+    is bit-identical.  Double is also held to the oracle at 1e-9."""
+    torch = _torch()
+    dtype = getattr(torch, dtype_name)
+    scene, cam = small_scene(n=3000, w=64, h=48, seed=4)
+    dbl = dtype_name == "float64"
+    bg = np.array([0.2, 0.3, 0.4]) if dbl else np.array([0.2, 0.3, 0.4], np.float32)
+    rng = np.random.default_rng(8)
+    grad = rng.normal(size=(3, cam.height, cam.width)) / (3 * cam.height * cam.width)
+    grad = grad if dbl else grad.astype(np.float32)
+    base = run_gpu(scene, cam, bg, dtype, grad_fb=grad, tile=tile)
+    # the tiles' large-step counts (written whatever the store keeps)
+    counts = run_gpu(scene, cam, bg, dtype, tile=tile, replay_steps=1)["tile_steps"]
+    assert counts.max() > counts.min()
+    for steps in (int(counts.min()), int(counts.max())):
+        out = run_gpu(scene, cam, bg, dtype, grad_fb=grad, tile=tile, replay_steps=steps)
+        np.testing.assert_array_equal(out["framebuffer"], base["framebuffer"])
+        np.testing.assert_array_equal(out["remaining"], base["remaining"])
+        ts = out["tile_steps"]
+        np.testing.assert_array_equal(ts, counts)
+        print(f"replay {steps}: tiles read back {(ts <= steps).sum()} of {ts.size}, most steps {ts.max()}")
+        tol = 1e-12 if dbl else 2e-5
+        for name in ("means", "scales", "rotations", "opacity", "sh"):
+            got, want = out["grads"][name], base["grads"][name]
+            rel = np.abs(got - want).max() / np.abs(want).max()
+            print(f"  {name}: max rel {rel:.3g}")
+            assert rel <= tol, (steps, name)
+    if dbl:
+        ref = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=False)
+        fb, T, _ = ref.render(bg)
+        r = ref.backward(bg, fb, T, grad)
+        n, k = scene.n, scene.sh.shape[1]
+        for name in ("means", "scales", "rotations", "opacity", "sh"):
+            got = out["grads"][name].reshape(n, -1)
+            want = r[name].reshape(n, -1)[:, :got.shape[1]]
+            assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max(), name
+
+
+def test_replay_store_after_a_debug_render_keeps_nothing():
+    """A debug-mode forward marks every tile as not kept, so a backward given
+    the same store recomputes everything (and matches the plain backward)."""
+    torch = _torch()
+    scene, cam = small_scene(n=1500, w=48, h=32, seed=6)
+    bg = np.array([0.2, 0.3, 0.4], np.float32)
+    out = run_gpu(scene, cam, bg, torch.float32, replay_steps=2, render_mode=1)
+    assert (out["tile_steps"] == 0xFFFFFFFF).all()
