@@ -78,6 +78,8 @@ def lib() -> ctypes.CDLL:
         f32 = ctypes.c_float
         L.orc_raster_render.argtypes = [i64, vp, vp, vp, vp, vp, i32, f64, vp, vp, vp, f64, f64, i32, i32, vp, i32, i32, f32, i32, i32, i32,
                                         i32, vp, vp]
+        L.orc_raster_render64.argtypes = [i64, vp, vp, vp, vp, vp, i32, f64, vp, vp, vp, f64, f64, i32, i32, vp, i32, i32, f32, i32, i32,
+                                          i32, i32, vp, vp, vp]
         L.orc_binsizer_run.restype = i32
         L.orc_binsizer_run.argtypes = [i32, f32, f32, i32, vp, vp, vp, vp, vp]
         L.orc_pl_create.argtypes = [vp, vp, vp, vp, f64, vp]
@@ -263,19 +265,33 @@ def scene_arrays(scene):
 RASTER_FULL, RASTER_BINS, RASTER_DEPTH = 0, 1, 2
 
 
-def raster_render(scene_arrays, cam, bg, mode=RASTER_FULL, debug_bin=-1, max_depth=20.0, crop=None, scale_multiplier=1.0):
+def raster_render(scene_arrays, cam, bg, mode=RASTER_FULL, debug_bin=-1, max_depth=20.0, crop=None, scale_multiplier=1.0, dbl=False):
     """Float restatement of vol_raster_forward (vol_raster_forward.cu:36-520):
     planar [3, H, W] image (pixels outside crop = (x0, y0, x1, y1) stay 0, and
     all stay 0 when nothing is binned, as the reference returns before
-    rendering) and the pair count of the binning."""
+    rendering) and the pair count of the binning.
+
+    dbl=True evaluates it in FP64 on the float32 pipeline's placement (the
+    scene's float32 values, the float32 binning and per-pixel ray, everything
+    else in double) and returns (image float64, pair count, ambiguity [H, W]):
+    the RasterTieKind bits of the decisions that lay within float rounding of
+    their thresholds (mass 1, centre order 2, cheapest merge 4, fullness 8,
+    approximation branch 16)."""
     means, scales, rots, opac, sh16, deg = scene_arrays
     f = np.float32
     keep = [_c(means, f), _c(scales, f), _c(rots, f), _c(np.reshape(opac, -1), f), _c(sh16, f), _c(cam.world_view_transform, f),
-            _c(cam.full_proj_transform, f), _c(cam.camera_center, f), _c(bg, f)]
+            _c(cam.full_proj_transform, f), _c(cam.camera_center, f), _c(bg, np.float64 if dbl else f)]
     W, H = cam.width, cam.height
     x0, y0, x1, y1 = crop if crop is not None else (0, 0, W, H)
-    fb = np.zeros((3, H, W), np.float32)
     n_pairs = ctypes.c_int64(0)
+    if dbl:
+        fb = np.zeros((3, H, W), np.float64)
+        amb = np.zeros((H, W), np.uint8)
+        lib().orc_raster_render64(means.shape[0], P(keep[0]), P(keep[1]), P(keep[2]), P(keep[3]), P(keep[4]), deg, scale_multiplier,
+                                  P(keep[5]), P(keep[6]), P(keep[7]), float(np.float32(cam.tan_fovx)), float(np.float32(cam.tan_fovy)),
+                                  W, H, P(keep[8]), mode, debug_bin, max_depth, x0, y0, x1, y1, P(fb), P(amb), ctypes.byref(n_pairs))
+        return fb, int(n_pairs.value), amb
+    fb = np.zeros((3, H, W), np.float32)
     lib().orc_raster_render(means.shape[0], P(keep[0]), P(keep[1]), P(keep[2]), P(keep[3]), P(keep[4]), deg, scale_multiplier,
                             P(keep[5]), P(keep[6]), P(keep[7]), float(np.float32(cam.tan_fovx)), float(np.float32(cam.tan_fovy)), W, H,
                             P(keep[8]), mode, debug_bin, max_depth, x0, y0, x1, y1, P(fb), ctypes.byref(n_pairs))

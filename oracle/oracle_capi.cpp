@@ -816,6 +816,60 @@ void orc_raster_render(int64_t n, const float* means, const float* scales, const
         }
 }
 
+// The rasteriser evaluated in FP64 on the float32 pipeline's placement: the
+// scene (float32 arrays) converted to double, the binning and the per-pixel
+// ray taken from the float32 restatement, everything else in double.
+// ambiguous (optional, [H,W]) gets each pixel's RasterTieKind bits: the
+// decisions that lay within float rounding of their thresholds.
+void orc_raster_render64(int64_t n, const float* means, const float* scales, const float* rotations, const float* opacity, const float* sh16,
+    int sh_degree, double scale_multiplier, const float* view, const float* proj, const float* cam_pos, double tan_fovx, double tan_fovy,
+    int width, int height, const double* bg, int mode, int debug_bin, float max_depth, int x0, int y0, int x1, int y1, double* fb,
+    uint8_t* ambiguous, int64_t* n_pairs)
+{
+    SceneStore<float> sf;
+    sf.load(n, means, scales, rotations, opacity, sh16, sh_degree, scale_multiplier, view, proj, cam_pos, tan_fovx, tan_fovy, width, height,
+        false);
+    SceneStore<double> sd;
+    sd.load(n, means, scales, rotations, opacity, sh16, sh_degree, scale_multiplier, view, proj, cam_pos, double(float(tan_fovx)),
+        double(float(tan_fovy)), width, height, false);
+    SplatGeometry<float> gf;
+    splat_preprocess(sf.scene, sf.cam, 16, 16, &gf);
+    SplatGeometry<double> gd;
+    splat_preprocess(sd.scene, sd.cam, 16, 16, &gd, &gf);
+    Binning b;
+    bin(gf.rect, gf.tiles, gf.xy, gf.key_depth, width, height, 16, 16, &b);
+    if (n_pairs)
+        *n_pairs = int64_t(b.point_list.size());
+    if (b.point_list.empty())
+        return;
+    RasterExtra<double> x;
+    raster_preprocess_extra(sd.scene, sd.cam, gd, &x);
+    double pd[16], ipd[16];
+    for (int i = 0; i < 16; ++i)
+        pd[i] = sf.cam.proj[i];
+    invert4(pd, ipd);
+    float ipf[16];
+    for (int i = 0; i < 16; ++i)
+        ipf[i] = float(ipd[i]);
+    crop_guard<double>(width, height, &x0, &y0, &x1, &y1);
+    const size_t plane = size_t(width) * height;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int py = y0; py < y1; ++py)
+        for (int px = x0; px < x1; ++px) {
+            float dirf[3];
+            raster_ray(sf.cam, ipf, px, py, dirf);
+            TieWatch tw;
+            tie_watch = ambiguous ? &tw : nullptr;
+            double out[3];
+            raster_pixel(sd.scene, gd, x, b, sd.cam, ipd, px, py, bg, mode, debug_bin, max_depth, out, dirf);
+            tie_watch = nullptr;
+            for (int c = 0; c < 3; ++c)
+                fb[c * plane + size_t(py) * width + px] = out[c];
+            if (ambiguous)
+                ambiguous[size_t(py) * width + px] = uint8_t(tw.hit);
+        }
+}
+
 // RasterBinSizer with n_bins = nb (4 or 32) in float: the adds (mass, centre,
 // sd) in order, full_after[k] = is_full() after add k, then finalise();
 // begin/end of every bin and is_full() afterwards.  Returns 0, or -1 for an

@@ -27,6 +27,36 @@ struct RasterConfig {
     static constexpr float filter_kernel_sd = 0.55f;
 };
 
+// Decisions of the rasteriser that a float32 run can take the other way when
+// the FP64 value lies within float rounding of the threshold (recorded through
+// the marcher's TieWatch, per pixel, in FP64 runs only)
+enum RasterTieKind : unsigned {
+    kRasterTieMass = 1,   // a mass against its threshold: on the ray (1e-3, 1000), the sizer's 1e-4,
+                          // in the bins (1e-3), per bin (1e-4), a bin's total (1e-6), a merge's weights (1e-7)
+    kRasterTieOrder = 2,  // the sizer's order of centres
+    kRasterTieMerge = 4,  // the sizer's cheapest merge against the runner-up
+    kRasterTieFull = 8,   // the sizer's accumulated mass against -ln(0.005)
+    kRasterTieShape = 16  // the piecewise-linear approximation's branch t2 > t1 (its m_r > m_l is decided
+                          // exactly: the reference's left share is 0.5, so m_l = m_r in every precision)
+};
+
+struct RasterTolerance {
+    static constexpr double rel = 1e-4;      // relative, for masses, costs and the fullness sum
+    // relative with a floor of 1, for the centres the sizer orders: a float
+    // centre is a few ulp of its distance off (no accumulation along the ray)
+    static constexpr double position = 4e-6;
+    static constexpr double knot = 2e-5;     // relative to the bin's width, for the approximation's knots
+    static constexpr double cdf_abs = 4e-7;  // absolute error of a float cdf difference (per unit of mass on the ray)
+};
+
+inline void raster_watch(double a, double b, double rel, double abs_tol, unsigned kind)
+{
+    if (!tie_watch || !std::isfinite(a) || !std::isfinite(b))
+        return;
+    if (std::abs(a - b) <= rel * std::max(std::abs(a), std::abs(b)) + abs_tol)
+        tie_watch->hit |= kind;
+}
+
 // stroke::gaussian::integrate_normalised_SD over [a, b]: Phi((b - c)/sd) - Phi((a - c)/sd)
 template <class S>
 inline S integrate_normalised_sd(S centre, S sd, S a, S b)
@@ -68,10 +98,15 @@ struct BinSizer {
     }
     S end_of(int i) const { return i == NB - 1 ? max_depth : borders[i]; }
     S begin_of(int i) const { return i == 0 ? S(0) : end_of(i - 1); }
-    bool is_full() const { return accumulated >= mass_threshold; }
+    bool is_full() const
+    {
+        raster_watch(double(accumulated), double(mass_threshold), RasterTolerance::rel, 0.0, kRasterTieFull);
+        return accumulated >= mass_threshold;
+    }
 
     void add_gaussian(S mass, S centre, S sd)
     {
+        raster_watch(double(mass), double(0.0001f), RasterTolerance::rel, 0.0, kRasterTieMass);
         if (mass < S(0.0001f))
             return;
         const S t_rel = centre + sd * relevance_sigma;
@@ -83,6 +118,7 @@ struct BinSizer {
         borders[last] = centre;
         sds[last] = sd;
         for (int i = last; i > 0; --i) {
+            raster_watch(double(borders[i - 1]), double(borders[i]), RasterTolerance::position, RasterTolerance::position, kRasterTieOrder);
             if (borders[i - 1] > borders[i]) {
                 std::swap(borders[i - 1], borders[i]);
                 std::swap(sds[i - 1], sds[i]);
@@ -104,6 +140,7 @@ struct BinSizer {
         const auto merge = [&](int a, int b) {
             const S sa = before(a) * masses[a];
             const S sb = before(b) * masses[b];
+            raster_watch(double(sa + sb), double(0.0000001f), RasterTolerance::rel, 0.0, kRasterTieMass);
             const S pa = (sa + sb >= S(0.0000001f)) ? std::min(sa / (sa + sb), S(0.999999f)) : S(0.5f);
             const S pb = 1 - pa;
             const S nc = pa * borders[a] + pb * borders[b];
@@ -123,13 +160,18 @@ struct BinSizer {
         if (masses[last] > 0) {
             int best = -1;
             S best_val = std::numeric_limits<S>::infinity();
+            S runner_up = std::numeric_limits<S>::infinity();
             for (int j = 0; j < NB - 1; ++j) {
                 const S v = cost(j);
                 if (v < best_val) {
+                    runner_up = best_val;
                     best_val = v;
                     best = j;
+                } else if (v < runner_up) {
+                    runner_up = v;
                 }
             }
+            raster_watch(double(best_val), double(runner_up), RasterTolerance::rel, 1e-12, kRasterTieMerge);
             // the reference asserts a candidate exists; with none (every cost
             // NaN or infinite) its merge would index slot -1, here slot 0 merges
             if (best < 0)
@@ -195,6 +237,8 @@ inline PiecewiseLinear<S> create_approximation(const S* percent_left, const S* t
         f.t1[i] = S(2) * m_l / eval_left[i];
         f.t2[i] = t_right - S(2) * m_r / eval_right[i];
         f.k2[i] = 0;
+        raster_watch(double(total[i]), 0.000001, RasterTolerance::rel, 0.0, kRasterTieMass);
+        raster_watch(double(f.t2[i]), double(f.t1[i]), 0.0, RasterTolerance::knot * double(t_right), kRasterTieShape);
         if (total[i] <= S(0.000001)) {
             f.d0[i] = 0;
             f.k0[i] = 0;
@@ -306,13 +350,21 @@ inline void raster_ray(const Camera<S>& cam, const S* inv_proj, int px, int py, 
 enum RasterMode { RasterFull = 0, RasterBins = 1, RasterDepth = 2 };
 
 // one pixel, vol_raster_forward.cu:280-516
+// ray_f (optional): the pixel's ray direction as the float32 pipeline
+// computes it, used instead of this precision's (the near-plane unprojection
+// leaves a float32 direction ~1e-5 off)
 template <class S>
 inline void raster_pixel(const Scene<S>& sc, const SplatGeometry<S>& g, const RasterExtra<S>& x, const Binning& b, const Camera<S>& cam,
-    const S* inv_proj, int px, int py, const S* bg, int mode, int debug_bin, float max_depth_scale, S* out)
+    const S* inv_proj, int px, int py, const S* bg, int mode, int debug_bin, float max_depth_scale, S* out, const float* ray_f = nullptr)
 {
     constexpr int NB = RasterConfig::n_bins;
     S d[3];
-    raster_ray(cam, inv_proj, px, py, d);
+    if (ray_f) {
+        for (int i = 0; i < 3; ++i)
+            d[i] = S(ray_f[i]);
+    } else {
+        raster_ray(cam, inv_proj, px, py, d);
+    }
     const int tile = (py / b.tile_h) * b.grid_x + (px / b.tile_w);
     const uint32_t first = b.ranges[2 * tile], last = b.ranges[2 * tile + 1];
     BinSizer<S, NB> sizer(RasterConfig::transmission_threshold, RasterConfig::relevance_sigma);
@@ -320,6 +372,8 @@ inline void raster_pixel(const Scene<S>& sc, const SplatGeometry<S>& g, const Ra
         const uint32_t id = b.point_list[k];
         const Gaussian1d<S> g1 = intersect_ray(sc.means + 3 * id, &x.inv_cov[6 * id], cam.pos, d);
         const S mor = g1.weight * x.mass[id];
+        raster_watch(double(mor), 0.001, RasterTolerance::rel, 0.0, kRasterTieMass);
+        raster_watch(double(mor), 1000.0, RasterTolerance::rel, 0.0, kRasterTieMass);
         if (mor < S(0.001) || mor > S(1000))
             continue;
         if (g1.var <= 0)
@@ -339,9 +393,12 @@ inline void raster_pixel(const Scene<S>& sc, const SplatGeometry<S>& g, const Ra
         const S sd = std::sqrt(var);
         const S inv_sd = S(1) / sd;
         const S w = g1.weight * x.mass[id];
+        raster_watch(double(w), 100000.0, RasterTolerance::rel, 0.0, kRasterTieMass);
         if (var <= 0 || std::isnan(var) || std::isnan(w) || w > S(100000))
             continue;
-        if (w * (cdf_inv_sd(c, inv_sd, sizer.end_of(NB - 1)) - cdf_inv_sd(c, inv_sd, S(0))) <= S(0.001f))
+        const S in_bins = w * (cdf_inv_sd(c, inv_sd, sizer.end_of(NB - 1)) - cdf_inv_sd(c, inv_sd, S(0)));
+        raster_watch(double(in_bins), double(0.001f), RasterTolerance::rel, double(w) * RasterTolerance::cdf_abs, kRasterTieMass);
+        if (in_bins <= S(0.001f))
             continue;
         const S* rgb = &g.rgb[3 * id];
         S cdf_start = cdf_inv_sd(c, inv_sd, S(0));
@@ -349,6 +406,7 @@ inline void raster_pixel(const Scene<S>& sc, const SplatGeometry<S>& g, const Ra
             const S cdf_end = cdf_inv_sd(c, inv_sd, sizer.end_of(k2));
             const S m = (cdf_end - cdf_start) * w;
             cdf_start = cdf_end;
+            raster_watch(double(m), double(0.0001f), RasterTolerance::rel, double(w) * RasterTolerance::cdf_abs, kRasterTieMass);
             if (m < S(0.0001f))
                 continue;
             for (int ch = 0; ch < 3; ++ch)
