@@ -118,21 +118,27 @@ __global__ __launch_bounds__(256) void k_zero_screen_grads(int64_t n_floats, flo
 // thread per Gaussian, summing its adjoint over the views of each gradient
 // group: group[v] is -1 (no gradient) or a group id whose views are
 // consecutive; group g's sums go to block g of the outputs ([n_groups][n][..]).
+// n_views here is the number of views up to the last one with a group: the
+// views after it are neither loaded nor visited.
 __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, int n_views, SplatParams p, const DevCamera<float>* __restrict__ cams,
     const int32_t* __restrict__ group, const uint32_t* __restrict__ tiles, const float4* __restrict__ conic_opacity,
     const uint8_t* __restrict__ clamped, const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales,
     float* __restrict__ g_rotations, float* __restrict__ g_opacity, float* __restrict__ g_sh)
 {
+    // the block's SH coefficients, staged once; the SH gradient rows of a
+    // group leave through s_out, half a block at a time (75 KB of LDS keeps
+    // two blocks per CU, which the registers allow anyway)
     __shared__ float s_sh[256 * kShStride];
+    __shared__ float s_out[128 * kShStride];
     const int64_t first = int64_t(blockIdx.x) * blockDim.x;
     const int count = int(min<int64_t>(blockDim.x, n - first));
     stage_sh(s_sh, p.sh, first, count, p.sh_coeffs);
     __syncthreads();
     // the view loop and its group boundaries are uniform over the block, so
-    // every thread takes part in the block-wide staging at a boundary
+    // every thread takes part in the block-wide copies at a boundary
     const int64_t i = first + threadIdx.x;
     const bool valid = i < n;
-    float* row = s_sh + threadIdx.x * kShStride;
+    const float* row = s_sh + threadIdx.x * kShStride;
     const int kc = 3 * p.sh_coeffs;
     // per group: the mean, opacity and SH gradients, and the view sums of
     // dL/dcov3 and dL/d(2 pi sa sb) that become scale and rotation gradients
@@ -178,8 +184,7 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
         if (gv != cur) {
             if (cur >= 0) {
                 // views of group cur done: write its block; the SH gradient rows
-                // go through LDS so the block writes them as one contiguous span
-                __syncthreads(); // every thread has read its SH row
+                // go through LDS so the block writes them as contiguous spans
                 if (valid) {
                     float g_s[3], g_q[4];
                     preprocess_backward_finish(s, q, pre, p, g_cov3, g_num, g_s, g_q);
@@ -192,20 +197,21 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
                     for (int a = 0; a < 4; ++a)
                         g_rotations[4 * o + a] = g_q[a];
                     g_opacity[o] = g_w;
-#pragma unroll
-                    for (int k = 0; k < 48; ++k)
-                        if (k < kc)
-                            row[k] = g_shv[k];
                 }
-                __syncthreads();
-                unstage_sh(g_sh + int64_t(cur) * n * kc, s_sh, first, count, p.sh_coeffs);
-                bool more = false;
-                for (int u = v; u < n_views; ++u)
-                    more |= group[u] >= 0;
-                if (more) {
+                float* dst = g_sh + int64_t(cur) * n * kc;
+                for (int half = 0; half < 2; ++half) {
+                    const int lo = half * 128;
+                    __syncthreads(); // s_out free
+                    if (valid && int(threadIdx.x) - lo >= 0 && int(threadIdx.x) - lo < 128) {
+                        float* out_row = s_out + (int(threadIdx.x) - lo) * kShStride;
+#pragma unroll
+                        for (int k = 0; k < 48; ++k)
+                            if (k < kc)
+                                out_row[k] = g_shv[k];
+                    }
                     __syncthreads();
-                    stage_sh(s_sh, p.sh, first, count, p.sh_coeffs);
-                    __syncthreads();
+                    if (count > lo)
+                        unstage_sh(dst, s_out, first + lo, min(128, count - lo), p.sh_coeffs);
                 }
             }
             cur = gv;
@@ -462,7 +468,13 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
     }
     DGMR_HIP_CHECK(hipMemcpyAsync(L.views, vb.data(), sizeof(ViewBlend) * n_views, hipMemcpyHostToDevice, st));
     DGMR_HIP_CHECK(hipMemcpyAsync(L.groups, grp.data(), sizeof(int32_t) * n_views, hipMemcpyHostToDevice, st));
-    const int64_t m = n * n_views;
+    // views after the last one with a gradient group take no part in the
+    // backward: their screen gradients are neither cleared nor read
+    int n_grad_views = 0;
+    for (int v = 0; v < n_views; ++v)
+        if (grp[v] >= 0)
+            n_grad_views = v + 1;
+    const int64_t m = n * n_grad_views;
     {
         StageScope sc("batch.clear", st, 36.0 * double(m));
         const int64_t blocks = std::min<int64_t>(ceil_div64(ceil_div64(9 * m, 4), 256), 8192);
@@ -487,7 +499,7 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
         DGMR_HIP_CHECK(hipMemsetAsync(grad_opacity, 0, per * sizeof(float), st));
         DGMR_HIP_CHECK(hipMemsetAsync(grad_sh, 0, per * 3 * g->sh_coeffs * sizeof(float), st));
     }
-    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams, L.groups, L.tiles, L.conic_opacity,
+    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_grad_views, p, L.cams, L.groups, L.tiles, L.conic_opacity,
         L.clamped, L.screen_grads, grad_means, grad_scales, grad_rotations, grad_opacity, grad_sh);
     DGMR_HIP_CHECK(hipGetLastError());
     return DGMR_OK;

@@ -21,6 +21,7 @@ class _FakeStream:
 def host_only(monkeypatch):
     """Streams and the batched splatter replaced by host stand-ins."""
     monkeypatch.setattr(serving, "_worker_stream", lambda device, k: _FakeStream())
+    monkeypatch.setattr(serving, "_inline_stream", lambda device: _FakeStream())
     monkeypatch.setattr(serving.torch.cuda, "stream", lambda s: contextlib.nullcontext())
     monkeypatch.setattr(serving, "BatchSplatter", lambda **kw: None)
 
@@ -104,6 +105,45 @@ def test_idle_server_runs_a_lone_request_without_a_hand_off(host_only, monkeypat
     stats = srv.run(burst)
     assert len(stats.latencies) == len(burst)
     assert any(t is not threading.current_thread() for t in threads)
+
+
+def test_batches_overlap_only_with_a_backlog(host_only, monkeypatch):
+    """Below overlap_backlog waiting requests one batch runs at a time (a
+    request does not share the GPU with another at low load); a burst over
+    several scenes fills every worker."""
+    import time
+    state = {"running": 0, "most": 0}
+    lock = threading.Lock()
+
+    def work(n):
+        with lock:
+            state["running"] += 1
+            state["most"] = max(state["most"], state["running"])
+        time.sleep(0.004)
+        with lock:
+            state["running"] -= 1
+        return ([None] * n,) * 2
+
+    monkeypatch.setattr(serving.ContinuousBatchingServer, "_single", lambda self, sc, r: work(1))
+    monkeypatch.setattr(serving.ContinuousBatchingServer, "_batched", staticmethod(lambda sp, sc, batch, n_bwd: work(len(batch))))
+    resident = [serving.ResidentScene([torch.zeros(10, 1, 3), torch.zeros(10, 1), torch.zeros(10, 3), torch.zeros(10, 3),
+                                       torch.zeros(10, 4)], 0) for _ in range(4)]
+    # pairs of requests of different scenes 1 ms apart, every 30 ms: the second waits for the first
+    trace = serving.make_trace(10, rate_hz=50.0, n_scenes=4, seed=4, min_res=(16, 16), max_res=(32, 32))
+    for k, r in enumerate(trace):
+        r.scene = k % 2
+        r.arrival = 0.03 * (k // 2) + 0.001 * (k % 2)
+    srv = serving.ContinuousBatchingServer(resident, hbm_budget_bytes=1 << 30, workers=3, device="cpu", overlap_backlog=3)
+    stats = srv.run(trace)
+    assert len(stats.latencies) == len(trace) and state["most"] == 1
+    burst = serving.make_trace(40, rate_hz=50.0, n_scenes=4, seed=5, min_res=(16, 16), max_res=(32, 32))
+    for r in burst:
+        r.arrival = 0.0
+    state["most"] = 0
+    srv = serving.ContinuousBatchingServer(resident, hbm_budget_bytes=1 << 30, workers=3, device="cpu", overlap_backlog=3)
+    stats = srv.run(burst)
+    assert len(stats.latencies) == len(burst) and state["most"] == 3
+    assert srv._batches == 0 and srv._in_flight == 0
 
 
 def test_resident_sizes_span_the_range():
