@@ -411,8 +411,10 @@ def batched_workload(n: int, n_cams: int, reps: int = 20):
 def marcher_workload(lib, with_cpu: bool, steps: int = 3, max_large_steps: int = 2):
     """Volumetric ray marcher: 200k Gaussians, 1024x1024 rays, at most
     max_large_steps x 127 bins per ray (256 marching steps), fp32 forward +
-    L1 backward through VolMarcher.apply.  Per-stage durations from the stage
-    profile; the host baseline is the float restatement on a band of rows."""
+    L1 backward through VolMarcher.apply, with the default backward (every
+    bin recomputed, as the reference's) and with the opt-in replay store (the
+    forward's bins read back).  Per-stage durations from the stage profile;
+    the host baseline is the float restatement on a band of rows."""
     import numpy as np
     import torch
 
@@ -429,12 +431,14 @@ def marcher_workload(lib, with_cpu: bool, steps: int = 3, max_large_steps: int =
     cam_args = (torch.tensor(cam.world_view_transform), torch.tensor(cam.full_proj_transform), torch.tensor(cam.camera_center),
                 torch.zeros(3), W, H, cam.tan_fovx, cam.tan_fovy, scene.sh_degree)
     saved = dict(vol_marcher.DEFAULT_OPTIONS)
-    vol_marcher.DEFAULT_OPTIONS["max_large_steps"] = max_large_steps
-    try:
-        def step():
-            img = vol_marcher.VolMarcher.apply(*params, *cam_args)
-            return torch.autograd.grad(losses.l1_loss(img, target), params)
 
+    def step():
+        img = vol_marcher.VolMarcher.apply(*params, *cam_args)
+        return torch.autograd.grad(losses.l1_loss(img, target), params)
+
+    def measure(replay: bool):
+        vol_marcher.DEFAULT_OPTIONS["max_large_steps"] = max_large_steps
+        vol_marcher.DEFAULT_OPTIONS["replay"] = replay
         step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -447,14 +451,19 @@ def marcher_workload(lib, with_cpu: bool, steps: int = 3, max_large_steps: int =
         torch.cuda.synchronize()
         lib.dgmr_profile_enable(0)
         stages = {k: round(v[0] / max(v[1], 1), 3) for k, v in stage_times(lib).items() if k.startswith("march")}
+        return {"ms_per_step": round(ms, 3), "mrays_per_s": round(W * H / ms / 1e3, 2), "stages_ms": stages}
+
+    try:
+        plain = measure(False)
+        stored = measure(True)
+        replay = vol_marcher.make_replay(W, H, torch.float32, dev, steps=max_large_steps)
+        stored["replay_store"] = None if replay is None else {"large_steps": replay.steps, "bytes": replay.buffer.numel()}
+        del replay
     finally:
+        vol_marcher.DEFAULT_OPTIONS.clear()
         vol_marcher.DEFAULT_OPTIONS.update(saved)
-    replay = vol_marcher.make_replay(W, H, torch.float32, dev, steps=max_large_steps)
     out = {"workload": f"volumetric marcher fwd+bwd, {scene.n} Gaussians SH3, {W}x{H} rays, <= {max_large_steps * 127} bins per ray, L1",
-           "ms_per_step": round(ms, 3), "mrays_per_s": round(W * H / ms / 1e3, 2), "stages_ms": stages,
-           "replay_store": None if replay is None else {"large_steps": replay.steps, "bytes": replay.buffer.numel(),
-                                                         "note": "the forward keeps each pixel's bins for the backward"}}
-    del replay
+           **plain, "with_replay_store": dict(stored, note="opt-in: the forward keeps each pixel's bins for the backward")}
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O

@@ -29,8 +29,12 @@ DEFAULT_TILING = ((16, 16), 256)
 # sample sections for gradient checks (DGMR_TORCH_GRAD_CHECK_CONST_SAMPLES)
 # and the reference's debug render modes (ForwardData::debug_render_mode,
 # debug_render_bin, max_depth; vol_marcher.h:44-55), forward only
+# and the replay store (off by default: the backward recomputes every bin, as
+# the reference's does; "replay": True makes VolMarcher.apply keep the
+# forward's bins for its backward, "replay_steps" large steps of them, 0 =
+# the large-step cap or 4, within "replay_max_bytes")
 DEFAULT_OPTIONS = {"max_large_steps": 0, "frozen": None, "bins_per_chunk": 0, "render_mode": 0, "debug_render_bin": -1,
-                   "max_depth": 20.0, "replay_steps": 0, "replay_max_bytes": 16 << 30}
+                   "max_depth": 20.0, "replay": False, "replay_steps": 0, "replay_max_bytes": 16 << 30}
 RENDER_MODES = {"full": 0, "bins": 1, "depth": 2}
 
 
@@ -215,7 +219,7 @@ class VolMarcher(torch.autograd.Function):
                 bg_color: torch.Tensor, image_width: int, image_height: int, tanfovx: float, tanfovy: float, active_sh_degree: int):
         # the replay store lives beside the reference's cache, for this call's backward only
         ctx.replay = None
-        if any(ctx.needs_input_grad[:5]) and int(DEFAULT_OPTIONS["render_mode"]) == 0:
+        if DEFAULT_OPTIONS.get("replay") and any(ctx.needs_input_grad[:5]) and int(DEFAULT_OPTIONS["render_mode"]) == 0:
             ctx.replay = make_replay(int(image_width), int(image_height), means3D.dtype, means3D.device)
         cache = cpp_binding.forward(shs, opacity, means3D, scales, rotations, world_view_transform, full_proj_transform, camera_center,
                                     bg_color, image_width, image_height, tanfovx, tanfovy, active_sh_degree, replay=ctx.replay)

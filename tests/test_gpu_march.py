@@ -266,11 +266,20 @@ def test_volmarcher_autograd_and_large_step_cap():
                    t(np.zeros(3, np.float32)), cam.width, cam.height, cam.tan_fovx, cam.tan_fovy, 3)
     img.sum().backward()
     assert all(p.grad is not None and torch.isfinite(p.grad).all() for p in params)
-    # autograd keeps a replay store between its forward and backward; the
-    # gradients are the recomputing backward's
-    plain = run_gpu(scene, cam, np.zeros(3, np.float32), torch.float32, grad_fb=np.ones((3, cam.height, cam.width), np.float32))
-    for p, name in zip(params, ("sh", "opacity", "means", "scales", "rotations")):
-        want = plain["grads"][name].reshape(p.shape)
+    # with the replay store turned on, autograd keeps one between its forward
+    # and backward; the gradients are the recomputing backward's
+    grads = [p.grad.cpu().numpy() for p in params]
+    saved = dict(vm.DEFAULT_OPTIONS)
+    try:
+        vm.DEFAULT_OPTIONS["replay"] = True
+        for p in params:
+            p.grad = None
+        img = vm.apply(*params, t(cam.world_view_transform), t(cam.full_proj_transform), t(cam.camera_center),
+                       t(np.zeros(3, np.float32)), cam.width, cam.height, cam.tan_fovx, cam.tan_fovy, 3)
+        img.sum().backward()
+    finally:
+        vm.DEFAULT_OPTIONS.update(saved)
+    for p, want, name in zip(params, grads, ("sh", "opacity", "means", "scales", "rotations")):
         assert np.abs(p.grad.cpu().numpy() - want).max() <= 2e-5 * np.abs(want).max(), name
     # a cap of 2 large steps (<= 254 bins per ray) matches the oracle under the same cap
     bg = np.zeros(3, np.float32)
