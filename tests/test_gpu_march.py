@@ -437,3 +437,41 @@ def test_replay_store_after_a_debug_render_keeps_nothing():
     bg = np.array([0.2, 0.3, 0.4], np.float32)
     out = run_gpu(scene, cam, bg, torch.float32, replay_steps=2, render_mode=1)
     assert (out["tile_steps"] == 0xFFFFFFFF).all()
+
+
+def test_lists_ending_in_negative_depths():
+    """Large Gaussians just in front of the camera have a negative depth
+    (distance minus 1.5 of their largest scale), and their float32 keys sort
+    after every positive one: a tile whose list ends with one takes the
+    section pass's full walk (a Gaussian that close covers every tile here),
+    where the other tests' tiles visit only the wave's candidates.  It holds
+    the float-placed FP64 bar."""
+    torch = _torch()
+    scene, cam = small_scene(n=2000, w=48, h=32, seed=12)
+    rng = np.random.default_rng(12)
+    near = 12
+    c = np.asarray(cam.camera_center, np.float64)
+    fwd = -c / np.linalg.norm(c)
+    pts = c + fwd * rng.uniform(0.45, 0.7, near)[:, None] + rng.normal(scale=0.08, size=(near, 3))
+    scene.means[:near] = pts.astype(np.float32)
+    scene.scales[:near] = 0.45
+    scene.opacity[:near] = 0.3
+    depth = np.linalg.norm(scene.means[:near] - c, axis=1) - 1.5 * 0.45
+    assert (depth < 0).any()
+    bg = np.array([0.2, 0.3, 0.4], np.float32)
+    out = run_gpu(scene, cam, bg, torch.float32)
+    ref32 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=False)
+    check_binning(out, ref32)
+    depths = ref32.geometry()["depth"]
+    pl = out["point_list"].view(np.uint32)
+    ranges = out["ranges"].view(np.uint32)
+    ends = [depths[pl[e - 1]] for s_, e in ranges if e > s_]
+    assert any(d < 0 for d in ends)
+    ref64 = O.MarchOracle(O.scene_arrays(scene), cam, dbl=True, float_binning=True)
+    fb, T, _, amb = ref64.render(bg, with_ambiguity=True, float_sampling=True)
+    ok = amb == 0
+    err = np.abs(out["framebuffer"] - fb).max(axis=0)
+    print(f"flagged {(~ok).mean():.4f}, max (unflagged) {err[ok].max():.3g}")
+    assert (~ok).mean() <= 0.05
+    assert err[ok].max() <= IMG_TOL
+    assert np.abs(out["remaining"] - T)[ok].max() <= IMG_TOL
