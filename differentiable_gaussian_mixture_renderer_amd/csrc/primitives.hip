@@ -231,15 +231,22 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __re
         const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
         const bool valid = i < count;
         const uint32_t digit = (keys[r] >> shift) & mask;
-        uint64_t peers = __ballot(valid);
+        const uint64_t valid_lanes = __ballot(valid);
+        // the lanes whose digit equals this lane's, bit by bit: with s = 0 or
+        // ~0 the sign extension of this lane's bit b, the lanes agreeing on
+        // bit b are ~(ballot(bit b) ^ s), one three-input bit operation per
+        // half of the mask (half the instructions of selecting m or ~m)
+        uint32_t peers_lo = uint32_t(valid_lanes), peers_hi = uint32_t(valid_lanes >> 32);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
             if (b < bits) {
-                const bool bit = (digit >> b) & 1u;
-                const uint64_t m = __ballot(bit);
-                peers &= bit ? m : ~m;
+                const uint32_t s = uint32_t(int32_t(digit << (31 - b)) >> 31);
+                const uint64_t m = __ballot(s != 0u);
+                peers_lo &= ~(uint32_t(m) ^ s);
+                peers_hi &= ~(uint32_t(m >> 32) ^ s);
             }
         }
+        const uint64_t peers = (uint64_t(peers_hi) << 32) | peers_lo;
         // the wave's LDS operations run in order: every lane reads the running
         // count before the lowest lane of its digit group writes it back
         const uint32_t before = s_count[wave][digit];
