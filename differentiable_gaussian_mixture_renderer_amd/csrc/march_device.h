@@ -23,14 +23,43 @@ struct Section {
     S start, end, dt;
 };
 
+// Storage of a pixel's sections: in the thread's own array (private memory
+// once indexed at run time), or in LDS, where thread t keeps field f of its
+// section i at base[(3 i + f) NT] (base = the kernel's array + t, so the
+// lanes of a wave touch consecutive words).
+template <class S, int CAP>
+struct SectionRegs {
+    Section<S> d[CAP];
+    __device__ __forceinline__ Section<S> get(int i) const { return d[i]; }
+    __device__ __forceinline__ S start(int i) const { return d[i].start; }
+    __device__ __forceinline__ S end(int i) const { return d[i].end; }
+    __device__ __forceinline__ S dt(int i) const { return d[i].dt; }
+    __device__ __forceinline__ void set(int i, const Section<S>& v) { d[i] = v; }
+};
+
+template <int NT>
+struct SectionLds {
+    float* base;
+    __device__ __forceinline__ Section<float> get(int i) const { return { base[(3 * i) * NT], base[(3 * i + 1) * NT], base[(3 * i + 2) * NT] }; }
+    __device__ __forceinline__ float start(int i) const { return base[(3 * i) * NT]; }
+    __device__ __forceinline__ float end(int i) const { return base[(3 * i + 1) * NT]; }
+    __device__ __forceinline__ float dt(int i) const { return base[(3 * i + 2) * NT]; }
+    __device__ __forceinline__ void set(int i, const Section<float>& v)
+    {
+        base[(3 * i) * NT] = v.start;
+        base[(3 * i + 1) * NT] = v.end;
+        base[(3 * i + 2) * NT] = v.dt;
+    }
+};
+
 // marching_steps::DensityArray (marching_steps.h:35-200): a sorted list of
 // disjoint sections over [start, end) holding the minimum step; when full,
 // end() shrinks to the last section's end.  put() follows the reference's
 // insertion step for step (in-place merge through a small ring buffer),
 // because the boundaries it leaves decide where the bin borders fall.
-template <class S, int CAP>
+template <class S, int CAP, class Store = SectionRegs<S, CAP>>
 struct SectionList {
-    Section<S> data[CAP];
+    Store data;
     int size;
     S start_, end_;
 
@@ -44,7 +73,7 @@ struct SectionList {
     __device__ __forceinline__ int first_ending_after(S v, int from) const
     {
         for (int i = from; i < size; ++i)
-            if (data[i].end > v)
+            if (data.end(i) > v)
                 return i;
         return size;
     }
@@ -71,25 +100,25 @@ struct SectionList {
         const int first = first_ending_after(e.start, 0);
         if (first >= size) {
             if (first < CAP)
-                data[size++] = e;
+                data.set(size++, e);
             else
                 end_ = e.start;
             return;
         }
         int stop = first_ending_after(e.end, first);
-        if (stop < size && data[stop].start < e.end)
+        if (stop < size && data.start(stop) < e.end)
             ++stop;
         Section<S> ring[CAP];
         unsigned ring_r = 0, ring_w = 0;
         int rd = first, wr = first, inserted = 0;
         while (rd < stop + inserted && wr < CAP) {
             if (rd < size)
-                ring[(ring_w++) % CAP] = data[rd++];
+                ring[(ring_w++) % CAP] = data.get(rd++);
             if (ring_r >= ring_w)
                 break;
             const Section<S> head = ring[ring_r % CAP];
             if (head.end <= e.start) {
-                data[wr++] = ring[(ring_r++) % CAP];
+                data.set(wr++, ring[(ring_r++) % CAP]);
                 continue;
             }
             if (head.start >= e.end)
@@ -106,9 +135,9 @@ struct SectionList {
                     e.start = parts[1].start;
                     continue;
                 }
-                data[wr++] = parts[1];
+                data.set(wr++, parts[1]);
             } else {
-                data[wr++] = parts[0];
+                data.set(wr++, parts[0]);
                 if (parts[1].dt == parts[2].dt) {
                     e.start = parts[1].start;
                     continue;
@@ -119,17 +148,17 @@ struct SectionList {
         }
         if (e.end - e.start > 0 && wr < CAP) {
             if (rd < size)
-                ring[(ring_w++) % CAP] = data[rd++];
-            data[wr++] = e;
+                ring[(ring_w++) % CAP] = data.get(rd++);
+            data.set(wr++, e);
         }
         while (ring_w - ring_r && wr < CAP) {
             if (rd < size)
-                ring[(ring_w++) % CAP] = data[rd++];
-            data[wr++] = ring[(ring_r++) % CAP];
+                ring[(ring_w++) % CAP] = data.get(rd++);
+            data.set(wr++, ring[(ring_r++) % CAP]);
         }
         size = wr + size - rd;
         if (size == CAP)
-            end_ = dmin(end_, data[size - 1].end);
+            end_ = dmin(end_, data.end(size - 1));
     }
 };
 
@@ -138,17 +167,20 @@ template <class S, int CAP>
 struct BorderWalk {
     int cur;
     S prev;
-    __device__ __forceinline__ void begin(const SectionList<S, CAP>& sec)
+    template <class L>
+    __device__ __forceinline__ void begin(const L& sec)
     {
         cur = 0;
-        prev = sec.size ? sec.data[0].start : S(0);
+        prev = sec.size ? sec.data.start(0) : S(0);
     }
-    __device__ __forceinline__ S next(const SectionList<S, CAP>& sec)
+    template <class L>
+    __device__ __forceinline__ S next(const L& sec)
     {
         if (sec.size == 0)
             return S(0);
-        const S v = dmin(sec.data[cur].end, prev + sec.data[cur].dt);
-        if (cur < sec.size - 1 && v == sec.data[cur].end)
+        const S end = sec.data.end(cur);
+        const S v = dmin(end, prev + sec.data.dt(cur));
+        if (cur < sec.size - 1 && v == end)
             ++cur;
         prev = v;
         return v;

@@ -61,6 +61,9 @@ class Request:
 class ResidentScene:
     params: list               # sh, opacity, means, scales, rotations on the device
     sh_degree: int
+    # the densest (tile, Gaussian) pairs per pixel a batch of this scene has
+    # had, for admission (at least PAIRS_PER_PIXEL)
+    pairs_per_pixel: float = PAIRS_PER_PIXEL
 
     @property
     def n(self) -> int:
@@ -129,8 +132,8 @@ def l1_grad(img: torch.Tensor, seed: int) -> torch.Tensor:
 def estimate_bytes(scene: ResidentScene, reqs: List[Request]) -> int:
     px = sum(r.pixels for r in reqs)
     n_bwd = sum(r.backward for r in reqs)
-    return (len(reqs) * scene.n * BYTES_PER_VIEW_GAUSSIAN + px * (PAIRS_PER_PIXEL * BYTES_PER_PAIR + 32)
-            + n_bwd * scene.n * BYTES_PER_GRADIENT_GAUSSIAN)
+    return int(len(reqs) * scene.n * BYTES_PER_VIEW_GAUSSIAN + px * (scene.pairs_per_pixel * BYTES_PER_PAIR + 32)
+               + n_bwd * scene.n * BYTES_PER_GRADIENT_GAUSSIAN)
 
 
 @dataclass
@@ -307,6 +310,8 @@ class ContinuousBatchingServer:
     @staticmethod
     def _batched(splatter, sc, batch, n_bwd):
         imgs, _ = splatter.forward(*sc.params, [r.camera for r in batch], sh_degree=sc.sh_degree)
+        # later admissions of this scene count its measured pair density
+        sc.pairs_per_pixel = max(sc.pairs_per_pixel, splatter.n_pairs / max(1, sum(r.pixels for r in batch)))
         grads = [None] * len(batch)
         if n_bwd:
             # the L1 gradient is formed inside the blend backward
