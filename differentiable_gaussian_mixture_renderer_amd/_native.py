@@ -182,6 +182,8 @@ SIGNATURES = {
     "dgmr_l1_loss_forward": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p, c_void_p]),
     "dgmr_l1_loss_backward": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int64, ctypes.c_float, c_void_p, c_void_p, c_void_p]),
     "dgmr_l1_loss_backward_many": (ctypes.c_int, [_P(L1Item), ctypes.c_int32, c_void_p]),
+    "dgmr_scan_workspace_size": (ctypes.c_int, [ctypes.c_int64, _P(ctypes.c_size_t)]),
+    "dgmr_scan_u32": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int32, c_void_p, ctypes.c_size_t, c_void_p]),
     "dgmr_sort_workspace_size": (ctypes.c_int, [ctypes.c_int64, _P(ctypes.c_size_t)]),
     "dgmr_sort_pairs": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, c_void_p,
                                        ctypes.c_size_t, c_void_p]),

@@ -11,12 +11,41 @@
 namespace dgmr {
 
 // --- exclusive scan of uint32 ---------------------------------------------
-// scratch bytes needed to scan n items
-size_t scan_scratch_bytes(int64_t n);
+// One pass over the input with decoupled look-back between tiles (2 048
+// items, or 8 192 from 262 144 items up).  One launch may scan two arrays of
+// the same length.  scratch bytes needed to scan n items of `arrays` arrays:
+size_t scan_scratch_bytes(int64_t n, int arrays = 1);
+// The scan's ticket counter and per-tile status words inside its scratch,
+// which must be zero when the scan starts: the scan clears them with a small
+// launch of its own unless the caller passes cleared = true, having had the
+// kernel that writes the scan's input call scan_clear from a grid of at
+// least n_status threads.
+struct ScanState {
+    uint32_t* ticket;
+    uint64_t* status;
+    int64_t n_status; // status words (0: the scan is one tile and keeps no state)
+};
+ScanState scan_state(void* scratch, int64_t n, int arrays = 1);
+// called by every thread of the producer's grid, before the scan's launch
+__device__ __forceinline__ void scan_clear(const ScanState& r)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0)
+        *r.ticket = 0;
+    if (i < r.n_status)
+        r.status[i] = 0;
+}
 // out[i] = sum(in[0..i-1]); *total (device) = sum(in) when total != nullptr.
 // inclusive == true gives out[i] = sum(in[0..i]).  in may alias out.
+struct ScanArray {
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t* total;
+    bool inclusive;
+};
+hipError_t scan_u32_arrays(const ScanArray* arrays, int count, int64_t n, void* scratch, hipStream_t stream, bool cleared = false);
 hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n, bool inclusive, void* scratch,
-    hipStream_t stream);
+    hipStream_t stream, bool cleared = false);
 
 // --- stable LSD radix sort of (uint32 or uint16 key, uint32 value) pairs --
 // Sorts bits [begin_bit, end_bit) in 8-bit digits.  Pass p reads buffer

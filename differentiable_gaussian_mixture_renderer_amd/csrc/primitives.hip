@@ -10,6 +10,10 @@ namespace {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;                       // per thread
 constexpr int kScanTile = kScanThreads * kScanItems; // 2048 items per block
+// scans of 128 small tiles or more take 8 192-item tiles: a quarter of the
+// tiles shortens the look-back (a 1M-item scan has 123 tiles, not 489)
+constexpr int kScanItemsLarge = 32;
+constexpr int64_t kScanLargeFrom = int64_t(128) * kScanTile;
 
 // inclusive scan of one value per thread across a 256-thread block
 __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s_waves, uint32_t* block_total)
@@ -38,26 +42,8 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s
     return v + prefix;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n, uint32_t* __restrict__ sums)
-{
-    __shared__ uint32_t s_waves[kScanThreads / 64];
-    const int64_t base = int64_t(blockIdx.x) * kScanTile;
-    uint32_t acc = 0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        const int64_t i = base + int64_t(j) * kScanThreads + threadIdx.x;
-        if (i < n)
-            acc += in[i];
-    }
-    uint32_t total;
-    block_inclusive_scan(acc, s_waves, &total);
-    if (threadIdx.x == 0)
-        sums[blockIdx.x] = total;
-}
-
-// scans one tile of 2048 items with a carried-in offset from `offsets` (or 0)
-__global__ __launch_bounds__(kScanThreads) void k_scan_tile(const uint32_t* in, uint32_t* out, int64_t n, const uint32_t* __restrict__ offsets,
-    uint32_t* total_out, int inclusive)
+// scans the one tile of a scan of at most 2048 items
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_out, int inclusive)
 {
     __shared__ uint32_t s_waves[kScanThreads / 64];
     const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
@@ -70,7 +56,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tile(const uint32_t* in, 
     }
     uint32_t total;
     const uint32_t incl = block_inclusive_scan(acc, s_waves, &total);
-    uint32_t run = incl - acc + (offsets ? offsets[blockIdx.x] : 0u);
+    uint32_t run = incl - acc;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const uint32_t x = v[j];
@@ -78,46 +64,225 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tile(const uint32_t* in, 
             out[base + j] = inclusive ? run + x : run;
         run += x;
     }
-    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
-        *total_out = (offsets ? offsets[blockIdx.x] : 0u) + total;
+    if (total_out && threadIdx.x == 0)
+        *total_out = total;
+}
+
+// Device-wide scans run in one pass with decoupled look-back (Merrill and
+// Garland, "Single-pass Parallel Prefix Scan with Decoupled Look-back",
+// 2016): each block takes the next tile index from a ticket counter, so the
+// tiles before it belong to blocks that are already running; it publishes its
+// tile sum as an aggregate, then learns the sum of everything before it from
+// its predecessors' published words and publishes its inclusive prefix.  The
+// status words are read and written with device-scope atomics (the XCDs' L2
+// caches are not coherent with each other), and each word carries its whole
+// state, so no other memory needs ordering.
+constexpr uint32_t kLookAggregate = 1u;
+constexpr uint32_t kLookPrefix = 2u;
+// a predecessor publishes within microseconds of starting; the bound only
+// keeps a protocol error from turning into a hang (about a second of polling)
+constexpr int kLookMaxSpins = 1 << 20;
+
+__device__ __forceinline__ void look_publish(uint64_t* slot, uint32_t flag, uint32_t value)
+{
+    __hip_atomic_store(slot, (uint64_t(flag) << 32) | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of the tiles before tile v, called by all 64 lanes of one wave: lane l
+// reads the word of tile j - l, 64 predecessors a round, and the round ends
+// the walk when it holds an inclusive prefix (the nearest one counts, with the
+// aggregates between it and tile v).
+__device__ uint32_t look_back(const uint64_t* status, int64_t v)
+{
+    const unsigned lane = lane_id();
+    uint32_t prefix = 0;
+    for (int64_t j = v - 1; j >= 0; j -= 64) {
+        const int64_t idx = j - int64_t(lane);
+        uint64_t s = uint64_t(kLookPrefix) << 32; // before tile 0: a prefix of 0
+        if (idx >= 0) {
+            for (int spin = 0; spin < kLookMaxSpins; ++spin) {
+                s = __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((s >> 32) != 0)
+                    break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        const uint64_t pre = __ballot(uint32_t(s >> 32) == kLookPrefix);
+        const unsigned nearest = pre ? unsigned(__ffsll((unsigned long long)pre) - 1) : 63u;
+        uint32_t part = lane <= nearest ? uint32_t(s) : 0u;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1)
+            part += __shfl_xor(part, d, 64);
+        prefix += part;
+        if (pre)
+            break;
+    }
+    return prefix;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_clear(ScanState r) { scan_clear(r); }
+
+// ITEMS consecutive items of one thread, with 16-byte loads when they are
+// all in range and the array is 16-byte aligned
+template <int ITEMS>
+__device__ __forceinline__ uint32_t load_items(const uint32_t* in, int64_t base, int64_t n, bool vec, uint32_t (&x)[ITEMS])
+{
+    uint32_t acc = 0;
+    if (vec && base + ITEMS <= n) {
+        const uint4* src = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (int j = 0; j < ITEMS / 4; ++j) {
+            const uint4 q = src[j];
+            x[4 * j] = q.x;
+            x[4 * j + 1] = q.y;
+            x[4 * j + 2] = q.z;
+            x[4 * j + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+            x[j] = (base + j < n) ? in[base + j] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+        acc += x[j];
+    return acc;
+}
+
+template <int ITEMS>
+__device__ __forceinline__ void store_items(uint32_t* out, int64_t base, int64_t n, bool vec, const uint32_t (&x)[ITEMS], uint32_t run,
+    bool inclusive)
+{
+    uint32_t y[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        y[j] = inclusive ? run + x[j] : run;
+        run += x[j];
+    }
+    if (vec && base + ITEMS <= n) {
+        uint4* dst = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+        for (int j = 0; j < ITEMS / 4; ++j)
+            dst[j] = make_uint4(y[4 * j], y[4 * j + 1], y[4 * j + 2], y[4 * j + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+            if (base + j < n)
+                out[base + j] = y[j];
+    }
+}
+
+// One tile of kScanThreads * ITEMS items of each of ARRAYS arrays per block,
+// in ticket order.  Array k's status words are r.status[k * tiles + tile]:
+// both aggregates go out before either look-back, so the two walks overlap.
+template <int ITEMS, int ARRAYS>
+__global__ __launch_bounds__(kScanThreads) void k_scan_onepass(ScanArray a0, ScanArray a1, int64_t n, ScanState r, int vec)
+{
+    constexpr int64_t kTile = int64_t(kScanThreads) * ITEMS;
+    __shared__ uint32_t s_waves[kScanThreads / 64];
+    __shared__ uint32_t s_tile, s_prefix[ARRAYS];
+    if (threadIdx.x == 0)
+        s_tile = atomicAdd(r.ticket, 1u);
+    __syncthreads();
+    const int64_t v = s_tile, tiles = gridDim.x;
+    const int64_t base = v * kTile + int64_t(threadIdx.x) * ITEMS;
+    const ScanArray arr[2] = { a0, a1 };
+    uint32_t x[ARRAYS][ITEMS], acc[ARRAYS], incl[ARRAYS], total[ARRAYS];
+#pragma unroll
+    for (int k = 0; k < ARRAYS; ++k)
+        acc[k] = load_items<ITEMS>(arr[k].in, base, n, vec != 0, x[k]);
+#pragma unroll
+    for (int k = 0; k < ARRAYS; ++k)
+        incl[k] = block_inclusive_scan(acc[k], s_waves, &total[k]);
+    if (threadIdx.x < 64) {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < ARRAYS; ++k)
+                look_publish(r.status + k * tiles + v, v == 0 ? kLookPrefix : kLookAggregate, total[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < ARRAYS; ++k) {
+            const uint32_t before = v == 0 ? 0u : look_back(r.status + k * tiles, v);
+            if (threadIdx.x == 0) {
+                if (v > 0)
+                    look_publish(r.status + k * tiles + v, kLookPrefix, before + total[k]);
+                s_prefix[k] = before;
+                if (arr[k].total && v == tiles - 1)
+                    *arr[k].total = before + total[k];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ARRAYS; ++k)
+        store_items<ITEMS>(arr[k].out, base, n, vec != 0, x[k], incl[k] - acc[k] + s_prefix[k], arr[k].inclusive);
 }
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 } // namespace
 
-size_t scan_scratch_bytes(int64_t n)
+static int64_t scan_tile_items(int64_t n) { return n >= kScanLargeFrom ? int64_t(kScanThreads) * kScanItemsLarge : kScanTile; }
+
+size_t scan_scratch_bytes(int64_t n, int arrays)
 {
-    size_t bytes = 0;
-    int64_t m = ceil_div(n, kScanTile);
-    while (m > 1) {
-        bytes += size_t(m) * sizeof(uint32_t) + 256;
-        m = ceil_div(m, kScanTile);
-    }
-    return bytes + 256;
+    // the ticket (padded to 256 bytes), then one status word per tile and array
+    const int64_t tiles = ceil_div(std::max<int64_t>(n, 1), scan_tile_items(n));
+    return 256 + ((size_t(tiles) * arrays * sizeof(uint64_t) + 255) / 256) * 256;
 }
 
-hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n, bool inclusive, void* scratch, hipStream_t stream)
+ScanState scan_state(void* scratch, int64_t n, int arrays)
 {
+    ScanState r;
+    r.ticket = static_cast<uint32_t*>(scratch);
+    r.status = reinterpret_cast<uint64_t*>(static_cast<char*>(scratch) + 256);
+    r.n_status = n > kScanTile ? ceil_div(n, scan_tile_items(n)) * arrays : 0;
+    return r;
+}
+
+hipError_t scan_u32_arrays(const ScanArray* arrays, int count, int64_t n, void* scratch, hipStream_t stream, bool cleared)
+{
+    if (count < 1 || count > 2)
+        return hipErrorInvalidValue;
     if (n <= 0) {
-        if (total)
-            return hipMemsetAsync(total, 0, sizeof(uint32_t), stream);
+        for (int k = 0; k < count; ++k)
+            if (arrays[k].total) {
+                const hipError_t e = hipMemsetAsync(arrays[k].total, 0, sizeof(uint32_t), stream);
+                if (e != hipSuccess)
+                    return e;
+            }
         return hipSuccess;
     }
-    const int64_t blocks = ceil_div(n, kScanTile);
-    if (blocks == 1) {
-        k_scan_tile<<<1, kScanThreads, 0, stream>>>(in, out, n, nullptr, total, inclusive ? 1 : 0);
+    if (n <= kScanTile) {
+        for (int k = 0; k < count; ++k)
+            k_scan_tile<<<1, kScanThreads, 0, stream>>>(arrays[k].in, arrays[k].out, n, arrays[k].total, arrays[k].inclusive ? 1 : 0);
         return hipGetLastError();
     }
-    // reduce tiles, scan the tile sums recursively (exclusive), then rescan tiles with offsets
-    uint32_t* sums = static_cast<uint32_t*>(scratch);
-    void* rest = static_cast<char*>(scratch) + ((size_t(blocks) * sizeof(uint32_t) + 255) / 256) * 256;
-    k_scan_reduce<<<unsigned(blocks), kScanThreads, 0, stream>>>(in, n, sums);
-    hipError_t e = scan_u32(sums, sums, nullptr, blocks, false, rest, stream);
-    if (e != hipSuccess)
-        return e;
-    k_scan_tile<<<unsigned(blocks), kScanThreads, 0, stream>>>(in, out, n, sums, total, inclusive ? 1 : 0);
+    const ScanState r = scan_state(scratch, n, count);
+    if (!cleared)
+        k_scan_clear<<<unsigned(ceil_div(r.n_status, kScanThreads)), kScanThreads, 0, stream>>>(r);
+    int vec = 1;
+    for (int k = 0; k < count; ++k)
+        vec &= (reinterpret_cast<uintptr_t>(arrays[k].in) % 16 == 0 && reinterpret_cast<uintptr_t>(arrays[k].out) % 16 == 0) ? 1 : 0;
+    const ScanArray a0 = arrays[0], a1 = count > 1 ? arrays[1] : arrays[0];
+    const unsigned tiles = unsigned(ceil_div(n, scan_tile_items(n)));
+    const bool large = n >= kScanLargeFrom;
+    if (large && count == 1)
+        k_scan_onepass<kScanItemsLarge, 1><<<tiles, kScanThreads, 0, stream>>>(a0, a1, n, r, vec);
+    else if (large)
+        k_scan_onepass<kScanItemsLarge, 2><<<tiles, kScanThreads, 0, stream>>>(a0, a1, n, r, vec);
+    else if (count == 1)
+        k_scan_onepass<kScanItems, 1><<<tiles, kScanThreads, 0, stream>>>(a0, a1, n, r, vec);
+    else
+        k_scan_onepass<kScanItems, 2><<<tiles, kScanThreads, 0, stream>>>(a0, a1, n, r, vec);
     return hipGetLastError();
+}
+
+hipError_t scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n, bool inclusive, void* scratch, hipStream_t stream,
+    bool cleared)
+{
+    const ScanArray a { in, out, total, inclusive };
+    return scan_u32_arrays(&a, 1, n, scratch, stream, cleared);
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -143,13 +308,15 @@ __device__ __forceinline__ int64_t sort_count(int64_t n_max, const uint32_t* n_d
 }
 
 // per-block histogram of the digit (key >> shift) & (2^bits - 1), stored
-// digit-major: hist[digit * n_blocks + block]
+// digit-major: hist[digit * n_blocks + block]; also clears the state of the
+// scan over hist that follows
 template <int ITEMS, class KeyT>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const KeyT* __restrict__ keys, int64_t n_max, const uint32_t* n_dev,
-    int shift, int bits, int n_blocks, uint32_t* __restrict__ hist)
+    int shift, int bits, int n_blocks, uint32_t* __restrict__ hist, ScanState scan)
 {
     constexpr int kSortItems = ITEMS, kSortTile = kSortThreads * ITEMS;
     __shared__ uint32_t s_hist[4][kRadixMax];
+    scan_clear(scan);
     const int64_t n = sort_count(n_max, n_dev);
     const int64_t base = int64_t(blockIdx.x) * kSortTile;
     const uint32_t mask = (1u << bits) - 1u;
@@ -430,8 +597,15 @@ template <int ITEMS, class KeyT>
 static void radix_pass(int blocks, const KeyT* kin, const uint32_t* vin, KeyT* kout, uint32_t* vout, int64_t n_max, const uint32_t* n_dev,
     int shift, int bits, uint32_t* hist, void* scan_scratch, KeyOut ko, hipStream_t stream, hipError_t* err)
 {
-    k_radix_hist<ITEMS, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist);
-    *err = scan_u32(hist, hist, nullptr, int64_t(blocks) << bits, false, scan_scratch, stream);
+    const int64_t n_hist = int64_t(blocks) << bits;
+    // the histogram grid clears the scan's state (it has a thread per status word and more)
+    const ScanState scan = scan_state(scan_scratch, n_hist);
+    if (scan.n_status > int64_t(blocks) * kSortThreads) {
+        *err = hipErrorInvalidValue;
+        return;
+    }
+    k_radix_hist<ITEMS, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist, scan);
+    *err = scan_u32(hist, hist, nullptr, n_hist, false, scan_scratch, stream, true);
     if (*err != hipSuccess)
         return;
     if (bits <= 7)

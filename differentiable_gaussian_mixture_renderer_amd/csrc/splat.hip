@@ -122,8 +122,10 @@ template <class S, bool RECTS>
 __global__ __launch_bounds__(256) void k_gather_owners(int64_t n, const uint32_t* __restrict__ n_sorted, const uint32_t* __restrict__ ids,
     const uint2* __restrict__ bin_rects,
     const int32_t* __restrict__ rects, const uint32_t* __restrict__ tiles, const S* __restrict__ xy, int grid_x, int grid_y, int tile_w,
-    int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ counts, uint2* __restrict__ owner_rects)
+    int tile_h, const int4* __restrict__ views, int64_t n_per_view, uint32_t* __restrict__ counts, uint2* __restrict__ owner_rects,
+    ScanState scan)
 {
+    scan_clear(scan); // for the scan of counts that follows
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n)
         return;
@@ -185,8 +187,9 @@ __device__ __forceinline__ void store_run(uint16_t* keys, uint32_t* vals, const 
 // (compacted sorts) or whose final sort key is culled_key.
 __global__ __launch_bounds__(256) void k_gather_reach(int64_t n, const uint32_t* __restrict__ n_sorted, const uint32_t* __restrict__ keys,
     uint32_t culled_key, const uint32_t* __restrict__ ids, const ReachRec* __restrict__ reach, int tile_w, int tile_h,
-    uint32_t* __restrict__ counts, ReachRec* __restrict__ owners)
+    uint32_t* __restrict__ counts, ReachRec* __restrict__ owners, ScanState scan)
 {
+    scan_clear(scan); // for the scan of counts that follows
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n)
         return;
@@ -656,7 +659,7 @@ static SplatWorkspace carve_splat(Carver& cv, int64_t n, int64_t cap, bool cull)
     w.owner_reach = reinterpret_cast<ReachRec*>(cv.take<char>(nn * rec));
     w.owner_rects = reinterpret_cast<uint2*>(w.owner_reach);
     w.sort_scratch = cv.take<char>(radix_scratch_bytes(std::max(nn, cc)));
-    w.scan_scratch = cv.take<char>(scan_scratch_bytes(nn));
+    w.scan_scratch = cv.take<char>(scan_scratch_bytes(nn, 2));
     return w;
 }
 
@@ -840,23 +843,26 @@ int bin_gaussians(int64_t n, const int32_t* rects, const uint32_t* tiles, const 
     const unsigned blocks_n = unsigned(ceil_div64(n, 256));
     {
         StageScope sc("bin.scan", st, 44.0 * double(n));
+        // the pair offsets of the depth order and, when asked for, the
+        // reference cache's inclusive scan of tiles in Gaussian order: one
+        // launch, its state cleared by the gather (one thread per item)
+        const ScanArray scans[2] = { { w.counts, w.pair_offsets, w.counter, false }, { tiles, bins->point_offsets, nullptr, true } };
+        const int n_scans = bins->point_offsets ? 2 : 1;
+        const ScanState scan = scan_state(w.scan_scratch, n, n_scans);
         if (cull)
             k_gather_reach<<<blocks_n, 256, 0, st>>>(n, n_sorted, K, batch && batch->n_views > 1 ? uint32_t(batch->n_views) : 0xffffffffu,
-                sorted_ids, w.reach, tile_w, tile_h, w.counts, w.owner_reach);
+                sorted_ids, w.reach, tile_w, tile_h, w.counts, w.owner_reach, scan);
         else if (bin_rects_written)
             k_gather_owners<float, true><<<blocks_n, 256, 0, st>>>(n, n_sorted, sorted_ids, w.bin_rects, rects, tiles, static_cast<const float*>(xy),
-                grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
+                grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects, scan);
         else if (xy_double)
             k_gather_owners<double, false><<<blocks_n, 256, 0, st>>>(n, n_sorted, sorted_ids, w.bin_rects, rects, tiles,
-                static_cast<const double*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
+                static_cast<const double*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects, scan);
         else
             k_gather_owners<float, false><<<blocks_n, 256, 0, st>>>(n, n_sorted, sorted_ids, w.bin_rects, rects, tiles,
-                static_cast<const float*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects);
+                static_cast<const float*>(xy), grid_x, grid_y, tile_w, tile_h, views, n_per_view, w.counts, w.owner_rects, scan);
         DGMR_HIP_CHECK(hipGetLastError());
-        DGMR_HIP_CHECK(scan_u32(w.counts, w.pair_offsets, w.counter, n, false, w.scan_scratch, st));
-        // the reference's cache keeps the inclusive scan in Gaussian order
-        if (bins->point_offsets)
-            DGMR_HIP_CHECK(scan_u32(tiles, bins->point_offsets, nullptr, n, true, w.scan_scratch, st));
+        DGMR_HIP_CHECK(scan_u32_arrays(scans, n_scans, n, w.scan_scratch, st, true));
     }
     // R stays on the device: the copy to the host is awaited by the caller
     const int rc_count = enqueue_count(w.counter, st, pending);
@@ -1025,6 +1031,23 @@ int dgmr_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
         DGMR_HIP_CHECK(hipMemcpyAsync(keys, keys_tmp, size_t(n) * 4, hipMemcpyDeviceToDevice, st));
         DGMR_HIP_CHECK(hipMemcpyAsync(vals, vals_tmp, size_t(n) * 4, hipMemcpyDeviceToDevice, st));
     }
+    return DGMR_OK;
+}
+
+int dgmr_scan_workspace_size(int64_t n, size_t* bytes)
+{
+    DGMR_ARG_CHECK(bytes && n >= 0, "dgmr_scan_workspace_size: n out of range");
+    *bytes = scan_scratch_bytes(n);
+    return DGMR_OK;
+}
+
+int dgmr_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n, int32_t inclusive, void* ws, size_t ws_bytes,
+    dgmr_stream stream)
+{
+    DGMR_ARG_CHECK(n >= 0, "dgmr_scan_u32: n out of range");
+    DGMR_ARG_CHECK(n == 0 || (in && out && ws), "dgmr_scan_u32: null buffer");
+    DGMR_ARG_CHECK(ws_bytes >= scan_scratch_bytes(n), "dgmr_scan_u32: workspace too small");
+    DGMR_HIP_CHECK(scan_u32(in, out, total, n, inclusive != 0, ws, reinterpret_cast<hipStream_t>(stream)));
     return DGMR_OK;
 }
 

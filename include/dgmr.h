@@ -342,6 +342,18 @@ int dgmr_sort_workspace_size(int64_t n, size_t* bytes);
 int dgmr_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, int64_t n, int32_t begin_bit, int32_t end_bit,
     void* ws, size_t ws_bytes, dgmr_stream stream);
 
+/* ------------------------------------------------------------ scan
+ * Prefix sum of n uint32 values (device), the primitive under every binning
+ * offset (replaces cub::DeviceScan::InclusiveSum, splat_forward.cu:127-137,
+ * vol_marcher_forward.cu:185-189, vol_raster_forward.cu:146-149).  inclusive
+ * != 0: out[i] = in[0] + ... + in[i]; 0: out[i] = in[0] + ... + in[i-1].
+ * Sums wrap modulo 2^32.  total (device, may be null) receives the sum of all
+ * n.  in may equal out.  ws (device) holds at least dgmr_scan_workspace_size(n)
+ * bytes; its contents need no initialisation. */
+int dgmr_scan_workspace_size(int64_t n, size_t* bytes);
+int dgmr_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n, int32_t inclusive, void* ws, size_t ws_bytes,
+    dgmr_stream stream);
+
 /* No counterpart in the reference: dgmr_sort_pairs for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  The same sort, with the scratch that

@@ -72,3 +72,40 @@ def test_sort_pairs_matches_stable_argsort(n, begin, end):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(v.cpu().numpy().view(np.uint32), order.astype(np.uint32))
     np.testing.assert_array_equal(k.cpu().numpy().view(np.uint32), keys[order])
+
+
+@pytest.mark.parametrize("n,inclusive", [(1, 0), (2048, 1), (2049, 0), (131072, 1), (1000003, 0), (1000003, 1), (262143, 1), (262144, 0), (12_000_001, 0)])
+def test_scan_matches_cumsum(n, inclusive):
+    """Bit-exact against numpy's cumulative sum modulo 2^32.  2 049 items take
+    two tiles of 2 048; from 262 144 items the tiles hold 8 192; 12M take
+    1 465 tiles, more than the chip runs at once, so later tiles look back
+    across blocks that have already finished.  Values
+    up to 2^31 make the running sum wrap.  The workspace is filled with 0xff
+    first: the scan must not depend on its contents."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import torch
+    from differentiable_gaussian_mixture_renderer_amd import _native as N
+    lib = N.load()
+    rng = np.random.default_rng(n + inclusive)
+    x = rng.integers(0, 2**31, size=n, dtype=np.uint64).astype(np.uint32)
+    x[rng.random(n) < 0.3] = 0
+    c = np.cumsum(x.astype(np.uint64)) & np.uint64(0xFFFFFFFF)
+    want = c if inclusive else np.concatenate([[0], c[:-1]]).astype(np.uint64)
+    dev = torch.device("cuda:0")
+    inp = torch.tensor(x.view(np.int32), device=dev)
+    out = torch.full_like(inp, -1)
+    total = torch.full((1,), -1, dtype=torch.int32, device=dev)
+    nbytes = ctypes.c_size_t(0)
+    assert lib.dgmr_scan_workspace_size(n, ctypes.byref(nbytes)) == N.DGMR_OK
+    ws = torch.full((max(nbytes.value, 1),), 255, dtype=torch.uint8, device=dev)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    for rep in range(2):  # a second call reuses the workspace as the first left it
+        assert lib.dgmr_scan_u32(p(inp), p(out), p(total), n, inclusive, p(ws), nbytes.value, None) == N.DGMR_OK
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), want.astype(np.uint32), err_msg=f"call {rep}")
+        assert int(total.cpu().numpy().view(np.uint32)[0]) == int(c[-1])
+    # in place
+    assert lib.dgmr_scan_u32(p(inp), p(inp), None, n, inclusive, p(ws), nbytes.value, None) == N.DGMR_OK
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(inp.cpu().numpy().view(np.uint32), want.astype(np.uint32))
