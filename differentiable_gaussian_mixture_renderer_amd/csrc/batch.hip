@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_batch_preprocess(int64_t n, int n_views
     gauss_mean(p, i, g);
     gauss_rest(p, i, g);
     for (int v = 0; v < n_views; ++v)
-        preprocess_view<true>(g, i, int64_t(v) * n + i, p, s_sh + threadIdx.x * kShStride, cams[v], out);
+        preprocess_view<true, false>(g, i, int64_t(v) * n + i, p, s_sh + threadIdx.x * kShStride, cams[v], out);
 }
 
 __device__ __forceinline__ int find_view(const ViewBlend* views, int n_views, int tile)

@@ -222,6 +222,37 @@ __device__ __forceinline__ void unstage_sh(float* dst, const float* s_sh, int64_
 // preprocess of Gaussian i for one camera (splat_forward.cu:71-118): cull
 // view-z < 0.2, EWA splat (Ots, 0.3 px^2 filter), rect ceil(3 sigma), tiles
 // touched, depth, SH colour, conic.  Writes slot o.  c: this Gaussian's SH row.
+// A Gaussian's SH row of coefficients, element f = 3k + channel for the
+// first kmax coefficients.  A row in global memory (ROW_GLOBAL) is read in
+// one burst of loads issued before any is used: a loop that loads and
+// accumulates a coefficient at a time waits for each load in turn, 16 round
+// trips to memory per Gaussian.  A row staged in LDS is read where it is
+// used, which keeps the 48 values out of the registers.
+template <bool ROW_GLOBAL>
+struct ShRow {
+    float v[ROW_GLOBAL ? 48 : 1];
+    const float* c;
+    __device__ __forceinline__ ShRow(const float* row, int kmax) : c(row)
+    {
+        if constexpr (ROW_GLOBAL) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const bool on = k < kmax;
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch)
+                    v[3 * k + ch] = on ? row[3 * k + ch] : 0.f;
+            }
+        }
+    }
+    __device__ __forceinline__ float operator[](int f) const
+    {
+        if constexpr (ROW_GLOBAL)
+            return v[f];
+        else
+            return c[f];
+    }
+};
+
 // a Gaussian's view-independent preprocess inputs: centre, activated scales,
 // rotation and opacity, 3D covariance and its two largest scales
 struct GaussIn {
@@ -243,21 +274,23 @@ __device__ __forceinline__ void gauss_rest(const SplatParams& p, int64_t i, Gaus
     larger2_values(g.s, &g.sa, &g.sb, &i0, &i1);
 }
 
-template <bool REST_READY>
+template <bool REST_READY, bool ROW_GLOBAL>
 __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o, const SplatParams& p, const float* c, const DevCamera<float>& cam,
     const SplatOut& out);
 
-// one view: g.mu set, the rest computed here after the view-space cull
+// one view: g.mu set, the rest computed here after the view-space cull; c in
+// global memory
 __device__ __forceinline__ void preprocess_gaussian(int64_t i, int64_t o, const SplatParams& p, const float* c, const DevCamera<float>& cam,
     const SplatOut& out)
 {
     GaussIn g;
     gauss_mean(p, i, g);
-    preprocess_view<false>(g, i, o, p, c, cam, out);
+    preprocess_view<false, true>(g, i, o, p, c, cam, out);
 }
 
-// REST_READY: g fully set (a Gaussian seen from many views fills it once)
-template <bool REST_READY>
+// REST_READY: g fully set (a Gaussian seen from many views fills it once);
+// ROW_GLOBAL: c is in global memory, not staged in LDS
+template <bool REST_READY, bool ROW_GLOBAL>
 __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o, const SplatParams& p, const float* c, const DevCamera<float>& cam,
     const SplatOut& out)
 {
@@ -329,11 +362,15 @@ __device__ __forceinline__ void preprocess_view(GaussIn& g, int64_t i, int64_t o
     float Y[16];
     sh_basis(p.sh_degree, dir, Y);
     const int kmax = min(p.sh_coeffs, (p.sh_degree + 1) * (p.sh_degree + 1));
+    const ShRow<ROW_GLOBAL> cf(c, kmax);
     float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-    for (int k = 0; k < kmax; ++k) {
-        r0 = fmaf(Y[k], c[3 * k + 0], r0);
-        r1 = fmaf(Y[k], c[3 * k + 1], r1);
-        r2 = fmaf(Y[k], c[3 * k + 2], r2);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (k < kmax) {
+            r0 = fmaf(Y[k], cf[3 * k + 0], r0);
+            r1 = fmaf(Y[k], cf[3 * k + 1], r1);
+            r2 = fmaf(Y[k], cf[3 * k + 2], r2);
+        }
     }
     r0 += 0.5f;
     r1 += 0.5f;
@@ -793,7 +830,7 @@ enum class ShGrad { Add, Export };
 // scales and rotation): adds into g_mu_acc, g_w_acc, the SH gradient, and the
 // view sums g_cov3_acc, g_num_acc that preprocess_backward_finish turns into
 // scale and rotation gradients once per Gaussian.
-template <ShGrad MODE = ShGrad::Add>
+template <ShGrad MODE = ShGrad::Add, bool ROW_GLOBAL = false>
 __device__ __forceinline__ void preprocess_backward_core(const float* mu, const OtsPre<float>& pre, float w, const SplatParams& p,
     const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_cov3_acc,
     float* g_num_acc, float* g_w_acc, float* g_sh);
@@ -817,7 +854,7 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
     gaussian_params(p, i, s, q, &w);
     const OtsPre<float> pre = ots_pre(s, q);
     float g_cov3[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f }, g_num = 0.f;
-    preprocess_backward_core<MODE>(mu, pre, w, p, c, cam, A, clamped3, sg, g_mu_acc, g_cov3, &g_num, g_w_acc, g_sh);
+    preprocess_backward_core<MODE, true>(mu, pre, w, p, c, cam, A, clamped3, sg, g_mu_acc, g_cov3, &g_num, g_w_acc, g_sh);
     float g_s[3], g_q[4];
     preprocess_backward_finish(s, q, pre, p, g_cov3, g_num, g_s, g_q);
 #pragma unroll
@@ -828,11 +865,19 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
         g_q_acc[a] += g_q[a];
 }
 
-template <ShGrad MODE>
+template <ShGrad MODE, bool ROW_GLOBAL>
 __device__ __forceinline__ void preprocess_backward_core(const float* mu, const OtsPre<float>& pre, float w, const SplatParams& p,
-    const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped3, const float* sg, float* g_mu_acc, float* g_cov3_acc,
+    const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped_in, const float* sg_in, float* g_mu_acc, float* g_cov3_acc,
     float* g_num_acc, float* g_w_acc, float* g_sh)
 {
+    // the inputs are read into registers before any arithmetic (and an SH
+    // row in global memory before the basis is evaluated): read where they
+    // are used, each load would be waited for in turn
+    float sg[9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a)
+        sg[a] = sg_in[a];
+    const bool clamped3[3] = { clamped_in[0] != 0, clamped_in[1] != 0, clamped_in[2] != 0 };
     // conic = cov^-1: dL/dcov = -A G A
     const float G00 = sg[2], G01 = 0.5f * sg[3], G11 = sg[4];
     const float t00 = A.x * G00 + A.y * G01, t01 = A.x * G01 + A.y * G11;
@@ -846,15 +891,16 @@ __device__ __forceinline__ void preprocess_backward_core(const float* mu, const 
     const float inv_len = fast_div(1.f, fast_sqrt(dot3(v, v)));
     const float dir[3] = { v[0] * inv_len, v[1] * inv_len, v[2] * inv_len };
     const float gr[3] = { clamped3[0] ? sg[6] : 0.f, clamped3[1] ? sg[7] : 0.f, clamped3[2] ? sg[8] : 0.f };
+    const int kmax = min(p.sh_coeffs, (p.sh_degree + 1) * (p.sh_degree + 1));
+    const ShRow<ROW_GLOBAL> cf(c, kmax);
     float Y[16], dY[16][3];
     sh_basis(p.sh_degree, dir, Y);
     sh_basis_grad(p.sh_degree, dir, dY);
-    const int kmax = min(p.sh_coeffs, (p.sh_degree + 1) * (p.sh_degree + 1));
     float g_dir[3] = { 0.f, 0.f, 0.f };
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         if (k < kmax) {
-            const float c0 = c[3 * k], c1 = c[3 * k + 1], c2 = c[3 * k + 2];
+            const float c0 = cf[3 * k], c1 = cf[3 * k + 1], c2 = cf[3 * k + 2];
             if constexpr (MODE == ShGrad::Export) {
                 g_sh[k] = Y[k];
             } else {
