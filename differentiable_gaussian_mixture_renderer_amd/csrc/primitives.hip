@@ -312,7 +312,7 @@ __device__ __forceinline__ int64_t sort_count(int64_t n_max, const uint32_t* n_d
 // scan over hist that follows
 template <int ITEMS, class KeyT>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const KeyT* __restrict__ keys, int64_t n_max, const uint32_t* n_dev,
-    int shift, int bits, int n_blocks, uint32_t* __restrict__ hist, ScanState scan)
+    int shift, int bits, int n_blocks, uint32_t* __restrict__ hist, ScanState scan, int vec)
 {
     constexpr int kSortItems = ITEMS, kSortTile = kSortThreads * ITEMS;
     __shared__ uint32_t s_hist[4][kRadixMax];
@@ -325,17 +325,34 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const KeyT* __restr
     __syncthreads();
     if (base < n) {
         const unsigned wave = threadIdx.x / 64;
-        // every load first (all in flight together), then the LDS counts
+        // each thread counts ITEMS consecutive keys (the order of counting
+        // does not matter), read with 16-byte loads when they are all in
+        // range; every load is in flight before the LDS counts
+        const int64_t first = base + int64_t(threadIdx.x) * kSortItems;
         uint32_t k[kSortItems];
+        if (vec && first + kSortItems <= n) {
+            constexpr int kPerVec = 16 / int(sizeof(KeyT));
+            const uint4* src = reinterpret_cast<const uint4*>(keys + first);
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
-            const int64_t i = base + int64_t(j) * kSortThreads + threadIdx.x;
-            k[j] = i < n ? uint32_t(keys[i]) : 0xffffffffu;
-        }
+            for (int v = 0; v < kSortItems / kPerVec; ++v) {
+                const uint4 q = src[v];
+                const uint32_t w[4] = { q.x, q.y, q.z, q.w };
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j)
-            if (base + int64_t(j) * kSortThreads + threadIdx.x < n)
+                for (int e = 0; e < kPerVec; ++e)
+                    k[v * kPerVec + e] = sizeof(KeyT) == 2 ? ((w[e / 2] >> (16 * (e % 2))) & 0xffffu) : w[e];
+            }
+#pragma unroll
+            for (int j = 0; j < kSortItems; ++j)
                 atomicAdd(&s_hist[wave][(k[j] >> shift) & mask], 1u);
+        } else {
+#pragma unroll
+            for (int j = 0; j < kSortItems; ++j)
+                k[j] = first + j < n ? uint32_t(keys[first + j]) : 0u;
+#pragma unroll
+            for (int j = 0; j < kSortItems; ++j)
+                if (first + j < n)
+                    atomicAdd(&s_hist[wave][(k[j] >> shift) & mask], 1u);
+        }
     }
     __syncthreads();
     if (threadIdx.x <= mask) {
@@ -468,7 +485,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __re
         }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < count; q += kSortThreads) {
+    const auto write_item = [&](int q) {
         const KeyT key = s_keys[q];
         const uint32_t d = (uint32_t(key) >> shift) & mask;
         const uint32_t dst = s_global[d] + (uint32_t(q) - s_local[d]);
@@ -478,6 +495,16 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __re
         else // the view of the id replaces the key (culled_view for a culled key 0xffffffff when set)
             keys_out[dst] = KeyT((ko.culled_view != 0xffffffffu && uint32_t(key) == 0xffffffffu) ? ko.culled_view : val / ko.n_per_view);
         vals_out[dst] = val;
+    };
+    if (count == kSortTile) {
+        // a full block: a fixed trip count, unrolled by 8, so the LDS reads
+        // of several items are in flight together
+#pragma unroll 8
+        for (int r = 0; r < kRounds; ++r)
+            write_item(r * kSortThreads + int(threadIdx.x));
+    } else {
+        for (int q = threadIdx.x; q < count; q += kSortThreads)
+            write_item(q);
     }
 }
 
@@ -604,7 +631,8 @@ static void radix_pass(int blocks, const KeyT* kin, const uint32_t* vin, KeyT* k
         *err = hipErrorInvalidValue;
         return;
     }
-    k_radix_hist<ITEMS, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist, scan);
+    const int vec = reinterpret_cast<uintptr_t>(kin) % 16 == 0 ? 1 : 0;
+    k_radix_hist<ITEMS, KeyT><<<blocks, kSortThreads, 0, stream>>>(kin, n_max, n_dev, shift, bits, blocks, hist, scan, vec);
     *err = scan_u32(hist, hist, nullptr, n_hist, false, scan_scratch, stream, true);
     if (*err != hipSuccess)
         return;

@@ -514,7 +514,7 @@ template <int TW, int TH, int B>
 __global__ __launch_bounds__(64 * kBlendWaves) void k_blend_backward(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, int width, int height, int grid_x,
     int n_tiles, float bg0, float bg1, float bg2, const float* __restrict__ framebuffer, const float* __restrict__ final_T,
-    const float* __restrict__ grad_fb, float* __restrict__ screen_grads)
+    const float* __restrict__ grad_fb, float* __restrict__ screen_grads, WavePriority prio)
 {
     // kBwdSplit waves per tile, each on TH / kBwdSplit rows: the tile's list
     // is walked once per part, and the longest tile's work is spread over
@@ -528,6 +528,7 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_blend_backward(const uint3
         return;
     BlendSmemBwd<B>& sm = sm_all[wave];
     const uint32_t first = uniform(ranges[2 * tile]), last = uniform(ranges[2 * tile + 1]);
+    set_wave_priority(last - first, prio);
     blend_tile_backward<TW, TH / SPLIT, B>(sm, first, last, (tile % grid_x) * TW, (tile / grid_x) * TH + part * (TH / SPLIT), width, height,
         bg0, bg1, bg2, point_list, xy, conic_opacity, rgb, framebuffer, final_T, grad_fb, screen_grads);
 }
@@ -686,17 +687,18 @@ static void launch_blend_fwd(int batch, int n_tiles, hipStream_t st, const uint3
 
 template <int TW, int TH>
 static void launch_blend_bwd(int batch, int n_tiles, hipStream_t st, const uint32_t* ranges, const uint32_t* pl, const float2* xy,
-    const float4* co, const float* rgb, int W, int H, int gx, const double* bg, const float* fb, const float* T, const float* gfb, float* sg)
+    const float4* co, const float* rgb, int W, int H, int gx, const double* bg, const float* fb, const float* T, const float* gfb, float* sg,
+    WavePriority prio)
 {
     constexpr int SPLIT = (TW * TH / kBwdSplit >= 64) ? kBwdSplit : 1;
     const dim3 grid(unsigned(ceil_div64(int64_t(n_tiles) * SPLIT, kBlendWaves))), block(64 * kBlendWaves);
     const float b0 = float(bg[0]), b1 = float(bg[1]), b2 = float(bg[2]);
     if (batch == 64)
-        k_blend_backward<TW, TH, 64><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);
+        k_blend_backward<TW, TH, 64><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg, prio);
     else if (batch == 128)
-        k_blend_backward<TW, TH, 128><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);
+        k_blend_backward<TW, TH, 128><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg, prio);
     else
-        k_blend_backward<TW, TH, 256><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg);
+        k_blend_backward<TW, TH, 256><<<grid, block, 0, st>>>(ranges, pl, xy, co, rgb, W, H, gx, n_tiles, b0, b1, b2, fb, T, gfb, sg, prio);
 }
 
 void splat_blend_forward(const dgmr_tiling& t, int n_tiles, hipStream_t st, const uint32_t* ranges, const uint32_t* pl, const float* xy,
@@ -712,17 +714,23 @@ void splat_blend_forward(const dgmr_tiling& t, int n_tiles, hipStream_t st, cons
         launch_blend_fwd<32, 8>(t.batch, n_tiles, st, ranges, pl, xy2, co4, rgb, W, H, gx, bg, fb, T);
 }
 
-void splat_blend_backward(const dgmr_tiling& t, int n_tiles, hipStream_t st, const uint32_t* ranges, const uint32_t* pl, const float* xy,
-    const float* co, const float* rgb, int W, int H, int gx, const double* bg, const float* fb, const float* T, const float* gfb, float* sg)
+void splat_blend_backward(const dgmr_tiling& t, int n_tiles, int64_t n_pairs, hipStream_t st, const uint32_t* ranges, const uint32_t* pl,
+    const float* xy, const float* co, const float* rgb, int W, int H, int gx, const double* bg, const float* fb, const float* T, const float* gfb,
+    float* sg)
 {
     const auto* xy2 = reinterpret_cast<const float2*>(xy);
     const auto* co4 = reinterpret_cast<const float4*>(co);
+    // priority thresholds at 0.55, 0.93 and 1.37 times the mean list length
+    // (1M Gaussians at 1080p: blend backward 0.210 -> 0.200 ms)
+    const double mean = double(n_pairs) / double(std::max(n_tiles, 1));
+    const auto at = [&](double f) { return uint32_t(std::min(f * mean, double(UINT32_MAX))); };
+    const WavePriority prio { at(0.55), at(0.93), at(1.37) };
     if (t.tile_w == 16)
-        launch_blend_bwd<16, 16>(t.batch, n_tiles, st, ranges, pl, xy2, co4, rgb, W, H, gx, bg, fb, T, gfb, sg);
+        launch_blend_bwd<16, 16>(t.batch, n_tiles, st, ranges, pl, xy2, co4, rgb, W, H, gx, bg, fb, T, gfb, sg, prio);
     else if (t.tile_w == 8)
-        launch_blend_bwd<8, 8>(t.batch, n_tiles, st, ranges, pl, xy2, co4, rgb, W, H, gx, bg, fb, T, gfb, sg);
+        launch_blend_bwd<8, 8>(t.batch, n_tiles, st, ranges, pl, xy2, co4, rgb, W, H, gx, bg, fb, T, gfb, sg, prio);
     else
-        launch_blend_bwd<32, 8>(t.batch, n_tiles, st, ranges, pl, xy2, co4, rgb, W, H, gx, bg, fb, T, gfb, sg);
+        launch_blend_bwd<32, 8>(t.batch, n_tiles, st, ranges, pl, xy2, co4, rgb, W, H, gx, bg, fb, T, gfb, sg, prio);
 }
 
 struct EmitArgs {
@@ -996,7 +1004,7 @@ int dgmr_splat_backward(const dgmr_gaussians* g, const dgmr_camera* cam, const d
     DGMR_HIP_CHECK(hipMemsetAsync(screen_grads, 0, size_t(n) * 9 * sizeof(float), st));
     if (n_pairs > 0) {
         StageScope sc("splat.blend_backward", st, 40.0 * double(n_pairs) + 28.0 * double(int64_t(cam->width) * cam->height) + 8.0 * double(n_tiles));
-        splat_blend_backward(*tiling, int(n_tiles), st, bins->ranges, bins->point_list, geom->xy, geom->conic_opacity, geom->rgb, cam->width,
+        splat_blend_backward(*tiling, int(n_tiles), n_pairs, st, bins->ranges, bins->point_list, geom->xy, geom->conic_opacity, geom->rgb, cam->width,
             cam->height, dc.grid_x, cam->background, framebuffer, final_transmittance, grad_framebuffer, screen_grads);
         DGMR_HIP_CHECK(hipGetLastError());
     }

@@ -615,6 +615,24 @@ __device__ __forceinline__ void blend_tile_forward(BlendSmem<B>& sm, uint32_t fi
 #endif
 constexpr int kBwdSplit = DGMR_BWD_SPLIT;
 
+// Issue priority of a backward blend wave from its tile's list length
+// (thresholds t1 < t2 < t3, set from the frame's mean list length): waves on
+// longer lists issue first on their SIMD, so they finish with the rest of the
+// launch instead of running on alone at its end.  Results do not depend on it.
+struct WavePriority {
+    uint32_t t1, t2, t3;
+};
+
+__device__ __forceinline__ void set_wave_priority(uint32_t len, const WavePriority& p)
+{
+    if (len > p.t3)
+        __builtin_amdgcn_s_setprio(3);
+    else if (len > p.t2)
+        __builtin_amdgcn_s_setprio(2);
+    else if (len > p.t1)
+        __builtin_amdgcn_s_setprio(1);
+}
+
 template <int B>
 struct BlendSmemBwd {
     float4 geo[B];   // x, y, A, B
