@@ -375,11 +375,15 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const KeyT* __restr
 // times slower).  RADIX (256, or 128 for digits of at most 7
 // bits) sizes the per-digit tables: 128 keeps a 16-bit-key block of 8 192
 // items within a third of the CU's LDS, so three blocks fit instead of two.
-template <int ITEMS, class KeyT, int RADIX>
+// BITS: the digit width when known at compile time (the bit loop then has no
+// run-time tests: with a run-time width each round's eight tests cost VALU
+// work to turn the uniform condition into a lane mask and back); 0 reads bits.
+template <int ITEMS, class KeyT, int RADIX, int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-    KeyT* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int bits, int n_blocks,
+    KeyT* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n_max, const uint32_t* n_dev, int shift, int bits_arg, int n_blocks,
     const uint32_t* __restrict__ offsets, KeyOut ko)
 {
+    const int bits = BITS > 0 ? BITS : bits_arg;
     constexpr int kSortTile = kSortThreads * ITEMS;
     constexpr int kWaves = kSortThreads / 64;
     constexpr int kRounds = kSortTile / kSortThreads; // rounds of 64 items per wave
@@ -402,7 +406,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __re
     for (int i = threadIdx.x; i < kWaves * RADIX; i += kSortThreads)
         (&s_count[0][0])[i] = 0;
     __syncthreads();
-    uint32_t keys[kRounds], vals[kRounds], rank[kRounds];
+    uint32_t keys[kRounds], vals[kRounds], rank[sizeof(KeyT) == 2 ? 1 : kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
@@ -423,7 +427,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __re
         uint32_t peers_lo = uint32_t(valid_lanes), peers_hi = uint32_t(valid_lanes >> 32);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            if (b < bits) {
+            if (BITS > 0 ? b < BITS : b < bits) {
                 const uint32_t s = uint32_t(int32_t(digit << (31 - b)) >> 31);
                 const uint64_t m = __ballot(s != 0u);
                 peers_lo &= ~(uint32_t(m) ^ s);
@@ -434,7 +438,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __re
         // the wave's LDS operations run in order: every lane reads the running
         // count before the lowest lane of its digit group writes it back
         const uint32_t before = s_count[wave][digit];
-        rank[r] = before + uint32_t(__popcll(peers & lt_mask));
+        const uint32_t rk = before + uint32_t(__popcll(peers & lt_mask));
+        // a 16-bit key and its rank (below 2^16) share one register
+        if constexpr (sizeof(KeyT) == 2)
+            keys[r] |= rk << 16;
+        else
+            rank[r] = rk;
         if (valid && (peers & lt_mask) == 0)
             s_count[wave][digit] = before + uint32_t(__popcll(peers));
         __builtin_amdgcn_wave_barrier();
@@ -479,8 +488,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __re
     for (int r = 0; r < kRounds; ++r) {
         const int i = int(wave) * (kRounds * 64) + r * 64 + int(lane);
         if (i < count) {
-            const uint32_t pos = s_count[wave][(keys[r] >> shift) & mask] + rank[r];
-            s_keys[pos] = KeyT(keys[r]);
+            const uint32_t key = sizeof(KeyT) == 2 ? (keys[r] & 0xffffu) : keys[r];
+            const uint32_t rk = sizeof(KeyT) == 2 ? (keys[r] >> 16) : rank[r];
+            const uint32_t pos = s_count[wave][(key >> shift) & mask] + rk;
+            s_keys[pos] = KeyT(key);
             s_vals[pos] = vals[r];
         }
     }
@@ -636,10 +647,18 @@ static void radix_pass(int blocks, const KeyT* kin, const uint32_t* vin, KeyT* k
     *err = scan_u32(hist, hist, nullptr, n_hist, false, scan_scratch, stream, true);
     if (*err != hipSuccess)
         return;
-    if (bits <= 7)
-        k_radix_scatter<ITEMS, KeyT, 128><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist, ko);
+    // the digit widths of the binnings (8-bit depth passes, 7 + 6 for tile
+    // ids) are compiled in; other widths read theirs at run time
+    if (bits == 8)
+        k_radix_scatter<ITEMS, KeyT, 256, 8><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist, ko);
+    else if (bits == 7)
+        k_radix_scatter<ITEMS, KeyT, 128, 7><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist, ko);
+    else if (bits == 6)
+        k_radix_scatter<ITEMS, KeyT, 128, 6><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist, ko);
+    else if (bits <= 7)
+        k_radix_scatter<ITEMS, KeyT, 128, 0><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist, ko);
     else
-        k_radix_scatter<ITEMS, KeyT, 256><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist, ko);
+        k_radix_scatter<ITEMS, KeyT, 256, 0><<<blocks, kSortThreads, 0, stream>>>(kin, vin, kout, vout, n_max, n_dev, shift, bits, blocks, hist, ko);
     *err = hipGetLastError();
 }
 
