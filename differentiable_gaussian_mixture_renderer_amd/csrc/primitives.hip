@@ -439,11 +439,18 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const KeyT* __re
         // count before the lowest lane of its digit group writes it back
         const uint32_t before = s_count[wave][digit];
         const uint32_t rk = before + uint32_t(__popcll(peers & lt_mask));
-        // a 16-bit key and its rank (below 2^16) share one register
-        if constexpr (sizeof(KeyT) == 2)
+        // a 16-bit key and its rank (below 2^16) share one register.  The
+        // empty asm makes the value opaque, so the rank is formed here and
+        // the placement recomputes its counter address: otherwise the
+        // compiler keeps every round's counter value and address live until
+        // the placement (about 64 registers at 32 rounds, one wave fewer)
+        if constexpr (sizeof(KeyT) == 2) {
             keys[r] |= rk << 16;
-        else
+            asm volatile("" : "+v"(keys[r]));
+        } else {
             rank[r] = rk;
+            asm volatile("" : "+v"(rank[r]));
+        }
         if (valid && (peers & lt_mask) == 0)
             s_count[wave][digit] = before + uint32_t(__popcll(peers));
         __builtin_amdgcn_wave_barrier();
