@@ -76,7 +76,9 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t
         uint32_t t = 0;
         if (lane_id() == 0)
             t = atomicAdd(queue, 1u);
-        const int tile = __shfl(int(t), 0, 64);
+        // lane 0's value, read into a scalar register: the view record and
+        // its pointers then stay in scalar registers too
+        const int tile = int(__builtin_amdgcn_readfirstlane(t));
         if (tile >= total_tiles)
             break;
         const ViewBlend& vb = views[find_view(views, n_views, tile)];
