@@ -12,6 +12,7 @@ namespace dgmr {
 template <class S>
 __device__ __forceinline__ void det3_grad(const S a[3][3], S g, S out[3][3])
 {
+#pragma clang fp contract(fast)
     out[0][0] = g * (a[1][1] * a[2][2] - a[1][2] * a[2][1]);
     out[0][1] = g * (a[1][2] * a[2][0] - a[1][0] * a[2][2]);
     out[0][2] = g * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
@@ -68,6 +69,7 @@ __device__ __forceinline__ void sym3_inverse(const S* c, S* inv)
 template <class S>
 __device__ __forceinline__ void sym3_inverse_grad(const S* inv, const S* g_inv, S* g_x)
 {
+#pragma clang fp contract(fast)
     S A[3][3], G[3][3], t[3][3], r[3][3];
     sym3_to_mat(inv, A);
     sym3_grad_to_mat(g_inv, G);
@@ -84,6 +86,7 @@ __device__ __forceinline__ void sym3_inverse_grad(const S* inv, const S* g_inv, 
 template <class S>
 __device__ __forceinline__ void quat_to_mat_grad(const S* q, const S G[3][3], S* gq)
 {
+#pragma clang fp contract(fast)
     const S w = q[0], x = q[1], y = q[2], z = q[3];
     gq[0] = S(2) * (z * G[1][0] - y * G[2][0] - z * G[0][1] + x * G[2][1] + y * G[0][2] - x * G[1][2]);
     gq[1] = S(2) * (y * G[1][0] + z * G[2][0] + y * G[0][1] - S(2) * x * G[1][1] + w * G[2][1] + z * G[0][2] - w * G[1][2] - S(2) * x * G[2][2]);
@@ -95,6 +98,7 @@ __device__ __forceinline__ void quat_to_mat_grad(const S* q, const S G[3][3], S*
 template <class S>
 __device__ __forceinline__ void compute_cov_grad(const S* s, const S* q, const S* g_cov, S* g_s, S* g_q)
 {
+#pragma clang fp contract(fast)
     S R[3][3], RS[3][3], G[3][3], gRS[3][3], gR[3][3];
     quat_to_mat(q, R);
 #pragma unroll
@@ -122,6 +126,7 @@ __device__ __forceinline__ void compute_cov_grad(const S* s, const S* q, const S
 template <class S>
 __device__ __forceinline__ void affine_cut_grad(const S* c, const S M[3][3], const S* g2, S* g_cov, S g_M[3][3], bool want_M)
 {
+#pragma clang fp contract(fast)
     const S Gy[3][3] = { { g2[0], g2[1] * S(0.5), 0 }, { g2[1] * S(0.5), g2[2], 0 }, { 0, 0, 0 } };
     S MT[3][3], t[3][3], gc[3][3];
 #pragma unroll
@@ -174,6 +179,7 @@ template <class S>
 __device__ __forceinline__ void splat_ots_grad_view(S w, const S* mu, const OtsPre<S>& pre, const DevCamera<S>& cam, S filter, S g_weight,
     const S* g_xy, const S* g_cov2, S* g_w, S* g_mu, S* g_cov3, S* g_num)
 {
+#pragma clang fp contract(fast)
     SplatIn<S> in;
     splat_projection<S, true>(mu, cam, &in);
 #pragma unroll
@@ -249,6 +255,7 @@ __device__ __forceinline__ void splat_ots_grad_view(S w, const S* mu, const OtsP
 template <class S>
 __device__ __forceinline__ void splat_ots_grad_finish(const S* s, const S* q, const OtsPre<S>& pre, const S* g_cov3, S g_num, S* g_s, S* g_q)
 {
+#pragma clang fp contract(fast)
     const S g_a = g_num * S(2 * kPi) * pre.sb, g_b = g_num * S(2 * kPi) * pre.sa;
     const S gs[3] = { pre.i0 == 0 ? g_a : S(0), pre.i0 == 1 ? g_a : (pre.i1 == 1 ? g_b : S(0)), pre.i1 == 2 ? g_b : S(0) };
     S gs2[3];

@@ -333,6 +333,7 @@ __device__ __forceinline__ void sh_basis(int deg, const S* dir, S* Y)
 template <class S>
 __device__ __forceinline__ void sh_basis_grad(int deg, const S* dir, S dY[16][3])
 {
+#pragma clang fp contract(fast)
     const S x = dir[0], y = dir[1], z = dir[2];
 #pragma unroll
     for (int k = 0; k < 16; ++k)

@@ -857,6 +857,7 @@ __device__ __forceinline__ void preprocess_backward_core(const float* mu, const 
 __device__ __forceinline__ void preprocess_backward_finish(const float* s, const float* q, const OtsPre<float>& pre, const SplatParams& p,
     const float* g_cov3, float g_num, float* g_s, float* g_q)
 {
+#pragma clang fp contract(fast)
     splat_ots_grad_finish(s, q, pre, g_cov3, g_num, g_s, g_q);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -888,6 +889,7 @@ __device__ __forceinline__ void preprocess_backward_core(const float* mu, const 
     const float* c, const DevCamera<float>& cam, const float4 A, const uint8_t* clamped_in, const float* sg_in, float* g_mu_acc, float* g_cov3_acc,
     float* g_num_acc, float* g_w_acc, float* g_sh)
 {
+#pragma clang fp contract(fast)
     // the inputs are read into registers before any arithmetic (and an SH
     // row in global memory before the basis is evaluated): read where they
     // are used, each load would be waited for in turn
