@@ -787,14 +787,16 @@ __device__ __forceinline__ void blend_tile_backward(BlendSmemBwd<B>& sm, uint32_
                     const float test_T = T[k] * tk;
                     // one select on the weight: fma(x, 0, s) is s exactly for finite x
                     const float wgt = use[k] ? alphak[k] * T[k] : 0.f;
-                    const float grgb = gC0[k] * c.z + gC1[k] * c.w + gC2[k] * blue; // <dL/dC, rgb>
+                    // <dL/dC, rgb> and dL/dalpha as fused multiply-adds (gradient
+                    // values only: no blend decision reads them)
+                    const float grgb = fmaf(gC2[k], blue, fmaf(gC1[k], c.w, gC0[k] * c.z));
                     D[k] = fmaf(-grgb, wgt, D[k]);
                     sr = fmaf(gC0[k], wgt, sr);
                     sg = fmaf(gC1[k], wgt, sg);
                     sbl = fmaf(gC2[k], wgt, sbl);
                     // 1 - alpha >= 0.01, so the hardware reciprocal (1 ulp) is ample
                     const float inv_tk = __builtin_amdgcn_rcpf(tk);
-                    const float g_alpha = grgb * T[k] - D[k] * inv_tk;
+                    const float g_alpha = fmaf(grgb, T[k], -(D[k] * inv_tk));
                     const float g_raw = (use[k] && raw < 0.99f) ? g_alpha : 0.f;
                     so = fmaf(Gk[k], g_raw, so);
                     const float gp = raw * g_raw; // dL/dpower
