@@ -362,6 +362,39 @@ __device__ __forceinline__ void sh_basis_grad(int deg, const S* dir, S dY[16][3]
     }
 }
 
+// g += sum_k w[k] dY_k/d dir over the basis of degree deg, without forming
+// the 16 x 3 derivatives (the same terms as sh_basis_grad, each scaled by
+// its weight once): the derivative table and the weights are never live
+// together
+template <class S>
+__device__ __forceinline__ void sh_dir_grad(int deg, const S* dir, const S w[16], S g[3])
+{
+#pragma clang fp contract(fast)
+    const S x = dir[0], y = dir[1], z = dir[2];
+    if (deg > 0) {
+        const S c1 = S(SH_C1);
+        g[1] -= c1 * w[1];
+        g[2] += c1 * w[2];
+        g[0] -= c1 * w[3];
+    }
+    if (deg > 1) {
+        const S a4 = S(SH_C2_0) * w[4], a5 = S(SH_C2_1) * w[5], a6 = S(SH_C2_2) * w[6], a7 = S(SH_C2_3) * w[7], a8 = S(SH_C2_4) * w[8];
+        g[0] += a4 * y + a6 * (S(-2) * x) + a7 * z + a8 * (S(2) * x);
+        g[1] += a4 * x + a5 * z + a6 * (S(-2) * y) + a8 * (S(-2) * y);
+        g[2] += a5 * y + a6 * (S(4) * z) + a7 * x;
+        if (deg > 2) {
+            const S xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+            const S b9 = S(SH_C3_0) * w[9], b10 = S(SH_C3_1) * w[10], b11 = S(SH_C3_2) * w[11], b12 = S(SH_C3_3) * w[12];
+            const S b13 = S(SH_C3_4) * w[13], b14 = S(SH_C3_5) * w[14], b15 = S(SH_C3_6) * w[15];
+            g[0] += b9 * (S(6) * xy) + b10 * yz + b11 * (S(-2) * xy) + b12 * (S(-6) * xz) + b13 * (S(4) * zz - S(3) * xx - yy)
+                + b14 * (S(2) * xz) + b15 * (S(3) * xx - S(3) * yy);
+            g[1] += b9 * (S(3) * xx - S(3) * yy) + b10 * xz + b11 * (S(4) * zz - xx - S(3) * yy) + b12 * (S(-6) * yz) + b13 * (S(-2) * xy)
+                + b14 * (S(-2) * yz) + b15 * (S(-6) * xy);
+            g[2] += b10 * xy + b11 * (S(8) * yz) + b12 * (S(6) * zz - S(3) * xx - S(3) * yy) + b13 * (S(8) * xz) + b14 * (xx - yy);
+        }
+    }
+}
+
 // portable natural logarithm from basic IEEE operations (bit decomposition plus
 // the atanh series log m = 2 atanh((m-1)/(m+1)); J.-M. Muller, "Elementary
 // Functions").  The host test suite evaluates the identical sequence.

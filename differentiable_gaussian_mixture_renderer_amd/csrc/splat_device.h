@@ -875,7 +875,7 @@ __device__ __forceinline__ void preprocess_backward_gaussian(int64_t i, const Sp
     gaussian_params(p, i, s, q, &w);
     const OtsPre<float> pre = ots_pre(s, q);
     float g_cov3[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f }, g_num = 0.f;
-    preprocess_backward_core<MODE, true>(mu, pre, w, p, c, cam, A, clamped3, sg, g_mu_acc, g_cov3, &g_num, g_w_acc, g_sh);
+    preprocess_backward_core<MODE, false>(mu, pre, w, p, c, cam, A, clamped3, sg, g_mu_acc, g_cov3, &g_num, g_w_acc, g_sh);
     float g_s[3], g_q[4];
     preprocess_backward_finish(s, q, pre, p, g_cov3, g_num, g_s, g_q);
 #pragma unroll
@@ -915,12 +915,13 @@ __device__ __forceinline__ void preprocess_backward_core(const float* mu, const 
     const float gr[3] = { clamped3[0] ? sg[6] : 0.f, clamped3[1] ? sg[7] : 0.f, clamped3[2] ? sg[8] : 0.f };
     const int kmax = min(p.sh_coeffs, (p.sh_degree + 1) * (p.sh_degree + 1));
     const ShRow<ROW_GLOBAL> cf(c, kmax);
-    float Y[16], dY[16][3];
+    float Y[16];
     sh_basis(p.sh_degree, dir, Y);
-    sh_basis_grad(p.sh_degree, dir, dY);
-    float g_dir[3] = { 0.f, 0.f, 0.f };
+    // per coefficient <c_k, dL/drgb>, the weight of dY_k/d dir
+    float wdot[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
+        wdot[k] = 0.f;
         if (k < kmax) {
             const float c0 = cf[3 * k], c1 = cf[3 * k + 1], c2 = cf[3 * k + 2];
             if constexpr (MODE == ShGrad::Export) {
@@ -930,12 +931,11 @@ __device__ __forceinline__ void preprocess_backward_core(const float* mu, const 
                 g_sh[3 * k + 1] += Y[k] * gr[1];
                 g_sh[3 * k + 2] += Y[k] * gr[2];
             }
-            const float dot_c = c0 * gr[0] + c1 * gr[1] + c2 * gr[2];
-            g_dir[0] = fmaf(dY[k][0], dot_c, g_dir[0]);
-            g_dir[1] = fmaf(dY[k][1], dot_c, g_dir[1]);
-            g_dir[2] = fmaf(dY[k][2], dot_c, g_dir[2]);
+            wdot[k] = c0 * gr[0] + c1 * gr[1] + c2 * gr[2];
         }
     }
+    float g_dir[3] = { 0.f, 0.f, 0.f };
+    sh_dir_grad(p.sh_degree, dir, wdot, g_dir);
     if constexpr (MODE == ShGrad::Export) {
 #pragma unroll
         for (int k = 0; k < 16; ++k)
