@@ -3,6 +3,8 @@
 hipcc cross-compiles for gfx950 without a GPU.  Flags:
   -ffp-contract=off          no implicit multiply-add fusion: binning math is a
                              plain IEEE sequence, bit-identical to the host oracle
+                             (functions that only compute gradient values opt in
+                             with #pragma clang fp contract(fast))
   -munsafe-fp-atomics        hardware float atomics (global_atomic_add_f32)
   -target-feature -packed-fp32-ops   keep packed FP32 instructions out of the code
 """
