@@ -3,12 +3,15 @@
 // resolutions in one preprocess launch, one binning pass and one persistent
 // blend launch (forward and backward).
 //
-// Layout: per-view geometry is stored at slot v * n + i, so Gaussian ids in
-// the pair list index every array directly; view v's tiles occupy a contiguous
-// range of the global tile index space.  The blend runs as a persistent grid
-// whose workgroups take the next tile of any view from a device counter, so
-// small and large views share the chip without per-view launches or syncs.
+// Layout: the Gaussians take slots in the Morton order of their centres
+// (slot s holds Gaussian order[s]), and per-view geometry is stored at entry
+// v * n + s, so the entry ids in the pair list index every array directly;
+// view v's tiles occupy a contiguous range of the global tile index space.
+// The blend runs as a persistent grid whose workgroups take the next tile of
+// any view from a device counter, so small and large views share the chip
+// without per-view launches or syncs.
 #include <algorithm>
+#include <numeric>
 #include <type_traits>
 #include <vector>
 
@@ -29,23 +32,182 @@ struct ViewBlend {
     float l1_scale;
 };
 
-// thread per Gaussian: its parameters are read once and projected into every view
+// ------------------------------------------------------------ spatial order
+// The preprocess kernels below give each thread one Gaussian and loop over the
+// views, and a wave runs a view's projection (or its adjoint) when any of its
+// 64 Gaussians is visible there.  In id order a wave's Gaussians are spread
+// over the scene and nearly every wave runs every view with about 0.6 of its
+// lanes busy (64 cameras around a 1M-Gaussian scene).  Slots therefore follow
+// the Morton order of the centres, 8 bits per axis over the scene's bounding
+// box: a wave's Gaussians are neighbours and are culled together.  Any
+// permutation is correct; this one only decides which thread does what.
+constexpr int kOrderBits = 24;
+// the radix sort ping-pongs between two buffers: an odd pass count leaves the
+// sorted ids in the second one, which is the order array
+static_assert(((kOrderBits + 7) / 8) % 2 == 1, "the sorted ids must land in the second sort buffer");
+
+// float bits whose unsigned order is the float order (NaN above +inf)
+__device__ __forceinline__ uint32_t ordered_bits(float x)
+{
+    const uint32_t b = __float_as_uint(x);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered_bits(uint32_t u)
+{
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// smallest and largest centre per axis (as ordered bits) of each block's
+// share: partial[6 b + a] and partial[6 b + 3 + a]
+constexpr int kBoundsBlocks = 256;
+__device__ __forceinline__ void bounds_reduce(uint32_t (&lo)[3], uint32_t (&hi)[3])
+{
+    __shared__ uint32_t s_box[4][6];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], uint32_t(__shfl_xor(int(lo[a]), off)));
+            hi[a] = max(hi[a], uint32_t(__shfl_xor(int(hi[a]), off)));
+        }
+    }
+    const int wave = int(threadIdx.x) / 64;
+    if (lane_id() == 0) {
+        for (int a = 0; a < 3; ++a) {
+            s_box[wave][a] = lo[a];
+            s_box[wave][3 + a] = hi[a];
+        }
+    }
+    __syncthreads();
+    // thread 0 ends with the block's bounds
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < int(blockDim.x) / 64; ++w) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], s_box[w][a]);
+                hi[a] = max(hi[a], s_box[w][3 + a]);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bounds(int64_t n, const float* __restrict__ means, uint32_t* __restrict__ partial)
+{
+    uint32_t lo[3] = { 0xffffffffu, 0xffffffffu, 0xffffffffu }, hi[3] = { 0u, 0u, 0u };
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t u = ordered_bits(means[3 * i + a]);
+            lo[a] = min(lo[a], u);
+            hi[a] = max(hi[a], u);
+        }
+    }
+    bounds_reduce(lo, hi);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            partial[6 * blockIdx.x + a] = lo[a];
+            partial[6 * blockIdx.x + 3 + a] = hi[a];
+        }
+    }
+}
+
+// box[0..2], box[3..5]: the blocks' partial bounds combined (one block)
+__global__ __launch_bounds__(256) void k_bounds_final(int blocks, const uint32_t* __restrict__ partial, uint32_t* __restrict__ box)
+{
+    uint32_t lo[3] = { 0xffffffffu, 0xffffffffu, 0xffffffffu }, hi[3] = { 0u, 0u, 0u };
+    for (int b = int(threadIdx.x); b < blocks; b += blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], partial[6 * b + a]);
+            hi[a] = max(hi[a], partial[6 * b + 3 + a]);
+        }
+    }
+    bounds_reduce(lo, hi);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            box[a] = lo[a];
+            box[3 + a] = hi[a];
+        }
+    }
+}
+
+// bit k of an 8-bit value to bit 3k
+__device__ __forceinline__ uint32_t spread_bits8(uint32_t x)
+{
+    x = (x | (x << 8)) & 0x0000f00fu;
+    x = (x | (x << 4)) & 0x000c30c3u;
+    x = (x | (x << 2)) & 0x00249249u;
+    return x;
+}
+
+// sort keys (Morton code of the centre in the bounding box) and ids 0..n-1;
+// non-finite centres or bounds clamp to cell 0 (the order stays a permutation)
+__global__ __launch_bounds__(256) void k_morton_keys(int64_t n, const float* __restrict__ means, const uint32_t* __restrict__ box,
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ ids)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t key = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float lo = from_ordered_bits(box[a]), hi = from_ordered_bits(box[3 + a]);
+        const float scale = 256.f / fmaxf(hi - lo, 1e-30f);
+        const float c = fminf(fmaxf((means[3 * i + a] - lo) * scale, 0.f), 255.f);
+        key |= spread_bits8(uint32_t(c)) << a;
+    }
+    keys[i] = key;
+    ids[i] = uint32_t(i);
+}
+
+// The per-Gaussian parameters (centre, scales, rotation, opacity) copied
+// into slot order (row s = Gaussian order[s]), so the preprocess kernels read
+// them as contiguous rows; the backward reads the forward's copy.  The SH
+// rows, 192 bytes each at degree 3, are read in place where they are staged.
+__global__ __launch_bounds__(256) void k_gather_params(int64_t n, SplatParams p, const uint32_t* __restrict__ order, float* __restrict__ means,
+    float* __restrict__ scales, float* __restrict__ rotations, float* __restrict__ opacity)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    const int64_t i = order[s];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        means[3 * s + a] = p.means[3 * i + a];
+        scales[3 * s + a] = p.scales[3 * i + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        rotations[4 * s + a] = p.rotations[4 * i + a];
+    opacity[s] = p.opacity[i];
+}
+
+// thread per slot s (Gaussian order[s]; p in slot order but for its SH rows,
+// read in place): the Gaussian's parameters are read once and projected into
+// every view
 __global__ __launch_bounds__(256) void k_batch_preprocess(int64_t n, int n_views, SplatParams p, const DevCamera<float>* __restrict__ cams,
-    SplatOut out)
+    const uint32_t* __restrict__ order, SplatOut out)
 {
     __shared__ float s_sh[256 * kShStride];
+    __shared__ uint32_t s_ids[256];
     const int64_t first = int64_t(blockIdx.x) * blockDim.x;
-    stage_sh(s_sh, p.sh, first, int(min<int64_t>(blockDim.x, n - first)), p.sh_coeffs);
+    const int count = int(min<int64_t>(blockDim.x, n - first));
+    if (int(threadIdx.x) < count)
+        s_ids[threadIdx.x] = order[first + threadIdx.x];
     __syncthreads();
-    const int64_t i = first + threadIdx.x;
-    if (i >= n)
+    stage_sh(s_sh, p.sh, s_ids, count, p.sh_coeffs);
+    __syncthreads();
+    const int64_t s = first + threadIdx.x;
+    if (s >= n)
         return;
     // the view-independent inputs once, then each view
     GaussIn g;
-    gauss_mean(p, i, g);
-    gauss_rest(p, i, g);
+    gauss_mean(p, s, g);
+    gauss_rest(p, s, g);
     for (int v = 0; v < n_views; ++v)
-        preprocess_view<true, false>(g, i, int64_t(v) * n + i, p, s_sh + threadIdx.x * kShStride, cams[v], out);
+        preprocess_view<true, false>(g, s, int64_t(v) * n + s, p, s_sh + threadIdx.x * kShStride, cams[v], out);
 }
 
 __device__ __forceinline__ int find_view(const ViewBlend* views, int n_views, int tile)
@@ -63,11 +225,12 @@ __device__ __forceinline__ int find_view(const ViewBlend* views, int n_views, in
 
 // Persistent blend over the global tile queue: kBlendWaves independent waves
 // per workgroup, each popping tiles for itself.  The queue counter starts at
-// 0; a wave exits once the counter has passed the last tile.
+// 0; a wave exits once the counter has passed the last tile.  Queue position
+// k is tile k, or tile_order[k] when given.
 template <int TW, int TH, int B, bool BACKWARD>
 __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ point_list,
     const float2* __restrict__ xy, const float4* __restrict__ conic_opacity, const float* __restrict__ rgb, const ViewBlend* __restrict__ views,
-    int n_views, int total_tiles, uint32_t* __restrict__ queue, float* __restrict__ screen_grads)
+    int n_views, int total_tiles, uint32_t* __restrict__ queue, float* __restrict__ screen_grads, const uint32_t* __restrict__ tile_order)
 {
     using Smem = typename std::conditional<BACKWARD, BlendSmemBwd<B>, BlendSmem<B>>::type;
     __shared__ Smem sm_all[kBlendWaves];
@@ -78,9 +241,11 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t
             t = atomicAdd(queue, 1u);
         // lane 0's value, read into a scalar register: the view record and
         // its pointers then stay in scalar registers too
-        const int tile = int(__builtin_amdgcn_readfirstlane(t));
+        int tile = int(__builtin_amdgcn_readfirstlane(t));
         if (tile >= total_tiles)
             break;
+        if (tile_order)
+            tile = int(tile_order[tile]);
         const ViewBlend& vb = views[find_view(views, n_views, tile)];
         const int local = tile - vb.tile_base;
         const int px0 = (local % vb.grid_x) * TW, py0 = (local / vb.grid_x) * TH;
@@ -101,6 +266,38 @@ __global__ __launch_bounds__(64 * kBlendWaves) void k_batch_blend(const uint32_t
     }
 }
 
+// Tile order of the blend backward.  It adds each (tile, Gaussian) sum into
+// the entry's screen gradients with float atomics, and tiles in flight at the
+// same time that touch the same cache lines (a shared Gaussian, or Gaussians
+// in neighbouring slots, which the spatial order makes neighbours on screen)
+// serialise on them.  The queue therefore visits the views round-robin, the
+// largest first, and steps through each view's tiles with a stride near
+// 0.618 of its tile count (coprime with it), so tiles in flight together
+// belong to different views or lie far apart.  by_size[j] = (first tile,
+// tiles, stride) of the j-th largest view; segment q = (first queue
+// position, first round, views) of the rounds in which the first `views`
+// views still have tiles.
+__global__ __launch_bounds__(256) void k_tile_order(int total, const int4* __restrict__ by_size, const int4* __restrict__ segs, int n_segs,
+    uint32_t* __restrict__ order)
+{
+    const int k = int(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= total)
+        return;
+    int lo = 0, hi = n_segs - 1; // the last segment starting at or before k
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (segs[mid].x <= k)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int4 q = segs[lo];
+    const int d = k - q.x;
+    const int round = q.y + d / q.z;
+    const int4 v = by_size[d % q.z];
+    order[k] = uint32_t(v.x + int((int64_t(round) * v.z) % v.y));
+}
+
 // clears the screen gradients (9 floats per entry) as one streaming pass of
 // 16-byte stores over the whole slice: the blend backward adds into the
 // visible entries' values and only those are read.  Clearing only the
@@ -117,29 +314,37 @@ __global__ __launch_bounds__(256) void k_zero_screen_grads(int64_t n_floats, flo
         screen_grads[k] = 0.f;
 }
 
-// thread per Gaussian, summing its adjoint over the views of each gradient
-// group: group[v] is -1 (no gradient) or a group id whose views are
-// consecutive; group g's sums go to block g of the outputs ([n_groups][n][..]).
-// n_views here is the number of views up to the last one with a group: the
-// views after it are neither loaded nor visited.
+// thread per slot s (Gaussian order[s]; p in slot order but for its SH rows,
+// read in place), summing its
+// adjoint over the views of each gradient group: group[v] is -1 (no
+// gradient) or a group id whose views are consecutive; group g's sums go to
+// block g of the outputs ([n_groups][n][..], in Gaussian order).  n_views here is the number of
+// views up to the last one with a group: the views after it are neither
+// loaded nor visited.
 __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, int n_views, SplatParams p, const DevCamera<float>* __restrict__ cams,
-    const int32_t* __restrict__ group, const uint32_t* __restrict__ tiles, const float4* __restrict__ conic_opacity,
-    const uint8_t* __restrict__ clamped, const float* __restrict__ screen_grads, float* __restrict__ g_means, float* __restrict__ g_scales,
-    float* __restrict__ g_rotations, float* __restrict__ g_opacity, float* __restrict__ g_sh)
+    const uint32_t* __restrict__ order, const int32_t* __restrict__ group, const uint32_t* __restrict__ tiles,
+    const float4* __restrict__ conic_opacity, const uint8_t* __restrict__ clamped, const float* __restrict__ screen_grads,
+    float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations, float* __restrict__ g_opacity,
+    float* __restrict__ g_sh)
 {
     // the block's SH coefficients, staged once; the SH gradient rows of a
     // group leave through s_out, half a block at a time (75 KB of LDS keeps
     // two blocks per CU, which the registers allow anyway)
     __shared__ float s_sh[256 * kShStride];
     __shared__ float s_out[128 * kShStride];
+    __shared__ uint32_t s_ids[256];
     const int64_t first = int64_t(blockIdx.x) * blockDim.x;
     const int count = int(min<int64_t>(blockDim.x, n - first));
-    stage_sh(s_sh, p.sh, first, count, p.sh_coeffs);
+    if (int(threadIdx.x) < count)
+        s_ids[threadIdx.x] = order[first + threadIdx.x];
+    __syncthreads();
+    stage_sh(s_sh, p.sh, s_ids, count, p.sh_coeffs);
     __syncthreads();
     // the view loop and its group boundaries are uniform over the block, so
     // every thread takes part in the block-wide copies at a boundary
-    const int64_t i = first + threadIdx.x;
+    const int64_t i = first + threadIdx.x; // the slot: row of p and of the per-view arrays
     const bool valid = i < n;
+    const int64_t gid = valid ? int64_t(s_ids[threadIdx.x]) : 0; // the Gaussian: row of the outputs
     const float* row = s_sh + threadIdx.x * kShStride;
     const int kc = 3 * p.sh_coeffs;
     // per group: the mean, opacity and SH gradients, and the view sums of
@@ -191,7 +396,7 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
                     float g_s[3], g_q[4];
                     preprocess_backward_finish(s, q, pre, p, g_cov3, g_num, g_s, g_q);
                     activation_backward(p, i, g_s, g_q, &g_w);
-                    const int64_t o = int64_t(cur) * n + i;
+                    const int64_t o = int64_t(cur) * n + gid;
                     for (int a = 0; a < 3; ++a) {
                         g_means[3 * o + a] = g_mu[a];
                         g_scales[3 * o + a] = g_s[a];
@@ -213,7 +418,7 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
                     }
                     __syncthreads();
                     if (count > lo)
-                        unstage_sh(dst, s_out, first + lo, min(128, count - lo), p.sh_coeffs);
+                        unstage_sh(dst, s_out, s_ids + lo, min(128, count - lo), p.sh_coeffs);
                 }
             }
             cur = gv;
@@ -250,6 +455,22 @@ struct BatchLayout {
     uint32_t* point_list;
     uint32_t* ranges;
     float* screen_grads;
+    // the spatial order (slot -> Gaussian), its sort's buffers, and the
+    // parameters in slot order
+    uint32_t* order;
+    uint32_t* order_keys;
+    uint32_t* order_ids;
+    uint32_t* order_keys2;
+    uint32_t* order_box; // 6 bounds, then kBoundsBlocks x 6 partial bounds
+    void* order_scratch;
+    float* slot_means;
+    float* slot_scales;
+    float* slot_rotations;
+    float* slot_opacity;
+    // the blend backward's tile order and the tables it is built from
+    uint32_t* tile_order;
+    int4* tile_sizes;
+    int4* tile_segs;
     void* binning;
     size_t binning_bytes;
     int64_t total_tiles;
@@ -277,10 +498,32 @@ BatchLayout carve_batch(void* ws, int64_t n, const dgmr_batch_view* views, int n
     L.point_list = cv.take<uint32_t>(std::max<int64_t>(cap, 1));
     L.ranges = cv.take<uint32_t>(2 * std::max<int64_t>(total_tiles, 1));
     L.screen_grads = cv.take<float>(9 * m);
+    const int64_t n1 = std::max<int64_t>(n, 1);
+    L.order = cv.take<uint32_t>(n1);
+    L.order_keys = cv.take<uint32_t>(n1);
+    L.order_ids = cv.take<uint32_t>(n1);
+    L.order_keys2 = cv.take<uint32_t>(n1);
+    L.order_box = cv.take<uint32_t>(64 + 6 * kBoundsBlocks);
+    L.order_scratch = cv.take<char>(radix_scratch_bytes(n1));
+    L.slot_means = cv.take<float>(3 * n1);
+    L.slot_scales = cv.take<float>(3 * n1);
+    L.slot_rotations = cv.take<float>(4 * n1);
+    L.slot_opacity = cv.take<float>(n1);
+    L.tile_order = cv.take<uint32_t>(std::max<int64_t>(total_tiles, 1));
+    L.tile_sizes = cv.take<int4>(n_views);
+    L.tile_segs = cv.take<int4>(n_views);
     L.binning_bytes = splat_workspace_bytes(m, cap, t.cull != 0);
     L.binning = cv.take<char>(L.binning_bytes);
     *used = cv.used;
     return L;
+}
+
+// the Gaussians' parameters in slot order, as the forward copied them, and
+// their SH rows in place
+SplatParams slot_params(const BatchLayout& L, const dgmr_gaussians* g)
+{
+    return SplatParams { L.slot_means, L.slot_scales, L.slot_rotations, L.slot_opacity, static_cast<const float*>(g->sh), g->sh_coeffs,
+        g->sh_degree, float(g->scale_multiplier), g->activation };
 }
 
 template <int TW, int TH>
@@ -291,10 +534,10 @@ void launch_batch_blend(bool backward, int batch, int grid, hipStream_t st, cons
 #define DGMR_BATCH_BLEND(B)                                                                                                            \
     if (backward)                                                                                                                      \
         k_batch_blend<TW, TH, B, true><<<grid, 64 * kBlendWaves, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt,  \
-            L.queue, L.screen_grads);                                                                                                  \
+            L.queue, L.screen_grads, L.tile_order);                                                                                    \
     else                                                                                                                               \
         k_batch_blend<TW, TH, B, false><<<grid, 64 * kBlendWaves, 0, st>>>(L.ranges, L.point_list, xy2, L.conic_opacity, L.rgb, L.views, n_views, tt, \
-            L.queue, L.screen_grads);
+            L.queue, L.screen_grads, nullptr);
     if (batch == 64) {
         DGMR_BATCH_BLEND(64)
     } else if (batch == 128) {
@@ -380,10 +623,23 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     void* recs;
     binning_key_buffers(L.binning, m, cull, &dkeys, &dids, &recs);
     if (n > 0) {
+        // the spatial order of the slots: bounding box, Morton keys, 24-bit sort
+        StageScope sc("batch.order", st, 52.0 * double(n));
+        const float* means = static_cast<const float*>(g->means);
+        const int bounds_blocks = int(std::min<int64_t>(ceil_div64(n, 256), kBoundsBlocks));
+        k_bounds<<<unsigned(bounds_blocks), 256, 0, st>>>(n, means, L.order_box + 64);
+        k_bounds_final<<<1, 256, 0, st>>>(bounds_blocks, L.order_box + 64, L.order_box);
+        k_morton_keys<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, means, L.order_box, L.order_keys, L.order_ids);
+        DGMR_HIP_CHECK(hipGetLastError());
+        DGMR_HIP_CHECK(radix_sort_pairs(SortBuffers { L.order_keys, L.order_ids, L.order_keys2, L.order }, n, nullptr, 0, kOrderBits,
+            L.order_scratch, st));
+        k_gather_params<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, make_splat_params(g), L.order, L.slot_means, L.slot_scales,
+            L.slot_rotations, L.slot_opacity);
+        DGMR_HIP_CHECK(hipGetLastError());
+    }
+    if (n > 0) {
         StageScope sc("batch.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 55.0);
-        const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
-            static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), g->activation };
-        k_batch_preprocess<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, p, L.cams,
+        k_batch_preprocess<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, slot_params(L, g), L.cams, L.order,
             SplatOut { nullptr, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, nullptr, dkeys, dids,
                 cull ? nullptr : static_cast<uint2*>(recs), cull ? static_cast<ReachRec*>(recs) : nullptr });
         DGMR_HIP_CHECK(hipGetLastError());
@@ -485,13 +741,39 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
     }
     if (n_pairs > 0) {
         StageScope sc("batch.blend_backward", st, 40.0 * double(n_pairs));
+        {
+            // views by tile count, largest first, and the segments of rounds
+            // with the same number of views (k_tile_order)
+            std::vector<int4> sizes(n_views);
+            for (int v = 0; v < n_views; ++v) {
+                const int nt = (v + 1 < n_views ? vb[v + 1].tile_base : int(L.total_tiles)) - vb[v].tile_base;
+                int64_t stride = std::max<int64_t>(1, int64_t(double(nt) * 0.6180339887498949 + 0.5));
+                while (std::gcd(stride, int64_t(nt)) != 1)
+                    ++stride;
+                sizes[v] = make_int4(vb[v].tile_base, nt, int(stride), 0);
+            }
+            std::stable_sort(sizes.begin(), sizes.end(), [](const int4& a, const int4& b) { return a.y > b.y; });
+            std::vector<int4> segs;
+            int start = 0;
+            for (int c = n_views; c >= 1; --c) {
+                const int r0 = c < n_views ? sizes[c].y : 0, r1 = sizes[c - 1].y;
+                if (r1 > r0) {
+                    segs.push_back(make_int4(start, r0, c, 0));
+                    start += c * (r1 - r0);
+                }
+            }
+            DGMR_ARG_CHECK(start == L.total_tiles && !segs.empty(), "dgmr_splat_batch_backward: inconsistent view tiles");
+            DGMR_HIP_CHECK(hipMemcpyAsync(L.tile_sizes, sizes.data(), sizeof(int4) * n_views, hipMemcpyHostToDevice, st));
+            DGMR_HIP_CHECK(hipMemcpyAsync(L.tile_segs, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice, st));
+            k_tile_order<<<unsigned(ceil_div64(L.total_tiles, 256)), 256, 0, st>>>(int(L.total_tiles), L.tile_sizes, L.tile_segs, int(segs.size()),
+                L.tile_order);
+            DGMR_HIP_CHECK(hipGetLastError());
+        }
         DGMR_HIP_CHECK(hipMemsetAsync(L.queue, 0, sizeof(uint32_t), st));
         batch_blend(true, *tiling, st, L, n_views);
         DGMR_HIP_CHECK(hipGetLastError());
     }
     StageScope sc("batch.preprocess_backward", st, double(n) * 2.0 * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 40.0);
-    const SplatParams p { static_cast<const float*>(g->means), static_cast<const float*>(g->scales), static_cast<const float*>(g->rotations),
-        static_cast<const float*>(g->opacity), static_cast<const float*>(g->sh), g->sh_coeffs, g->sh_degree, float(g->scale_multiplier), g->activation };
     if (!every_group_has_views) {
         // a group with no view is written by no thread: clear the outputs first
         const size_t per = size_t(n) * n_groups;
@@ -501,8 +783,8 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
         DGMR_HIP_CHECK(hipMemsetAsync(grad_opacity, 0, per * sizeof(float), st));
         DGMR_HIP_CHECK(hipMemsetAsync(grad_sh, 0, per * 3 * g->sh_coeffs * sizeof(float), st));
     }
-    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_grad_views, p, L.cams, L.groups, L.tiles, L.conic_opacity,
-        L.clamped, L.screen_grads, grad_means, grad_scales, grad_rotations, grad_opacity, grad_sh);
+    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_grad_views, slot_params(L, g), L.cams, L.order, L.groups, L.tiles,
+        L.conic_opacity, L.clamped, L.screen_grads, grad_means, grad_scales, grad_rotations, grad_opacity, grad_sh);
     DGMR_HIP_CHECK(hipGetLastError());
     return DGMR_OK;
 }

@@ -195,28 +195,50 @@ struct SplatOut {
 SplatParams make_splat_params(const dgmr_gaussians* g);
 hipError_t launch_splat_preprocess(const dgmr_gaussians* g, const DevCamera<float>& cam, const SplatOut& out, hipStream_t stream);
 
-// SH coefficients of a block of Gaussians staged in LDS: the block's
-// contiguous [n_block, K, 3] slice is read with coalesced loads and each
-// Gaussian's row sits at a stride of 3K + 1 floats (odd, so the 64 lanes of a
-// wave reading their own rows hit distinct LDS banks).
+// SH coefficients of a block of Gaussians staged in LDS: row r of the block
+// is Gaussian ids[r], whose [K, 3] coefficients are contiguous and read by
+// consecutive lanes, eight loads in flight per thread (the rows may be
+// scattered over the scene).  Each row sits in LDS at a stride of 3K + 1
+// floats (odd, so the 64 lanes of a wave reading their own rows hit distinct
+// LDS banks).  The block has 256 threads.
 constexpr int kShStride = 49;
+constexpr int kStageInFlight = 8;
 
-__device__ __forceinline__ void stage_sh(float* s_sh, const float* sh, int64_t first, int count, int sh_coeffs)
+__device__ __forceinline__ void stage_sh(float* s_sh, const float* sh, const uint32_t* ids, int count, int sh_coeffs)
 {
     const int row = sh_coeffs * 3;
     const int total = count * row;
-    const float* src = sh + first * row;
-    for (int e = threadIdx.x; e < total; e += blockDim.x)
-        s_sh[(e / row) * kShStride + (e % row)] = src[e];
+    for (int base = 0; base < total; base += kStageInFlight * 256) {
+        float v[kStageInFlight];
+#pragma unroll
+        for (int u = 0; u < kStageInFlight; ++u) {
+            const int e = base + u * 256 + int(threadIdx.x);
+            if (e < total) {
+                const int r = e / row;
+                v[u] = sh[int64_t(ids[r]) * row + (e - r * row)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStageInFlight; ++u) {
+            const int e = base + u * 256 + int(threadIdx.x);
+            if (e < total) {
+                const int r = e / row;
+                s_sh[r * kShStride + (e - r * row)] = v[u];
+            }
+        }
+    }
 }
 
-__device__ __forceinline__ void unstage_sh(float* dst, const float* s_sh, int64_t first, int count, int sh_coeffs)
+// the staged rows written back to Gaussians ids[0..count): each row's [K, 3]
+// coefficients are contiguous, written by consecutive lanes
+__device__ __forceinline__ void unstage_sh(float* dst, const float* s_sh, const uint32_t* __restrict__ ids, int count, int sh_coeffs)
 {
     const int row = sh_coeffs * 3;
     const int total = count * row;
-    float* out = dst + first * row;
-    for (int e = threadIdx.x; e < total; e += blockDim.x)
-        out[e] = s_sh[(e / row) * kShStride + (e % row)];
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int r = e / row, k = e - r * row;
+        dst[int64_t(ids[r]) * row + k] = s_sh[r * kShStride + k];
+    }
 }
 
 // preprocess of Gaussian i for one camera (splat_forward.cu:71-118): cull

@@ -34,19 +34,30 @@ def _server(n_requests=12):
 
 
 def test_worker_failure_ends_run_with_the_error(host_only, monkeypatch):
+    # the first dispatch fails, whichever path (one request alone or a batch)
+    # it takes: how many dispatches the trace makes, and down which path,
+    # depends on thread timing (under load every request may have arrived
+    # before the first is scheduled, and all of them form one batch)
     calls = {"n": 0}
     lock = threading.Lock()
 
-    def flaky_single(self, sc, r):
+    def first_fails():
         with lock:
             calls["n"] += 1
             k = calls["n"]
-        if k == 3:
+        if k == 1:
             raise DgmrError(2, "dgmr_splat_forward", "pair capacity kept growing")
+
+    def flaky_single(self, sc, r):
+        first_fails()
         return [None], [None]
 
+    def flaky_batched(sp, sc, batch, n_bwd):
+        first_fails()
+        return ([None] * len(batch),) * 2
+
     monkeypatch.setattr(serving.ContinuousBatchingServer, "_single", flaky_single)
-    monkeypatch.setattr(serving.ContinuousBatchingServer, "_batched", staticmethod(lambda sp, sc, batch, n_bwd: ([None] * len(batch),) * 2))
+    monkeypatch.setattr(serving.ContinuousBatchingServer, "_batched", staticmethod(flaky_batched))
     srv, trace = _server()
     done = {}
 
