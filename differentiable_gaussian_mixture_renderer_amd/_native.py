@@ -55,7 +55,8 @@ class Camera(ctypes.Structure):
 
 
 class Tiling(ctypes.Structure):
-    _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32), ("batch", ctypes.c_int32), ("cull", ctypes.c_int32)]
+    _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32), ("batch", ctypes.c_int32), ("cull", ctypes.c_int32),
+                ("slot_order", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

@@ -29,10 +29,14 @@ def _cam_struct(cam, bg):
 class BatchSplatter:
     """Forward and backward of one scene over a list of cameras (scenes.Camera)."""
 
-    def __init__(self, tile=(16, 16), batch: int = 64, device="cuda:0", cull: bool = False):
+    def __init__(self, tile=(16, 16), batch: int = 64, device="cuda:0", cull: bool = False, spatial_order: bool = True):
         # cull: bin into the tiles where alpha can reach 1/255 only (same images and gradients, about a
         # fifth fewer pairs; over many small views the per-row reach arithmetic costs more than it saves)
+        # spatial_order: the Gaussians take their slots in the Morton order of their centres, so a wave's
+        # Gaussians are culled together in each view (dgmr_tiling.slot_order 0); in id order instead, a
+        # backward writes its gradient rows contiguously, which pays when it returns many gradient groups
         self.tiling = C.tiling_struct(tile, batch, cull)
+        self.spatial_order = spatial_order
         self.device = torch.device(device)
         self.capacity = 0
         self._ws = None
@@ -62,6 +66,15 @@ class BatchSplatter:
         self._views = self._view_array(self.cameras, self.bg, fbs, Ts)
         self._fbs, self._Ts = fbs, Ts
         return dev
+
+    @property
+    def spatial_order(self) -> bool:
+        return self.tiling.slot_order == 0
+
+    @spatial_order.setter
+    def spatial_order(self, on: bool):
+        # read by the forward and by the backward that follows it: change it between steps only
+        self.tiling.slot_order = 0 if on else 1
 
     def _launch_forward(self, dev):
         n = self._keep[2].shape[0]

@@ -23,7 +23,7 @@ bool valid_tiling(const dgmr_tiling* t)
         return false;
     const bool shape = (t->tile_w == 16 && t->tile_h == 16) || (t->tile_w == 8 && t->tile_h == 8) || (t->tile_w == 32 && t->tile_h == 8);
     const bool batch = t->batch == 64 || t->batch == 128 || t->batch == 256;
-    return shape && batch && (t->cull == 0 || t->cull == 1);
+    return shape && batch && (t->cull == 0 || t->cull == 1) && (t->slot_order == 0 || t->slot_order == 1);
 }
 
 uint32_t higher_msb(uint32_t n)

@@ -184,6 +184,14 @@ __global__ __launch_bounds__(256) void k_gather_params(int64_t n, SplatParams p,
     opacity[s] = p.opacity[i];
 }
 
+// slot s holds Gaussian s (slot_order 1)
+__global__ __launch_bounds__(256) void k_iota(int64_t n, uint32_t* __restrict__ ids)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        ids[i] = uint32_t(i);
+}
+
 // thread per slot s (Gaussian order[s]; p in slot order but for its SH rows,
 // read in place): the Gaussian's parameters are read once and projected into
 // every view
@@ -314,6 +322,19 @@ __global__ __launch_bounds__(256) void k_zero_screen_grads(int64_t n_floats, flo
         screen_grads[k] = 0.f;
 }
 
+// a block's per-Gaussian gradient rows staged in LDS: centre 3, scales 3,
+// rotation 4, opacity 1; columns [col, col + W) of row r go to
+// dst[ids[r] * W + k], consecutive lanes on consecutive floats
+constexpr int kSmallRow = 11;
+template <int W>
+__device__ __forceinline__ void unstage_rows(float* __restrict__ dst, const float* s_rows, int col, const uint32_t* ids, int count)
+{
+    for (int e = threadIdx.x; e < count * W; e += blockDim.x) {
+        const int r = e / W, k = e - r * W;
+        dst[int64_t(ids[r]) * W + k] = s_rows[r * kSmallRow + col + k];
+    }
+}
+
 // thread per slot s (Gaussian order[s]; p in slot order but for its SH rows,
 // read in place), summing its
 // adjoint over the views of each gradient group: group[v] is -1 (no
@@ -390,21 +411,32 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
             load_view(v + 1, nx);
         if (gv != cur) {
             if (cur >= 0) {
-                // views of group cur done: write its block; the SH gradient rows
-                // go through LDS so the block writes them as contiguous spans
+                // views of group cur done: write its block.  The slots' rows
+                // are scattered over the outputs (slot order is not Gaussian
+                // order), so they leave through LDS and consecutive lanes write
+                // consecutive floats of each row: a store instruction then
+                // touches a third as many cache lines as one row per lane
+                float g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f };
                 if (valid) {
-                    float g_s[3], g_q[4];
                     preprocess_backward_finish(s, q, pre, p, g_cov3, g_num, g_s, g_q);
                     activation_backward(p, i, g_s, g_q, &g_w);
-                    const int64_t o = int64_t(cur) * n + gid;
+                }
+                __syncthreads(); // s_out free
+                if (valid) {
+                    float* r = s_out + threadIdx.x * kSmallRow;
                     for (int a = 0; a < 3; ++a) {
-                        g_means[3 * o + a] = g_mu[a];
-                        g_scales[3 * o + a] = g_s[a];
+                        r[a] = g_mu[a];
+                        r[3 + a] = g_s[a];
                     }
                     for (int a = 0; a < 4; ++a)
-                        g_rotations[4 * o + a] = g_q[a];
-                    g_opacity[o] = g_w;
+                        r[6 + a] = g_q[a];
+                    r[10] = g_w;
                 }
+                __syncthreads();
+                unstage_rows<3>(g_means + int64_t(cur) * n * 3, s_out, 0, s_ids, count);
+                unstage_rows<3>(g_scales + int64_t(cur) * n * 3, s_out, 3, s_ids, count);
+                unstage_rows<4>(g_rotations + int64_t(cur) * n * 4, s_out, 6, s_ids, count);
+                unstage_rows<1>(g_opacity + int64_t(cur) * n, s_out, 10, s_ids, count);
                 float* dst = g_sh + int64_t(cur) * n * kc;
                 for (int half = 0; half < 2; ++half) {
                     const int lo = half * 128;
@@ -519,9 +551,11 @@ BatchLayout carve_batch(void* ws, int64_t n, const dgmr_batch_view* views, int n
 }
 
 // the Gaussians' parameters in slot order, as the forward copied them, and
-// their SH rows in place
-SplatParams slot_params(const BatchLayout& L, const dgmr_gaussians* g)
+// their SH rows in place (in id order: the parameters themselves)
+SplatParams slot_params(const BatchLayout& L, const dgmr_gaussians* g, const dgmr_tiling& t)
 {
+    if (t.slot_order == 1)
+        return make_splat_params(g);
     return SplatParams { L.slot_means, L.slot_scales, L.slot_rotations, L.slot_opacity, static_cast<const float*>(g->sh), g->sh_coeffs,
         g->sh_degree, float(g->scale_multiplier), g->activation };
 }
@@ -622,7 +656,10 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     uint32_t *dkeys, *dids;
     void* recs;
     binning_key_buffers(L.binning, m, cull, &dkeys, &dids, &recs);
-    if (n > 0) {
+    if (n > 0 && tiling->slot_order == 1) {
+        k_iota<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, L.order); // id order
+        DGMR_HIP_CHECK(hipGetLastError());
+    } else if (n > 0) {
         // the spatial order of the slots: bounding box, Morton keys, 24-bit sort
         StageScope sc("batch.order", st, 52.0 * double(n));
         const float* means = static_cast<const float*>(g->means);
@@ -639,7 +676,7 @@ int dgmr_splat_batch_forward(const dgmr_gaussians* g, const dgmr_batch_view* vie
     }
     if (n > 0) {
         StageScope sc("batch.preprocess", st, double(n) * (44.0 + 12.0 * g->sh_coeffs) + double(m) * 55.0);
-        k_batch_preprocess<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, slot_params(L, g), L.cams, L.order,
+        k_batch_preprocess<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_views, slot_params(L, g, *tiling), L.cams, L.order,
             SplatOut { nullptr, L.tiles, L.xy, L.conic_opacity, L.rgb, L.clamped, nullptr, dkeys, dids,
                 cull ? nullptr : static_cast<uint2*>(recs), cull ? static_cast<ReachRec*>(recs) : nullptr });
         DGMR_HIP_CHECK(hipGetLastError());
@@ -783,7 +820,7 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
         DGMR_HIP_CHECK(hipMemsetAsync(grad_opacity, 0, per * sizeof(float), st));
         DGMR_HIP_CHECK(hipMemsetAsync(grad_sh, 0, per * 3 * g->sh_coeffs * sizeof(float), st));
     }
-    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_grad_views, slot_params(L, g), L.cams, L.order, L.groups, L.tiles,
+    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_grad_views, slot_params(L, g, *tiling), L.cams, L.order, L.groups, L.tiles,
         L.conic_opacity, L.clamped, L.screen_grads, grad_means, grad_scales, grad_rotations, grad_opacity, grad_sh);
     DGMR_HIP_CHECK(hipGetLastError());
     return DGMR_OK;

@@ -538,7 +538,7 @@ int dgmr_raster_forward(const dgmr_gaussians* g, const dgmr_camera* cam, const d
     const int64_t n = g->n;
     DGMR_ARG_CHECK(workspace_bytes >= splat_workspace_bytes(n, bins->pair_capacity), "dgmr_raster_forward: workspace too small");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const dgmr_tiling tiling { kTile, kTile, 64, 0 }; // the reference's rectangles
+    const dgmr_tiling tiling { kTile, kTile, 64, 0, 0 }; // the reference's rectangles
     const DevCamera<float> dc = make_dev_camera<float>(*cam, tiling);
     const int64_t n_tiles = int64_t(dc.grid_x) * dc.grid_y;
     if (stats) {

@@ -90,6 +90,10 @@ typedef struct dgmr_tiling {
     int32_t tile_w, tile_h;   /* 8x8, 16x16 or 32x8 */
     int32_t batch;            /* Gaussians staged in LDS per round: 64, 128 or 256 */
     int32_t cull;             /* 0: the reference's rectangles; 1: tiles reaching alpha 1/255 */
+    int32_t slot_order;       /* batched views: 0 = Gaussians take slots in the Morton order of
+                                 their centres, 1 = in id order (fewer scattered gradient
+                                 writes when a backward returns many gradient groups);
+                                 the backward takes the forward's value; other paths ignore it */
 } dgmr_tiling;
 
 typedef struct dgmr_stats {
