@@ -343,7 +343,7 @@ __device__ __forceinline__ void unstage_rows(float* __restrict__ dst, const floa
 // views up to the last one with a group: the views after it are neither
 // loaded nor visited.
 __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, int n_views, SplatParams p, const DevCamera<float>* __restrict__ cams,
-    const uint32_t* __restrict__ order, const int32_t* __restrict__ group, const uint32_t* __restrict__ tiles,
+    const uint32_t* __restrict__ order, bool id_order, const int32_t* __restrict__ group, const uint32_t* __restrict__ tiles,
     const float4* __restrict__ conic_opacity, const uint8_t* __restrict__ clamped, const float* __restrict__ screen_grads,
     float* __restrict__ g_means, float* __restrict__ g_scales, float* __restrict__ g_rotations, float* __restrict__ g_opacity,
     float* __restrict__ g_sh)
@@ -411,32 +411,46 @@ __global__ __launch_bounds__(256) void k_batch_preprocess_backward(int64_t n, in
             load_view(v + 1, nx);
         if (gv != cur) {
             if (cur >= 0) {
-                // views of group cur done: write its block.  The slots' rows
-                // are scattered over the outputs (slot order is not Gaussian
-                // order), so they leave through LDS and consecutive lanes write
-                // consecutive floats of each row: a store instruction then
-                // touches a third as many cache lines as one row per lane
+                // views of group cur done: write its block.  In Morton slot
+                // order the slots' rows are scattered over the outputs, so they
+                // leave through LDS and consecutive lanes write consecutive
+                // floats of each row: a store instruction then touches a third
+                // as many cache lines as one row per lane.  In id order each
+                // thread writes its own rows, which the wave's lanes coalesce.
                 float g_s[3] = { 0.f, 0.f, 0.f }, g_q[4] = { 0.f, 0.f, 0.f, 0.f };
                 if (valid) {
                     preprocess_backward_finish(s, q, pre, p, g_cov3, g_num, g_s, g_q);
                     activation_backward(p, i, g_s, g_q, &g_w);
                 }
-                __syncthreads(); // s_out free
-                if (valid) {
-                    float* r = s_out + threadIdx.x * kSmallRow;
-                    for (int a = 0; a < 3; ++a) {
-                        r[a] = g_mu[a];
-                        r[3 + a] = g_s[a];
+                if (id_order) {
+                    if (valid) {
+                        const int64_t o = int64_t(cur) * n + i;
+                        for (int a = 0; a < 3; ++a) {
+                            g_means[3 * o + a] = g_mu[a];
+                            g_scales[3 * o + a] = g_s[a];
+                        }
+                        for (int a = 0; a < 4; ++a)
+                            g_rotations[4 * o + a] = g_q[a];
+                        g_opacity[o] = g_w;
                     }
-                    for (int a = 0; a < 4; ++a)
-                        r[6 + a] = g_q[a];
-                    r[10] = g_w;
+                } else {
+                    __syncthreads(); // s_out free
+                    if (valid) {
+                        float* r = s_out + threadIdx.x * kSmallRow;
+                        for (int a = 0; a < 3; ++a) {
+                            r[a] = g_mu[a];
+                            r[3 + a] = g_s[a];
+                        }
+                        for (int a = 0; a < 4; ++a)
+                            r[6 + a] = g_q[a];
+                        r[10] = g_w;
+                    }
+                    __syncthreads();
+                    unstage_rows<3>(g_means + int64_t(cur) * n * 3, s_out, 0, s_ids, count);
+                    unstage_rows<3>(g_scales + int64_t(cur) * n * 3, s_out, 3, s_ids, count);
+                    unstage_rows<4>(g_rotations + int64_t(cur) * n * 4, s_out, 6, s_ids, count);
+                    unstage_rows<1>(g_opacity + int64_t(cur) * n, s_out, 10, s_ids, count);
                 }
-                __syncthreads();
-                unstage_rows<3>(g_means + int64_t(cur) * n * 3, s_out, 0, s_ids, count);
-                unstage_rows<3>(g_scales + int64_t(cur) * n * 3, s_out, 3, s_ids, count);
-                unstage_rows<4>(g_rotations + int64_t(cur) * n * 4, s_out, 6, s_ids, count);
-                unstage_rows<1>(g_opacity + int64_t(cur) * n, s_out, 10, s_ids, count);
                 float* dst = g_sh + int64_t(cur) * n * kc;
                 for (int half = 0; half < 2; ++half) {
                     const int lo = half * 128;
@@ -820,7 +834,7 @@ int dgmr_splat_batch_backward(const dgmr_gaussians* g, const dgmr_batch_view* vi
         DGMR_HIP_CHECK(hipMemsetAsync(grad_opacity, 0, per * sizeof(float), st));
         DGMR_HIP_CHECK(hipMemsetAsync(grad_sh, 0, per * 3 * g->sh_coeffs * sizeof(float), st));
     }
-    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_grad_views, slot_params(L, g, *tiling), L.cams, L.order, L.groups, L.tiles,
+    k_batch_preprocess_backward<<<unsigned(ceil_div64(n, 256)), 256, 0, st>>>(n, n_grad_views, slot_params(L, g, *tiling), L.cams, L.order, tiling->slot_order == 1, L.groups, L.tiles,
         L.conic_opacity, L.clamped, L.screen_grads, grad_means, grad_scales, grad_rotations, grad_opacity, grad_sh);
     DGMR_HIP_CHECK(hipGetLastError());
     return DGMR_OK;
