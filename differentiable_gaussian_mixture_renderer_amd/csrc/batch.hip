@@ -4,8 +4,8 @@
 // blend launch (forward and backward).
 //
 // Layout: the Gaussians take slots in the Morton order of their centres
-// (slot s holds Gaussian order[s]), and per-view geometry is stored at entry
-// v * n + s, so the entry ids in the pair list index every array directly;
+// (slot s holds Gaussian order[s]; in id order when dgmr_tiling.slot_order is
+// 1), and per-view geometry is stored at entry v * n + s, so the entry ids in the pair list index every array directly;
 // view v's tiles occupy a contiguous range of the global tile index space.
 // The blend runs as a persistent grid whose workgroups take the next tile of
 // any view from a device counter, so small and large views share the chip
