@@ -117,6 +117,35 @@ def test_tight_culling_flag_validation_and_workspace():
     assert b"bad tiling" in lib.dgmr_last_error()
 
 
+def test_slot_order_validation_and_batch_workspace():
+    """dgmr_tiling.slot_order takes 0 (Morton slots) or 1 (id order); the
+    batched workspace holds the order, its sort buffers and the parameters
+    copied into slot order in either case, so one size serves both."""
+    from differentiable_gaussian_mixture_renderer_amd import _native as N
+    lib = N.load()
+    views = (N.BatchView * 2)()
+    for v in views:
+        v.cam.width, v.cam.height = 320, 240
+    n, cap = 50_000, 1_000_000
+    sizes = {}
+    for order in (0, 1):
+        t = N.Tiling(16, 16, 64, 0, order)
+        size = ctypes.c_size_t(0)
+        assert lib.dgmr_batch_workspace_size(n, views, 2, ctypes.byref(t), cap, ctypes.byref(size)) == N.DGMR_OK
+        sizes[order] = size.value
+    assert sizes[0] == sizes[1]
+    # order, three sort buffers and centre, scales, rotation, opacity in slot order
+    t_small = N.Tiling(16, 16, 64, 0, 0)
+    small = ctypes.c_size_t(0)
+    assert lib.dgmr_batch_workspace_size(0, views, 2, ctypes.byref(t_small), cap, ctypes.byref(small)) == N.DGMR_OK
+    assert sizes[0] - small.value >= n * (4 * 4 + 11 * 4)
+    for bad_order in (2, -1):
+        bad = N.Tiling(16, 16, 64, 0, bad_order)
+        size = ctypes.c_size_t(0)
+        assert lib.dgmr_batch_workspace_size(n, views, 2, ctypes.byref(bad), cap, ctypes.byref(size)) == N.DGMR_ERR_ARGUMENT
+        assert lib.dgmr_workspace_size(n, 256, 256, ctypes.byref(bad), cap, ctypes.byref(size)) == N.DGMR_ERR_ARGUMENT
+
+
 def test_l1_many_argument_checks_without_gpu():
     """losses.l1_grad_many rejects mismatched shapes and dtypes before any
     launch; the C entry point rejects null and misaligned arrays."""
